@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Render throughput benchmark for libquasi_hip.so on one GPU.
+
+Workload (default): the 1M-triangle soup (centroids U[-1,1]^3, sizes
+log-uniform in [1e-3, 5e-2], albedo U[0,1], four quad lights of radiance 15),
+binned-SAH BVH2 (32 bins, leaf <= 4), 1920x1080, 64 spp, depth 8, Russian
+roulette from bounce 3, next-event estimation, FP32.
+
+One step = one complete 64-spp frame from a cleared accumulation, rendered
+through the plugin C ABI (Q_plugin_render) in launches of --spp-per-launch
+samples.  Rays = BVH queries (camera + extension + shadow, misses included),
+counted on the device.  The scene and BVH are built and uploaded before the
+timed region (build time reported separately).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s (+ % HBM roofline) at 1080p, batched vs plain; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(mesh, args, seconds: float):
+    """The scalar oracle (FP32, std::thread pool) on this host's cores over a
+    bounded, evenly spread subset of the same workload's pixels."""
+    from oracle import oracle as O
+    threads = min(16, O.hardware_concurrency())
+    s = O.Scene(mesh)
+    common = dict(max_depth=args.depth, rr_start=3, integrator=1, rng_mode=1, seed=0, precision=32,
+                  threads=threads)
+    npix = args.width * args.height
+    stride = 997
+    r = s.render(args.width, args.height, 1, pixel_stride=stride, **common)
+    per_path = r.counts.seconds / max(1, r.counts.camera_rays)
+    # choose the pixel stride so that one pass over samples 0..3 takes about `seconds`
+    target_paths = max(1000, seconds / max(per_path, 1e-9) / 4)
+    stride = max(1, int(npix / target_paths))
+    r = s.render(args.width, args.height, 4, pixel_stride=stride, **common)
+    c = r.counts
+    rays = c.camera_rays + c.extension_rays + c.shadow_rays
+    return dict(value=rays / c.seconds / 1e6, unit="Mrays/s", cores=int(c.threads), kind="port",
+                sample=(f"oracle FP32, {args.width}x{args.height} soup, samples 0..3 of every {stride}th pixel "
+                        f"({c.camera_rays} paths, {rays} rays, {c.seconds:.1f} s, BVH build excluded)"))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--ntri", type=int, default=1_000_000)
+    ap.add_argument("--spp-per-launch", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "multi-GPU sharding of this path is designed but not built", file=sys.stderr)
+        sys.exit(2)
+    if args.spp % args.spp_per_launch:
+        sys.exit("--spp must be a multiple of --spp-per-launch")
+
+    from quasi_amd import abi, scenes
+    from quasi_amd.plugin import HipRenderer, make_camera
+
+    t0 = time.perf_counter()
+    mesh = scenes.soup(args.ntri, seed=1)
+    log(f"scene: {mesh.num_triangles} triangles generated in {time.perf_counter() - t0:.1f} s")
+    alg = json.loads((ROOT / "tests" / "golden" / "alg_bytes.json").read_text())["soup_1m_1080p"]
+
+    W, H = args.width, args.height
+    launches_per_step = args.spp // args.spp_per_launch
+    with HipRenderer(W, H, device=0) as r:
+        t0 = time.perf_counter()
+        r.set_scene(mesh)
+        build_s = time.perf_counter() - t0
+        c = mesh.camera
+        r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+        r.set_params(spp_per_render=args.spp_per_launch, max_depth=args.depth, rr_start=3, rng_mode=1,
+                     integrator=1, seed=0, flags=abi.Q_FLAG_TIME_KERNELS)
+        st = r.stats()
+        log(f"BVH: {st.bvh_nodes} nodes, depth {st.bvh_depth}, build+upload {st.bvh_build_ms:.0f} ms "
+            f"(set_scene {build_s:.2f} s)")
+
+        def step():
+            for i in range(launches_per_step):
+                f = r.frame(camera_dirty=(i == 0))
+                r.plugin.render(f)
+
+        for i in range(args.warmup):
+            step()
+            r.synchronize()
+            log(f"warmup step {i + 1}/{args.warmup} done")
+        r.reset_stats()
+        r.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+        r.synchronize()
+        elapsed = time.perf_counter() - t0
+        st = r.stats()
+        log(f"timed {args.steps} steps in {elapsed:.3f} s")
+
+    rays = {"camera": st.rays_camera, "extension": st.rays_extension, "shadow": st.rays_shadow}
+    total_rays = sum(rays.values())
+    value = total_rays / elapsed / 1e6
+    k = st.kernels[0]
+    launches = int(k.launches)
+    avg_ms = k.total_ms / max(1, launches)
+    bytes_rays = sum(rays[t] * alg[t]["bytes_per_ray"] for t in rays)
+    bytes_fb = 32.0 * W * H * args.spp * args.steps
+    per_launch = (bytes_rays + bytes_fb) / max(1, launches)
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "hbm_traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("megakernel_soup_1m_1080p", {}).get("bytes_per_launch")
+        except (ValueError, AttributeError):
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain wavefront-config soup: {mesh.num_triangles} triangles, {W}x{H}, "
+                               f"{args.spp} spp, depth {args.depth}, RR from 3, NEE",
+                   "engine": "megakernel", "spp_per_launch": args.spp_per_launch, "bvh": "binned-SAH BVH2"},
+        "rays": rays,
+        "rays_per_path": round(total_rays / (W * H * args.spp * args.steps), 4),
+        "bvh_build_ms": round(st.bvh_build_ms, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "megakernel", "avg_launch_ms": round(avg_ms, 3), "launches": launches,
+                     "alg_bytes_per_launch": round(per_launch)},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(mesh, args, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

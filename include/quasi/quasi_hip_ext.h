@@ -1,0 +1,142 @@
+/* quasi_hip_ext.h -- optional extension exports of libquasi_hip.so.
+ *
+ * None of these exist in the reference.  A host discovers them with dlsym
+ * exactly as the reference loader discovers its optional v2/v3 symbols
+ * (loader.hpp:114-128), so the ABI version stays 3 and an unmodified host that
+ * never looks them up still works.
+ *
+ * Conventions (all functions):
+ *   - every struct starts with uint32_t struct_size = sizeof(struct);
+ *   - int32_t return: Q_EXT_OK (0) or a negative Q_EXT_ERR_* code, with a message
+ *     sent through Q_plugin_context.log;
+ *   - inputs are caller-owned and copied before the call returns;
+ *   - no C++ exception and no HIP error crosses the ABI.
+ */
+#ifndef QUASI_HIP_EXT_H
+#define QUASI_HIP_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "plugin_interface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define Q_EXT_OK               0
+#define Q_EXT_ERR_ARGUMENT    -1
+#define Q_EXT_ERR_STATE       -2
+#define Q_EXT_ERR_DEVICE      -3
+#define Q_EXT_ERR_OUT_OF_MEMORY -4
+#define Q_EXT_ERR_UNSUPPORTED -5
+
+/* Random stream seeding.
+ *   Q_RNG_REFERENCE: state = pcg_hash(x + y*1920 + sample*1920*1080) (plugin.mm:295)
+ *   Q_RNG_HASHED:    state = pcg_hash((sample*W*H + y*W + x) ^ pcg_hash(seed))
+ * The stream itself is always state = pcg_hash(state), u = float(state)/2^32
+ * (plugin.mm:98-107). */
+#define Q_RNG_REFERENCE 0u
+#define Q_RNG_HASHED    1u
+
+/* Light transport.
+ *   Q_INTEGRATOR_REFERENCE: the reference bounce loop (plugin.mm:223-276): emission
+ *     collected when a BSDF-sampled ray hits an emitter, no next-event estimation.
+ *   Q_INTEGRATOR_NEE: emission seen directly only by camera rays; at every
+ *     non-emissive hit before the last segment one light triangle is sampled
+ *     (probability proportional to area x summed radiance) and a shadow ray is
+ *     traced.  Same expectation as the reference loop at the same max_depth. */
+#define Q_INTEGRATOR_REFERENCE 0u
+#define Q_INTEGRATOR_NEE       1u
+
+/* Execution engine for the sample passes. */
+#define Q_ENGINE_MEGAKERNEL 0u  /* one thread per (pixel, sample) path, whole path in one kernel */
+#define Q_ENGINE_WAVEFRONT  1u  /* raygen / extend / shade / shadow kernels over compacted queues */
+
+/* Q_render_params.flags */
+#define Q_FLAG_RECORD_IDS    1u  /* record primary triangle ids of sample id_sample */
+#define Q_FLAG_TIME_KERNELS  2u  /* bracket every kernel launch with hipEvents */
+
+/* Indexed triangle mesh with per-triangle material.  Triangles whose largest
+ * emission channel exceeds 0.1 are emitters (plugin.mm:243, :255). */
+typedef struct Q_scene_desc {
+    uint32_t struct_size;
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    uint32_t reserved;
+    const float* vertices;    /* xyz, 3 * num_vertices */
+    const uint32_t* indices;  /* 3 * num_triangles */
+    const float* albedo;      /* rgb per triangle, 3 * num_triangles */
+    const float* emission;    /* rgb radiance per triangle, 3 * num_triangles */
+} Q_scene_desc;
+
+/* Defaults after Q_plugin_create mirror the reference: spp_per_render 1,
+ * max_depth 5 (plugin.mm:22), rr_start 3 (plugin.mm:264, "bounce > 2"),
+ * seed 0, Q_RNG_REFERENCE, Q_INTEGRATOR_REFERENCE, Q_ENGINE_MEGAKERNEL. */
+typedef struct Q_render_params {
+    uint32_t struct_size;
+    uint32_t spp_per_render;   /* samples per pixel added by one Q_plugin_render call */
+    uint32_t max_depth;        /* closest-hit traces per path, 1..16 */
+    uint32_t rr_start;         /* Russian roulette when bounce index >= rr_start */
+    uint32_t seed;
+    uint32_t rng_mode;         /* Q_RNG_* */
+    uint32_t integrator;       /* Q_INTEGRATOR_* */
+    uint32_t engine;           /* Q_ENGINE_* */
+    uint32_t bvh_width;        /* 2 (only value accepted so far) */
+    uint32_t samples_per_pass; /* wavefront: samples per pixel in flight at once (0 = auto) */
+    uint32_t block_size;       /* threads per block for the path kernels (0 = auto) */
+    uint32_t flags;            /* Q_FLAG_* */
+    uint32_t id_sample;        /* sample index whose primary ids are recorded */
+} Q_render_params;
+
+#define Q_STATS_MAX_KERNELS 16
+
+typedef struct Q_kernel_stat {
+    char name[48];
+    uint64_t launches;
+    double total_ms;           /* summed hipEvent time (Q_FLAG_TIME_KERNELS only) */
+} Q_kernel_stat;
+
+typedef struct Q_render_stats {
+    uint32_t struct_size;
+    uint32_t width;
+    uint32_t height;
+    uint32_t sample_count;     /* samples per pixel in the accumulation */
+    uint32_t num_triangles;
+    uint32_t num_lights;
+    uint32_t bvh_nodes;
+    uint32_t bvh_depth;
+    uint64_t rays_camera;      /* BVH queries since the last Q_plugin_reset_stats */
+    uint64_t rays_extension;
+    uint64_t rays_shadow;
+    double bvh_build_ms;       /* host wall time of the last BVH build + upload */
+    double render_ms;          /* hipEvent time of all render calls since the reset */
+    uint32_t num_kernels;
+    uint32_t reserved;
+    Q_kernel_stat kernels[Q_STATS_MAX_KERNELS];
+} Q_render_stats;
+
+/* Replaces the hard-coded scene of plugin.mm:584 (cornell_box.hpp:95-134) with a
+ * caller mesh; builds the BVH and resets the accumulation. */
+int32_t Q_plugin_set_scene(Q_plugin_handle* handle, const Q_scene_desc* scene);
+
+/* Reads / replaces the render parameters; set_params resets the accumulation. */
+int32_t Q_plugin_get_params(Q_plugin_handle* handle, Q_render_params* out);
+int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_params* params);
+
+/* Primary-hit triangle index (input order; 0xFFFFFFFF for a miss) of sample
+ * id_sample for every pixel, row-major top-to-bottom.  Synchronises. */
+int32_t Q_plugin_readback_ids(Q_plugin_handle* handle, uint32_t* out, uint64_t count);
+
+/* Synchronises the plugin's streams, then fills *out. */
+int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out);
+int32_t Q_plugin_reset_stats(Q_plugin_handle* handle);
+
+/* Waits until every queued render has finished. */
+int32_t Q_plugin_synchronize(Q_plugin_handle* handle);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* QUASI_HIP_EXT_H */

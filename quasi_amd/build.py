@@ -1,0 +1,100 @@
+"""Builds the native parts in-tree.
+
+- ``quasi_amd/libquasi_hip.so``: the plugin (HIP kernels for gfx950 + the
+  Q_plugin_* C ABI), compiled with hipcc.
+- ``quasi_amd/bin/quasi_host``: the headless C++ host that dlopens a plugin
+  exactly like the reference host (src/quasi/host/main.cpp).
+
+The CPU oracle used by the tests is built separately by ``oracle/Makefile``.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+HOST = PKG / "host"
+BUILD = ROOT / "_build"
+PLUGIN_SO = PKG / "libquasi_hip.so"
+HOST_BIN = PKG / "bin" / "quasi_host"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+# Device code: gfx950 only, packed-FP32 instruction selection disabled.
+DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-fvisibility=hidden",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOSTONLY = ["-x", "c++", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+
+PLUGIN_SOURCES = [
+    ("kernels_mega.hip", DEVICE),
+    ("kernels_util.hip", DEVICE),
+    ("plugin.cpp", HOSTONLY),
+    ("bvh_build.cpp", HOSTONLY),
+    ("scene_cornell.cpp", HOSTONLY),
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(str(c) for c in cmd), flush=True)
+    r = subprocess.run([str(c) for c in cmd], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(map(str, cmd))}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build_plugin(verbose: bool = False, jobs: int = 8) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    objs = []
+    cmds = []
+    for src, flags in PLUGIN_SOURCES:
+        obj = BUILD / (src + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC, *COMMON, *flags, "-I", ROOT / "include", "-c", CSRC / src, "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    tmp = PLUGIN_SO.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"], verbose)
+    os.replace(tmp, PLUGIN_SO)
+    return PLUGIN_SO
+
+
+def build_host(verbose: bool = False) -> Path | None:
+    main = HOST / "main.cpp"
+    if not main.exists():
+        return None
+    HOST_BIN.parent.mkdir(exist_ok=True)
+    srcs = sorted(str(p) for p in HOST.glob("*.cpp"))
+    tmp = HOST_BIN.with_suffix(".tmp")
+    _run([HIPCC, "-O2", "-std=c++20", "-Wall", "-Wextra", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+          "-I", ROOT / "include", "-I", HOST, *srcs, "-o", tmp, "-ldl", "-lz", "-pthread"], verbose)
+    os.replace(tmp, HOST_BIN)
+    return HOST_BIN
+
+
+def build_oracle(verbose: bool = False) -> None:
+    """The checker used by tests (oracle/liboracle.so and, when the reference
+    sources are present, oracle/_ref)."""
+    _run(["make", "-s", "-C", ROOT / "oracle", "-B", "liboracle.so"], verbose)
+    if Path("/root/reference/src/quasi/scene/cornell_box.hpp").exists():
+        _run(["make", "-s", "-C", ROOT / "oracle", "-B", "ref"], verbose)
+
+
+def build_all(verbose: bool = False) -> None:
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    build_plugin(verbose)
+    build_host(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-v" in sys.argv)
+    print("built", PLUGIN_SO)

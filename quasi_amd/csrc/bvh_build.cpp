@@ -1,0 +1,328 @@
+// bvh_build.cpp -- parallel binned-SAH build (see bvh_build.hpp).
+//
+// Work is a set of independent (node, prim range) tasks on a shared queue;
+// each task bins its range on all three axes in one pass, picks the cheapest
+// of the 31 x 3 candidate planes, partitions in place and pushes both halves.
+// Node slots come from an atomic counter, so the temporary tree depends on
+// scheduling; the final pair-node layout is produced by a depth-first walk
+// from the root that orders children left before right, which makes the
+// output independent of the thread count.
+#include "bvh_build.hpp"
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+namespace quasi_hip {
+namespace {
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    double area() const {
+        double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+        if (!(dx >= 0) || !(dy >= 0) || !(dz >= 0)) return 0.0;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct TmpNode {
+    Box box;
+    uint32_t left = 0, right = 0;   // children (internal)
+    uint32_t first = 0, count = 0;  // leaf range in prims (count > 0)
+};
+
+struct Builder {
+    const float* p0;
+    const float* p1;
+    const float* p2;
+    uint32_t ntri;
+    std::vector<float> tlo, thi, cen;   // 3 per triangle
+    std::vector<uint32_t> prims;
+    std::vector<TmpNode> nodes;
+    std::atomic<uint32_t> next_node{0};
+
+    struct Task { uint32_t node, begin, end; };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Task> queue;
+    uint32_t pending = 0;  // queued + running tasks
+
+    void prepare(unsigned threads) {
+        tlo.resize(3 * size_t(ntri));
+        thi.resize(3 * size_t(ntri));
+        cen.resize(3 * size_t(ntri));
+        prims.resize(ntri);
+        auto work = [&](size_t b, size_t e) {
+            for (size_t t = b; t < e; ++t) {
+                for (int k = 0; k < 3; ++k) {
+                    float a = p0[3 * t + k], c = p1[3 * t + k], d = p2[3 * t + k];
+                    float lo = std::min(a, std::min(c, d)), hi = std::max(a, std::max(c, d));
+                    tlo[3 * t + k] = lo;
+                    thi[3 * t + k] = hi;
+                    cen[3 * t + k] = 0.5f * (lo + hi);
+                }
+                prims[t] = uint32_t(t);
+            }
+        };
+        run_parallel(ntri, threads, work);
+        nodes.resize(std::max<size_t>(1, 2 * size_t(ntri)));
+    }
+
+    template <class F>
+    static void run_parallel(size_t n, unsigned threads, F&& f) {
+        if (threads <= 1 || n < 65536) { f(0, n); return; }
+        std::vector<std::thread> pool;
+        size_t chunk = (n + threads - 1) / threads;
+        for (unsigned i = 0; i < threads; ++i) {
+            size_t b = i * chunk, e = std::min(n, b + chunk);
+            if (b >= e) break;
+            pool.emplace_back([&f, b, e] { f(b, e); });
+        }
+        for (auto& t : pool) t.join();
+    }
+
+    void process(const Task& task, std::vector<Task>& out) {
+        TmpNode& nd = nodes[task.node];
+        const uint32_t n = task.end - task.begin;
+        Box box, cbox;
+        for (uint32_t i = task.begin; i < task.end; ++i) {
+            uint32_t p = prims[i];
+            for (int k = 0; k < 3; ++k) {
+                box.lo[k] = std::min(box.lo[k], tlo[3 * size_t(p) + k]);
+                box.hi[k] = std::max(box.hi[k], thi[3 * size_t(p) + k]);
+                cbox.lo[k] = std::min(cbox.lo[k], cen[3 * size_t(p) + k]);
+                cbox.hi[k] = std::max(cbox.hi[k], cen[3 * size_t(p) + k]);
+            }
+        }
+        nd.box = box;
+        constexpr int B = kSahBins;
+        int best_axis = -1, best_split = -1;
+        double best_cost = INFINITY;
+        float scale[3] = {0, 0, 0};
+        if (n > 1) {
+            Box bins[3][B];
+            uint32_t cnt[3][B] = {};
+            for (int k = 0; k < 3; ++k) {
+                float ext = cbox.hi[k] - cbox.lo[k];
+                scale[k] = ext > 0.f ? float(B) / ext : 0.f;
+            }
+            for (uint32_t i = task.begin; i < task.end; ++i) {
+                uint32_t p = prims[i];
+                Box tb;
+                for (int k = 0; k < 3; ++k) { tb.lo[k] = tlo[3 * size_t(p) + k]; tb.hi[k] = thi[3 * size_t(p) + k]; }
+                for (int k = 0; k < 3; ++k) {
+                    if (scale[k] == 0.f) continue;
+                    int b = int((cen[3 * size_t(p) + k] - cbox.lo[k]) * scale[k]);
+                    b = std::min(std::max(b, 0), B - 1);
+                    cnt[k][b]++;
+                    bins[k][b].grow(tb);
+                }
+            }
+            const double inv_parent = 1.0 / std::max(box.area(), 1e-30);
+            for (int k = 0; k < 3; ++k) {
+                if (scale[k] == 0.f) continue;
+                double rarea[B];
+                uint32_t rcnt[B];
+                Box acc;
+                uint32_t run = 0;
+                for (int b = B - 1; b >= 1; --b) {
+                    acc.grow(bins[k][b]);
+                    run += cnt[k][b];
+                    rarea[b] = acc.area();
+                    rcnt[b] = run;
+                }
+                Box lacc;
+                uint32_t lrun = 0;
+                for (int s = 1; s < B; ++s) {
+                    lacc.grow(bins[k][s - 1]);
+                    lrun += cnt[k][s - 1];
+                    if (lrun == 0 || rcnt[s] == 0) continue;
+                    double cost = 1.0 + (lacc.area() * lrun + rarea[s] * rcnt[s]) * inv_parent;
+                    if (cost < best_cost) { best_cost = cost; best_axis = k; best_split = s; }
+                }
+            }
+        }
+        bool leaf = (n <= 1) || (n <= uint32_t(kLeafMax) && (best_axis < 0 || double(n) <= best_cost));
+        if (leaf) {
+            nd.first = task.begin;
+            nd.count = n;
+            return;
+        }
+        uint32_t mid;
+        if (best_axis < 0) {
+            mid = task.begin + n / 2;
+        } else {
+            const int k = best_axis;
+            const float lo = cbox.lo[k], sc = scale[k];
+            auto* m = std::partition(prims.data() + task.begin, prims.data() + task.end, [&](uint32_t p) {
+                int b = int((cen[3 * size_t(p) + k] - lo) * sc);
+                b = std::min(std::max(b, 0), B - 1);
+                return b < best_split;
+            });
+            mid = uint32_t(m - prims.data());
+            if (mid == task.begin || mid == task.end) mid = task.begin + n / 2;
+        }
+        uint32_t l = next_node.fetch_add(2);
+        nd.left = l;
+        nd.right = l + 1;
+        nd.count = 0;
+        out.push_back({l, task.begin, mid});
+        out.push_back({l + 1, mid, task.end});
+    }
+
+    void worker() {
+        std::vector<Task> local_out;
+        std::vector<Task> local_stack;
+        for (;;) {
+            Task t;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !queue.empty() || pending == 0; });
+                if (queue.empty()) return;
+                t = queue.front();
+                queue.pop_front();
+            }
+            // Small subtrees are finished locally without touching the queue.
+            local_stack.clear();
+            local_stack.push_back(t);
+            uint32_t done = 0;
+            while (!local_stack.empty()) {
+                Task cur = local_stack.back();
+                local_stack.pop_back();
+                local_out.clear();
+                process(cur, local_out);
+                done++;
+                for (const Task& c : local_out) {
+                    if (c.end - c.begin > 32768) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        queue.push_back(c);
+                        pending++;
+                        cv.notify_one();
+                    } else {
+                        local_stack.push_back(c);
+                    }
+                }
+            }
+            (void)done;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                pending--;
+                if (pending == 0) cv.notify_all();
+            }
+        }
+    }
+
+    void build(unsigned threads) {
+        next_node = 1;
+        queue.push_back({0, 0, ntri});
+        pending = 1;
+        unsigned nth = std::max(1u, threads);
+        std::vector<std::thread> pool;
+        for (unsigned i = 1; i < nth; ++i) pool.emplace_back([this] { worker(); });
+        worker();
+        for (auto& t : pool) t.join();
+    }
+};
+
+}  // namespace
+
+BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint32_t ntri,
+                         unsigned threads) {
+    BvhBuildResult res;
+    if (ntri == 0) return res;
+    if (threads == 0) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    Builder b;
+    b.p0 = p0; b.p1 = p1; b.p2 = p2; b.ntri = ntri;
+    b.prepare(threads);
+    b.build(threads);
+
+    // Depth-first conversion to pair nodes.  A root that is itself a leaf gets
+    // a pair node whose right child is an empty box (never entered).
+    const TmpNode& root = b.nodes[0];
+    res.order.reserve(ntri);
+    std::vector<float>& out = res.nodes;
+    auto encode_leaf = [&](const TmpNode& lf) -> int32_t {
+        uint32_t first = uint32_t(res.order.size());
+        for (uint32_t i = 0; i < lf.count; ++i) res.order.push_back(b.prims[lf.first + i]);
+        res.num_leaves++;
+        return ~int32_t((first << 2) | (lf.count - 1));
+    };
+    auto write_node = [&](uint32_t idx, const Box& L, const Box& R, int32_t cl, int32_t cr) {
+        float* n = &out[16 * size_t(idx)];
+        n[0] = L.lo[0]; n[1] = L.hi[0]; n[2] = L.lo[1]; n[3] = L.hi[1];
+        n[4] = R.lo[0]; n[5] = R.hi[0]; n[6] = R.lo[1]; n[7] = R.hi[1];
+        n[8] = L.lo[2]; n[9] = L.hi[2]; n[10] = R.lo[2]; n[11] = R.hi[2];
+        std::memcpy(&n[12], &cl, 4);
+        std::memcpy(&n[13], &cr, 4);
+        n[14] = 0.f; n[15] = 0.f;
+    };
+    if (root.count > 0) {
+        out.assign(16, 0.f);
+        Box empty;
+        int32_t cl = encode_leaf(root);
+        int32_t cr = ~int32_t(0);  // never entered: its box is empty
+        write_node(0, root.box, empty, cl, cr);
+        res.num_nodes = 1;
+        res.depth = 1;
+        res.sah_cost = 1.0 + root.count;
+        return res;
+    }
+    // Count internal nodes for allocation.
+    uint32_t internal = 0;
+    {
+        std::vector<uint32_t> st{0};
+        while (!st.empty()) {
+            uint32_t i = st.back(); st.pop_back();
+            if (b.nodes[i].count) continue;
+            internal++;
+            st.push_back(b.nodes[i].right);
+            st.push_back(b.nodes[i].left);
+        }
+    }
+    out.assign(16 * size_t(internal), 0.f);
+    // Preorder walk, left subtree first: a node's output index is assigned when
+    // it is popped, and the parent's child slot is patched at that moment.
+    struct Walk { uint32_t tmp; int64_t parent; uint32_t side; uint32_t depth; };
+    uint32_t next_out = 0;
+    double root_area = std::max(root.box.area(), 1e-30);
+    double cost = 0;
+    std::vector<Walk> st{{0, -1, 0, 1}};
+    while (!st.empty()) {
+        Walk it = st.back(); st.pop_back();
+        const uint32_t idx = next_out++;
+        if (it.parent >= 0) {
+            int32_t c = int32_t(idx);
+            std::memcpy(&out[16 * size_t(it.parent) + 12 + it.side], &c, 4);
+        }
+        const TmpNode& nd = b.nodes[it.tmp];
+        res.depth = std::max(res.depth, it.depth);
+        cost += nd.box.area() / root_area;
+        const TmpNode& L = b.nodes[nd.left];
+        const TmpNode& R = b.nodes[nd.right];
+        int32_t cl = 0, cr = 0;
+        if (L.count) { cl = encode_leaf(L); cost += L.box.area() / root_area * L.count; }
+        if (R.count) { cr = encode_leaf(R); cost += R.box.area() / root_area * R.count; }
+        write_node(idx, L.box, R.box, cl, cr);
+        if (!R.count) st.push_back({nd.right, int64_t(idx), 1, it.depth + 1});
+        if (!L.count) st.push_back({nd.left, int64_t(idx), 0, it.depth + 1});
+    }
+    res.num_nodes = internal;
+    res.sah_cost = cost;
+    return res;
+}
+
+}  // namespace quasi_hip

@@ -1,0 +1,714 @@
+// plugin.cpp -- libquasi_hip.so: the Quasi plugin C ABI on HIP (gfx950).
+//
+// Replaces the host side of the reference backend, backends/metal/plugin.mm:
+// the 10 exported symbols (plugin.mm:543-902, declared in
+// src/quasi/plugin/plugin_interface.hpp:108-148) plus the optional extension
+// exports declared in include/quasi/quasi_hip_ext.h.  No exception and no HIP
+// error crosses the ABI: failures are logged through Q_plugin_context.log and
+// reported as NULL handles, NULL readback data or negative return codes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/quasi/plugin_interface.h"
+#include "../../include/quasi/quasi_hip_ext.h"
+#include "bvh_build.hpp"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "scene_cornell.hpp"
+
+#define Q_EXPORT __attribute__((visibility("default")))
+
+using namespace quasi_hip;
+
+namespace {
+
+constexpr const char* kName = "HIP Path Tracer";
+constexpr const char* kDescription = "Cornell Box path tracer using HIP on AMD Instinct (gfx950)";
+constexpr const char* kAuthor = "Quasi";
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum KernelSlot : uint32_t { K_MEGAKERNEL = 0, K_RESOLVE, K_FILL, K_COUNT };
+const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve", "fill"};
+
+struct TimedLaunch {
+    uint32_t slot;
+    hipEvent_t start, stop;
+};
+
+struct plugin_state {
+    Q_plugin_context* context = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipStream_t last_stream = nullptr;
+
+    Q_render_params params{};
+
+    // host copy of the scene
+    std::vector<float> verts;
+    std::vector<uint32_t> indices;
+    std::vector<float> albedo, emission;
+
+    // device scene
+    DeviceBuffer tris, mats, nodes, lights;
+    uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
+    double bvh_build_ms = 0;
+
+    // framebuffer
+    uint32_t width = 0, height = 0;
+    uint32_t sample_count = 0;
+    DeviceBuffer accum[Q_AOV_COUNT];
+    DeviceBuffer ids;
+    DeviceBuffer resolved;
+    DeviceBuffer counters;
+
+    // stats
+    std::vector<hipEvent_t> event_pool;
+    std::vector<TimedLaunch> timed;
+    uint64_t launches[K_COUNT] = {};
+    double kernel_ms[K_COUNT] = {};
+    double render_ms = 0;
+};
+
+void log_msg(plugin_state* st, const char* fmt, ...) {
+    if (!st || !st->context || !st->context->log) return;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    st->context->log(st->context->host_data, buf);
+}
+
+void log_ctx(Q_plugin_context* ctx, const char* msg) {
+    if (ctx && ctx->log) ctx->log(ctx->host_data, msg);
+}
+
+bool hip_ok(plugin_state* st, hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    log_msg(st, "HIP error in %s: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+void free_buffer(DeviceBuffer& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+bool ensure_buffer(plugin_state* st, DeviceBuffer& b, size_t bytes, const char* what) {
+    if (b.bytes >= bytes && b.ptr) return true;
+    free_buffer(b);
+    if (bytes == 0) return true;
+    if (!hip_ok(st, hipMalloc(&b.ptr, bytes), what)) {
+        b.ptr = nullptr;
+        return false;
+    }
+    b.bytes = bytes;
+    return true;
+}
+
+hipEvent_t take_event(plugin_state* st) {
+    if (!st->event_pool.empty()) {
+        hipEvent_t e = st->event_pool.back();
+        st->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one launch with events when kernel timing is on.
+struct LaunchTimer {
+    plugin_state* st;
+    hipStream_t s;
+    uint32_t slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(plugin_state* st_, hipStream_t s_, uint32_t slot_) : st(st_), s(s_), slot(slot_) {
+        st->launches[slot]++;
+        if (st->params.flags & Q_FLAG_TIME_KERNELS) {
+            a = take_event(st);
+            b = take_event(st);
+            if (a && b) (void)hipEventRecord(a, s);
+        }
+    }
+    ~LaunchTimer() {
+        if (a && b) {
+            (void)hipEventRecord(b, s);
+            st->timed.push_back({slot, a, b});
+        } else {
+            if (a) st->event_pool.push_back(a);
+            if (b) st->event_pool.push_back(b);
+        }
+    }
+};
+
+// Sums finished event pairs into the per-kernel totals (synchronises on them).
+void harvest_events(plugin_state* st) {
+    for (const TimedLaunch& t : st->timed) {
+        float ms = 0.f;
+        if (hipEventSynchronize(t.stop) == hipSuccess && hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess)
+            st->kernel_ms[t.slot] += ms;
+        st->event_pool.push_back(t.start);
+        st->event_pool.push_back(t.stop);
+    }
+    st->timed.clear();
+}
+
+Q_render_params default_params() {
+    Q_render_params p{};
+    p.struct_size = sizeof(Q_render_params);
+    p.spp_per_render = 1;          // plugin.mm:23 (one sample per frame)
+    p.max_depth = 5;               // plugin.mm:22
+    p.rr_start = 3;                // plugin.mm:264 ("bounce > 2")
+    p.seed = 0;
+    p.rng_mode = Q_RNG_REFERENCE;  // plugin.mm:295
+    p.integrator = Q_INTEGRATOR_REFERENCE;
+    p.engine = Q_ENGINE_MEGAKERNEL;
+    p.bvh_width = 2;
+    p.samples_per_pass = 0;
+    p.block_size = 0;
+    p.flags = 0;
+    p.id_sample = 0;
+    return p;
+}
+
+bool clear_accumulation(plugin_state* st, hipStream_t s) {
+    const size_t npix = size_t(st->width) * st->height;
+    for (auto& b : st->accum) {
+        LaunchTimer lt(st, s, K_FILL);
+        if (!hip_ok(st, launch_fill_u32(b.ptr, 4 * npix, 0u, s), "clear accumulation")) return false;
+    }
+    {
+        LaunchTimer lt(st, s, K_FILL);
+        if (!hip_ok(st, launch_fill_u32(st->ids.ptr, npix, kMissId, s), "clear ids")) return false;
+    }
+    st->sample_count = 0;
+    return true;
+}
+
+bool resize_framebuffer(plugin_state* st, uint32_t w, uint32_t h, hipStream_t s) {
+    const size_t npix = size_t(w) * h;
+    for (auto& b : st->accum)
+        if (!ensure_buffer(st, b, std::max<size_t>(16, 16 * npix), "accumulation buffer")) return false;
+    if (!ensure_buffer(st, st->ids, std::max<size_t>(4, 4 * npix), "id buffer")) return false;
+    st->width = w;
+    st->height = h;
+    return clear_accumulation(st, s);
+}
+
+// Uploads a mesh: BVH build on the host, then triangle / material / node /
+// light arrays in the layout of common.hpp.
+int32_t upload_scene(plugin_state* st) {
+    const uint32_t nt = uint32_t(st->indices.size() / 3);
+    const uint32_t nv = uint32_t(st->verts.size() / 3);
+    for (uint32_t i : st->indices)
+        if (i >= nv) {
+            log_msg(st, "scene: vertex index %u out of range (%u vertices)", i, nv);
+            return Q_EXT_ERR_ARGUMENT;
+        }
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<float> p0(3 * size_t(nt)), p1(3 * size_t(nt)), p2(3 * size_t(nt));
+    for (size_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            p0[3 * t + k] = st->verts[3 * size_t(st->indices[3 * t + 0]) + k];
+            p1[3 * t + k] = st->verts[3 * size_t(st->indices[3 * t + 1]) + k];
+            p2[3 * t + k] = st->verts[3 * size_t(st->indices[3 * t + 2]) + k];
+        }
+    BvhBuildResult bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, 0);
+    if (bvh.depth > megakernel_max_depth()) {
+        log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", bvh.depth, megakernel_max_depth());
+        return Q_EXT_ERR_UNSUPPORTED;
+    }
+    std::vector<uint32_t> bvh_pos(nt);
+    std::vector<float> tri4(12 * size_t(nt)), mat4(8 * size_t(nt));
+    for (uint32_t i = 0; i < nt; ++i) {
+        const uint32_t t = bvh.order[i];
+        bvh_pos[t] = i;
+        float* d = &tri4[12 * size_t(i)];
+        for (int k = 0; k < 3; ++k) {
+            d[k] = p0[3 * size_t(t) + k];
+            d[4 + k] = p1[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+            d[8 + k] = p2[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+        }
+        std::memcpy(&d[3], &t, 4);
+        d[7] = 0.f;
+        d[11] = 0.f;
+        float* m = &mat4[8 * size_t(i)];
+        const float* a = &st->albedo[3 * size_t(t)];
+        const float* e = &st->emission[3 * size_t(t)];
+        m[0] = a[0]; m[1] = a[1]; m[2] = a[2]; m[3] = std::max(e[0], std::max(e[1], e[2]));
+        m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
+    }
+    // Light table in input-triangle order: weight = area * (Le.r + Le.g + Le.b),
+    // prefix sums in double, cdf stored as float with the last entry set to 1.
+    std::vector<LightEntry> lights;
+    std::vector<double> wts;
+    for (uint32_t t = 0; t < nt; ++t) {
+        const float* e = &st->emission[3 * size_t(t)];
+        if (!(std::max(e[0], std::max(e[1], e[2])) > 0.1f)) continue;
+        double a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = double(p1[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
+            b[k] = double(p2[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
+        }
+        const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        lights.push_back({bvh_pos[t], 0.f, 0.f, float(area)});
+        wts.push_back(area * (double(e[0]) + double(e[1]) + double(e[2])));
+    }
+    double total = 0;
+    for (double w : wts) total += w;
+    double run = 0;
+    for (size_t i = 0; i < lights.size(); ++i) {
+        run += wts[i];
+        lights[i].cdf = total > 0 ? float(run / total) : 0.f;
+        lights[i].psel = total > 0 ? float(wts[i] / total) : 0.f;
+    }
+    if (!lights.empty()) lights.back().cdf = 1.0f;
+
+    if (!ensure_buffer(st, st->tris, std::max<size_t>(16, tri4.size() * 4), "triangles") ||
+        !ensure_buffer(st, st->mats, std::max<size_t>(16, mat4.size() * 4), "materials") ||
+        !ensure_buffer(st, st->nodes, std::max<size_t>(64, bvh.nodes.size() * 4), "BVH nodes") ||
+        !ensure_buffer(st, st->lights, std::max<size_t>(16, lights.size() * sizeof(LightEntry)), "lights"))
+        return Q_EXT_ERR_OUT_OF_MEMORY;
+    hipStream_t s = st->stream;
+    if (!hip_ok(st, hipStreamSynchronize(s), "scene upload") ||
+        (nt && !hip_ok(st, hipMemcpy(st->tris.ptr, tri4.data(), tri4.size() * 4, hipMemcpyHostToDevice), "triangle upload")) ||
+        (nt && !hip_ok(st, hipMemcpy(st->mats.ptr, mat4.data(), mat4.size() * 4, hipMemcpyHostToDevice), "material upload")) ||
+        (!bvh.nodes.empty() && !hip_ok(st, hipMemcpy(st->nodes.ptr, bvh.nodes.data(), bvh.nodes.size() * 4, hipMemcpyHostToDevice), "BVH upload")) ||
+        (!lights.empty() && !hip_ok(st, hipMemcpy(st->lights.ptr, lights.data(), lights.size() * sizeof(LightEntry), hipMemcpyHostToDevice), "light upload")))
+        return Q_EXT_ERR_DEVICE;
+    st->num_tris = nt;
+    st->num_nodes = bvh.num_nodes;
+    st->bvh_depth = bvh.depth;
+    st->num_lights = uint32_t(lights.size());
+    st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return Q_EXT_OK;
+}
+
+// Host camera packing (plugin.mm:649-674) and the basis of get_camera_ray
+// (plugin.mm:134-145), evaluated once per frame in double.
+CameraBasis make_camera(const Q_camera& c, uint32_t w, uint32_t h) {
+    float dx = c.target[0] - c.position[0], dy = c.target[1] - c.position[1], dz = c.target[2] - c.position[2];
+    const float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (len > 0.0f) { dx /= len; dy /= len; dz /= len; }
+    const double aspect = double(w) / double(h);
+    const double theta = double(c.fov) * M_PI / 180.0;
+    const double hh = std::tan(theta / 2.0);
+    const double vh = 2.0 * hh, vw = aspect * vh;
+    double wv[3] = {-double(dx), -double(dy), -double(dz)};
+    double wl = std::sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+    for (double& x : wv) x /= wl;
+    const double up[3] = {c.up[0], c.up[1], c.up[2]};
+    double right[3] = {up[1] * wv[2] - up[2] * wv[1], up[2] * wv[0] - up[0] * wv[2], up[0] * wv[1] - up[1] * wv[0]};
+    double rl = std::sqrt(right[0] * right[0] + right[1] * right[1] + right[2] * right[2]);
+    for (double& x : right) x /= rl;
+    const double cup[3] = {wv[1] * right[2] - wv[2] * right[1], wv[2] * right[0] - wv[0] * right[2],
+                           wv[0] * right[1] - wv[1] * right[0]};
+    CameraBasis cb{};
+    for (int k = 0; k < 3; ++k) {
+        const double hor = right[k] * vw, ver = cup[k] * vh;
+        cb.origin[k] = c.position[k];
+        cb.horizontal[k] = float(hor);
+        cb.vertical[k] = float(ver);
+        cb.llo[k] = float(-hor * 0.5 - ver * 0.5 - wv[k]);
+    }
+    return cb;
+}
+
+plugin_state* as_state(Q_plugin_handle* h) { return reinterpret_cast<plugin_state*>(h); }
+
+void destroy_state(plugin_state* st) {
+    if (!st) return;
+    (void)hipSetDevice(st->device);
+    if (st->last_stream) (void)hipStreamSynchronize(st->last_stream);
+    if (st->stream) (void)hipStreamSynchronize(st->stream);
+    harvest_events(st);
+    for (hipEvent_t e : st->event_pool) (void)hipEventDestroy(e);
+    st->event_pool.clear();
+    free_buffer(st->tris);
+    free_buffer(st->mats);
+    free_buffer(st->nodes);
+    free_buffer(st->lights);
+    for (auto& b : st->accum) free_buffer(b);
+    free_buffer(st->ids);
+    free_buffer(st->resolved);
+    free_buffer(st->counters);
+    if (st->own_stream && st->stream) (void)hipStreamDestroy(st->stream);
+    delete st;
+}
+
+bool sync_render(plugin_state* st) {
+    if (st->last_stream && !hip_ok(st, hipStreamSynchronize(st->last_stream), "render synchronise")) return false;
+    return hip_ok(st, hipStreamSynchronize(st->stream), "stream synchronise");
+}
+
+// Mean of one accumulation buffer into a malloc'd RGBA32F array.
+Q_aov_buffer read_mean(plugin_state* st, uint32_t aov) {
+    Q_aov_buffer out{};
+    const size_t npix = size_t(st->width) * st->height;
+    if (st->sample_count == 0 || npix == 0) return out;
+    if (!ensure_buffer(st, st->resolved, 16 * npix, "readback staging")) return out;
+    {
+        LaunchTimer lt(st, st->stream, K_RESOLVE);
+        if (!hip_ok(st, launch_resolve(static_cast<const float4*>(st->accum[aov].ptr),
+                                       static_cast<float4*>(st->resolved.ptr), npix, st->sample_count, st->stream),
+                    "resolve"))
+            return out;
+    }
+    float* data = static_cast<float*>(std::malloc(16 * npix));
+    if (!data) return out;
+    if (!hip_ok(st, hipMemcpyAsync(data, st->resolved.ptr, 16 * npix, hipMemcpyDeviceToHost, st->stream), "readback copy") ||
+        !hip_ok(st, hipStreamSynchronize(st->stream), "readback synchronise")) {
+        std::free(data);
+        return out;
+    }
+    out.data = data;
+    out.width = st->width;
+    out.height = st->height;
+    out.channels = 4;
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+Q_EXPORT uint32_t Q_plugin_abi_version(void) { return Q_PLUGIN_ABI_VERSION; }
+
+Q_EXPORT Q_plugin_info Q_plugin_get_info(void) {
+    Q_plugin_info info{};
+    info.name = kName;
+    info.version = {1, 0, 0};
+    info.description = kDescription;
+    info.author = kAuthor;
+    return info;
+}
+
+Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
+    if (!ctx || !ctx->gpu) return nullptr;
+    if (ctx->gpu->backend != Q_GPU_BACKEND_HIP) {
+        log_ctx(ctx, "HIP backend requires HIP GPU context");
+        return nullptr;
+    }
+    plugin_state* st = nullptr;
+    try {
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count <= 0) {
+            log_ctx(ctx, "HIP backend: no HIP device available");
+            return nullptr;
+        }
+        const int dev = ctx->gpu->device ? *static_cast<const int*>(ctx->gpu->device) : 0;
+        if (dev < 0 || dev >= count) {
+            log_ctx(ctx, "HIP backend: device ordinal out of range");
+            return nullptr;
+        }
+        st = new plugin_state{};
+        st->context = ctx;
+        st->device = dev;
+        st->params = default_params();
+        if (!hip_ok(st, hipSetDevice(dev), "hipSetDevice")) { delete st; return nullptr; }
+        if (ctx->gpu->queue) {
+            st->stream = static_cast<hipStream_t>(ctx->gpu->queue);
+        } else {
+            if (!hip_ok(st, hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking), "stream create")) {
+                delete st;
+                return nullptr;
+            }
+            st->own_stream = true;
+        }
+        if (!ensure_buffer(st, st->counters, sizeof(Counters), "counters") ||
+            !hip_ok(st, launch_fill_u32(st->counters.ptr, sizeof(Counters) / 4, 0u, st->stream), "counter clear")) {
+            destroy_state(st);
+            return nullptr;
+        }
+        // Default scene, as plugin.mm:581-584.
+        const float aspect = ctx->viewport_height ? float(ctx->viewport_width) / float(ctx->viewport_height) : 1.0f;
+        make_cornell_box(aspect, st->verts, st->indices, st->albedo, st->emission);
+        if (upload_scene(st) != Q_EXT_OK) { destroy_state(st); return nullptr; }
+        if (!resize_framebuffer(st, std::max(1u, ctx->viewport_width), std::max(1u, ctx->viewport_height), st->stream)) {
+            destroy_state(st);
+            return nullptr;
+        }
+        hipDeviceProp_t prop{};
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            log_msg(st, "Cornell Box path tracer initialized (HIP device %d: %s, %s)", dev, prop.name, prop.gcnArchName);
+        else
+            log_msg(st, "Cornell Box path tracer initialized (HIP device %d)", dev);
+        return reinterpret_cast<Q_plugin_handle*>(st);
+    } catch (...) {
+        log_ctx(ctx, "HIP backend: exception during create");
+        if (st) destroy_state(st);
+        return nullptr;
+    }
+}
+
+Q_EXPORT void Q_plugin_destroy(Q_plugin_handle* handle) {
+    if (!handle) return;
+    plugin_state* st = as_state(handle);
+    try {
+        log_msg(st, "Path tracer destroyed");
+        destroy_state(st);
+    } catch (...) {
+    }
+}
+
+Q_EXPORT void Q_plugin_update(Q_plugin_handle* handle, float delta_time) {
+    (void)handle;
+    (void)delta_time;
+}
+
+Q_EXPORT void Q_plugin_render(Q_plugin_handle* handle, Q_render_frame* frame) {
+    if (!handle || !frame) return;
+    plugin_state* st = as_state(handle);
+    try {
+        if (frame->width == 0 || frame->height == 0) return;
+        (void)hipSetDevice(st->device);
+        hipStream_t s = frame->command_buffer ? static_cast<hipStream_t>(frame->command_buffer) : st->stream;
+        if (st->last_stream && st->last_stream != s) (void)hipStreamSynchronize(st->last_stream);
+        st->last_stream = s;
+        if (frame->width != st->width || frame->height != st->height) {      // plugin.mm:638-640
+            if (!resize_framebuffer(st, frame->width, frame->height, s)) return;
+        }
+        if (frame->camera_dirty) {                                            // plugin.mm:643-646
+            if (!clear_accumulation(st, s)) return;
+        }
+        const Q_render_params& rp = st->params;
+        PassParams p{};
+        p.tris = st->tris.ptr;
+        p.mats = st->mats.ptr;
+        p.nodes = st->nodes.ptr;
+        p.lights = static_cast<const LightEntry*>(st->lights.ptr);
+        p.accum_beauty = st->accum[Q_AOV_BEAUTY].ptr;
+        p.accum_albedo = st->accum[Q_AOV_ALBEDO].ptr;
+        p.accum_normal = st->accum[Q_AOV_NORMAL].ptr;
+        p.accum_depth = st->accum[Q_AOV_DEPTH].ptr;
+        p.ids = static_cast<uint32_t*>(st->ids.ptr);
+        p.counters = static_cast<Counters*>(st->counters.ptr);
+        p.cam = make_camera(frame->camera, st->width, st->height);
+        p.width = st->width;
+        p.height = st->height;
+        p.sample_begin = st->sample_count;
+        p.sample_count = rp.spp_per_render;
+        p.max_depth = rp.max_depth;
+        p.rr_start = rp.rr_start;
+        p.seed = rp.seed;
+        p.rng_mode = rp.rng_mode;
+        p.integrator = rp.integrator;
+        p.num_lights = st->num_lights;
+        p.num_nodes = st->num_nodes;
+        p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
+        p.id_sample = rp.id_sample;
+        {
+            LaunchTimer lt(st, s, K_MEGAKERNEL);
+            if (!hip_ok(st, launch_megakernel(p, st->bvh_depth, s), "megakernel launch")) return;
+        }
+        st->sample_count += rp.spp_per_render;                                // plugin.mm:791
+    } catch (...) {
+        log_msg(st, "HIP backend: exception during render");
+    }
+}
+
+Q_EXPORT Q_readback_result Q_plugin_readback(Q_plugin_handle* handle) {
+    Q_readback_result result{};
+    if (!handle) return result;
+    plugin_state* st = as_state(handle);
+    try {
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return result;
+        Q_aov_buffer b = read_mean(st, Q_AOV_BEAUTY);
+        result.data = b.data;
+        result.width = b.width;
+        result.height = b.height;
+        result.channels = b.channels;
+        if (b.data) log_msg(st, "HDR readback complete");
+    } catch (...) {
+        log_msg(st, "HIP backend: exception during readback");
+    }
+    return result;
+}
+
+Q_EXPORT void Q_plugin_readback_free(Q_readback_result* result) {
+    if (result && result->data) {
+        std::free(result->data);
+        result->data = nullptr;
+    }
+}
+
+Q_EXPORT Q_readback_aov_result Q_plugin_readback_aov(Q_plugin_handle* handle) {
+    Q_readback_aov_result result{};
+    if (!handle) return result;
+    plugin_state* st = as_state(handle);
+    try {
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return result;
+        for (uint32_t a = 0; a < Q_AOV_COUNT; ++a) result.buffers[a] = read_mean(st, a);
+        log_msg(st, "AOV readback complete");
+    } catch (...) {
+        log_msg(st, "HIP backend: exception during AOV readback");
+    }
+    return result;
+}
+
+Q_EXPORT void Q_plugin_readback_aov_free(Q_readback_aov_result* result) {
+    if (!result) return;
+    for (int i = 0; i < Q_AOV_COUNT; ++i) {
+        if (result->buffers[i].data) {
+            std::free(result->buffers[i].data);
+            result->buffers[i].data = nullptr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- extensions
+
+Q_EXPORT int32_t Q_plugin_set_scene(Q_plugin_handle* handle, const Q_scene_desc* scene) {
+    if (!handle || !scene || scene->struct_size < sizeof(Q_scene_desc)) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    try {
+        if ((scene->num_triangles && (!scene->indices || !scene->albedo || !scene->emission)) ||
+            (scene->num_vertices && !scene->vertices)) {
+            log_msg(st, "set_scene: missing array");
+            return Q_EXT_ERR_ARGUMENT;
+        }
+        if (scene->num_triangles >= (1u << 29)) {
+            log_msg(st, "set_scene: too many triangles (%u)", scene->num_triangles);
+            return Q_EXT_ERR_UNSUPPORTED;
+        }
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        const size_t nv = scene->num_vertices, nt = scene->num_triangles;
+        st->verts.assign(scene->vertices, scene->vertices + 3 * nv);
+        st->indices.assign(scene->indices, scene->indices + 3 * nt);
+        st->albedo.assign(scene->albedo, scene->albedo + 3 * nt);
+        st->emission.assign(scene->emission, scene->emission + 3 * nt);
+        int32_t rc = upload_scene(st);
+        if (rc != Q_EXT_OK) return rc;
+        if (!clear_accumulation(st, st->stream)) return Q_EXT_ERR_DEVICE;
+        return Q_EXT_OK;
+    } catch (const std::bad_alloc&) {
+        log_msg(st, "set_scene: host allocation failed");
+        return Q_EXT_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        log_msg(st, "set_scene: exception");
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+Q_EXPORT int32_t Q_plugin_get_params(Q_plugin_handle* handle, Q_render_params* out) {
+    if (!handle || !out || out->struct_size < sizeof(Q_render_params)) return Q_EXT_ERR_ARGUMENT;
+    *out = as_state(handle)->params;
+    return Q_EXT_OK;
+}
+
+Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_params* in) {
+    if (!handle || !in || in->struct_size < sizeof(Q_render_params)) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    try {
+        Q_render_params p = *in;
+        p.struct_size = sizeof(Q_render_params);
+        if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > 16 ||
+            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_MEGAKERNEL ||
+            (p.bvh_width != 0 && p.bvh_width != 2)) {
+            log_msg(st, "set_params: unsupported parameter combination");
+            return Q_EXT_ERR_UNSUPPORTED;
+        }
+        if (p.bvh_width == 0) p.bvh_width = 2;
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        st->params = p;
+        if (!clear_accumulation(st, st->stream)) return Q_EXT_ERR_DEVICE;
+        return Q_EXT_OK;
+    } catch (...) {
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+Q_EXPORT int32_t Q_plugin_readback_ids(Q_plugin_handle* handle, uint32_t* out, uint64_t count) {
+    if (!handle || !out) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    const uint64_t npix = uint64_t(st->width) * st->height;
+    if (count < npix) {
+        log_msg(st, "readback_ids: buffer holds %llu ids, %llu needed", (unsigned long long)count,
+                (unsigned long long)npix);
+        return Q_EXT_ERR_ARGUMENT;
+    }
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    if (!hip_ok(st, hipMemcpy(out, st->ids.ptr, 4 * npix, hipMemcpyDeviceToHost), "id readback")) return Q_EXT_ERR_DEVICE;
+    return Q_EXT_OK;
+}
+
+Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out) {
+    if (!handle || !out || out->struct_size < sizeof(Q_render_stats)) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    harvest_events(st);
+    Counters c{};
+    if (!hip_ok(st, hipMemcpy(&c, st->counters.ptr, sizeof c, hipMemcpyDeviceToHost), "counter readback"))
+        return Q_EXT_ERR_DEVICE;
+    Q_render_stats s{};
+    s.struct_size = sizeof(Q_render_stats);
+    s.width = st->width;
+    s.height = st->height;
+    s.sample_count = st->sample_count;
+    s.num_triangles = st->num_tris;
+    s.num_lights = st->num_lights;
+    s.bvh_nodes = st->num_nodes;
+    s.bvh_depth = st->bvh_depth;
+    s.rays_camera = c.rays_camera;
+    s.rays_extension = c.rays_extension;
+    s.rays_shadow = c.rays_shadow;
+    s.bvh_build_ms = st->bvh_build_ms;
+    s.render_ms = st->kernel_ms[K_MEGAKERNEL];
+    s.num_kernels = K_COUNT;
+    for (uint32_t k = 0; k < K_COUNT; ++k) {
+        std::snprintf(s.kernels[k].name, sizeof s.kernels[k].name, "%s", kKernelNames[k]);
+        s.kernels[k].launches = st->launches[k];
+        s.kernels[k].total_ms = st->kernel_ms[k];
+    }
+    *out = s;
+    return Q_EXT_OK;
+}
+
+Q_EXPORT int32_t Q_plugin_reset_stats(Q_plugin_handle* handle) {
+    if (!handle) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    harvest_events(st);
+    for (uint32_t k = 0; k < K_COUNT; ++k) { st->launches[k] = 0; st->kernel_ms[k] = 0; }
+    st->render_ms = 0;
+    if (!hip_ok(st, launch_fill_u32(st->counters.ptr, sizeof(Counters) / 4, 0u, st->stream), "counter clear") ||
+        !hip_ok(st, hipStreamSynchronize(st->stream), "counter clear"))
+        return Q_EXT_ERR_DEVICE;
+    return Q_EXT_OK;
+}
+
+Q_EXPORT int32_t Q_plugin_synchronize(Q_plugin_handle* handle) {
+    if (!handle) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    (void)hipSetDevice(st->device);
+    return sync_render(st) ? Q_EXT_OK : Q_EXT_ERR_DEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+"""Synthetic scene generators for the render workloads.
+
+Every generator is deterministic for its seed (numpy PCG64 streams) and
+returns a ``Mesh``: float32 vertices, uint32 triangle indices and per-triangle
+float32 albedo / emission, plus the camera the workload is rendered with.
+
+- ``cornell_ref``: the reference's own scene (src/quasi/scene/cornell_box.hpp:95-134),
+  16 quads as 32 triangles.
+- ``cornell36``: 36-triangle Cornell-style room (5 walls, two axis-aligned
+  6-face boxes, one ceiling light of radiance 15), albedo U[0.1, 0.8] per quad.
+- ``soup``: triangle soup with centroids uniform in [-1, 1]^3 and sizes
+  log-uniform in [1e-3, 5e-2], albedo U[0, 1], four 0.5 x 0.5 ceiling lights.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+F32 = np.float32
+
+
+@dataclass
+class Mesh:
+    vertices: np.ndarray          # (V, 3) float32
+    indices: np.ndarray           # (T, 3) uint32
+    albedo: np.ndarray            # (T, 3) float32
+    emission: np.ndarray          # (T, 3) float32
+    camera: dict = field(default_factory=dict)
+    name: str = ""
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.indices.shape[0])
+
+    def triangles(self) -> np.ndarray:
+        """(T, 3, 3) float32 vertex positions."""
+        return self.vertices[self.indices]
+
+
+REFERENCE_CAMERA = dict(position=(0.0, 1.0, 3.5), target=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=40.0)
+
+
+class _Builder:
+    def __init__(self):
+        self.v, self.i, self.a, self.e = [], [], [], []
+
+    def quad(self, o, u, v, albedo, emission=(0.0, 0.0, 0.0)):
+        """Quad (o, u, v) as triangles (o, o+u, o+u+v) and (o, o+u+v, o+v), in float32."""
+        o, u, v = (np.asarray(x, dtype=F32) for x in (o, u, v))
+        base = sum(len(x) for x in self.v)
+        ou = o + u
+        self.v.append(np.stack([o, ou, ou + v, o + v]))
+        self.i.append(np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32) + base)
+        self.a.append(np.tile(np.asarray(albedo, dtype=F32), (2, 1)))
+        self.e.append(np.tile(np.asarray(emission, dtype=F32), (2, 1)))
+
+    def triangles(self, tri, albedo, emission):
+        base = sum(len(x) for x in self.v)
+        n = tri.shape[0]
+        self.v.append(tri.reshape(-1, 3).astype(F32))
+        self.i.append((np.arange(3 * n, dtype=np.uint32) + base).reshape(n, 3))
+        self.a.append(albedo.astype(F32))
+        self.e.append(emission.astype(F32))
+
+    def mesh(self, camera, name) -> Mesh:
+        return Mesh(np.concatenate(self.v).astype(F32), np.concatenate(self.i).astype(np.uint32),
+                    np.concatenate(self.a).astype(F32), np.concatenate(self.e).astype(F32), dict(camera), name)
+
+
+def _rotate_y(p, deg):
+    """cornell_box.hpp:32-37, in float32."""
+    rad = F32(deg) * F32(3.14159265359) / F32(180.0)
+    c, s = F32(np.cos(rad)), F32(np.sin(rad))
+    p = np.asarray(p, dtype=F32)
+    return np.array([p[0] * c + p[2] * s, p[1], -p[0] * s + p[2] * c], dtype=F32)
+
+
+def _box_corners(center, size, deg):
+    hw, h, hd = F32(size[0]) * F32(0.5), F32(size[1]), F32(size[2]) * F32(0.5)
+    local = [(-hw, 0, -hd), (hw, 0, -hd), (-hw, 0, hd), (hw, 0, hd),
+             (-hw, h, -hd), (hw, h, -hd), (-hw, h, hd), (hw, h, hd)]
+    c = np.asarray(center, dtype=F32)
+    return [(_rotate_y(q, deg) + c).astype(F32) for q in local]
+
+
+def _box5(b: _Builder, center, size, deg, albedo):
+    """Five faces (no bottom), cornell_box.hpp:45-90."""
+    p = _box_corners(center, size, deg)
+    b.quad(p[4], p[5] - p[4], p[6] - p[4], albedo)
+    b.quad(p[0], p[1] - p[0], p[4] - p[0], albedo)
+    b.quad(p[3], p[2] - p[3], p[7] - p[3], albedo)
+    b.quad(p[2], p[0] - p[2], p[6] - p[2], albedo)
+    b.quad(p[1], p[3] - p[1], p[5] - p[1], albedo)
+
+
+WHITE = (0.73, 0.73, 0.73)
+RED = (0.65, 0.05, 0.05)
+GREEN = (0.12, 0.45, 0.15)
+LIGHT_E = (15.0, 15.0, 15.0)
+
+
+def _walls(b: _Builder, albedos=None):
+    """Floor, ceiling, back, left, right (cornell_box.hpp:103-116)."""
+    quads = [((-1, 0, -1), (2, 0, 0), (0, 0, 2)),
+             ((-1, 2, 1), (2, 0, 0), (0, 0, -2)),
+             ((-1, 0, -1), (2, 0, 0), (0, 2, 0)),
+             ((-1, 0, 1), (0, 0, -2), (0, 2, 0)),
+             ((1, 0, -1), (0, 0, 2), (0, 2, 0))]
+    cols = albedos if albedos is not None else [WHITE, WHITE, WHITE, RED, GREEN]
+    for (o, u, v), a in zip(quads, cols):
+        b.quad(o, u, v, a)
+
+
+def _ceiling_light(b: _Builder):
+    """cornell_box.hpp:119-120."""
+    b.quad((-0.25, 1.99, -0.25), (0.5, 0, 0), (0, 0, 0.5), (0.0, 0.0, 0.0), LIGHT_E)
+
+
+def cornell_ref() -> Mesh:
+    b = _Builder()
+    _walls(b)
+    _ceiling_light(b)
+    _box5(b, (-0.35, 0.0, 0.3), (0.5, 1.2, 0.5), 15.0, WHITE)
+    _box5(b, (0.35, 0.0, -0.3), (0.55, 0.55, 0.55), -18.0, WHITE)
+    return b.mesh(REFERENCE_CAMERA, "cornell_ref")
+
+
+def _box6_axis_aligned(b: _Builder, center, size, albedos):
+    """Six outward-facing faces of an axis-aligned box standing on y = 0."""
+    cx, cy, cz = center
+    hw, h, hd = size[0] * 0.5, size[1], size[2] * 0.5
+    x0, x1, y0, y1, z0, z1 = cx - hw, cx + hw, cy, cy + h, cz - hd, cz + hd
+    faces = [((x0, y1, z1), (x1 - x0, 0, 0), (0, 0, z0 - z1)),   # top (+y)
+             ((x0, y0, z0), (x1 - x0, 0, 0), (0, 0, z1 - z0)),   # bottom (-y)
+             ((x1, y0, z0), (x0 - x1, 0, 0), (0, y1 - y0, 0)),   # back (-z)
+             ((x0, y0, z1), (x1 - x0, 0, 0), (0, y1 - y0, 0)),   # front (+z)
+             ((x0, y0, z0), (0, 0, z1 - z0), (0, y1 - y0, 0)),   # left (-x)
+             ((x1, y0, z1), (0, 0, z0 - z1), (0, y1 - y0, 0))]   # right (+x)
+    for (o, u, v), a in zip(faces, albedos):
+        b.quad(o, u, v, a)
+
+
+def cornell36(seed: int = 0) -> Mesh:
+    """36 triangles: 5 walls + 2 six-face axis-aligned boxes (at the reference's
+    box centres and sizes) + the ceiling light; albedo U[0.1, 0.8] per quad and
+    channel from PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    alb = rng.uniform(0.1, 0.8, size=(17, 3)).astype(F32)
+    b = _Builder()
+    _walls(b, [tuple(a) for a in alb[:5]])
+    _ceiling_light(b)
+    _box6_axis_aligned(b, (-0.35, 0.0, 0.3), (0.5, 1.2, 0.5), [tuple(a) for a in alb[5:11]])
+    _box6_axis_aligned(b, (0.35, 0.0, -0.3), (0.55, 0.55, 0.55), [tuple(a) for a in alb[11:17]])
+    m = b.mesh(REFERENCE_CAMERA, "cornell36")
+    assert m.num_triangles == 36
+    return m
+
+
+SOUP_CAMERA = dict(position=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=40.0)
+SOUP_LIGHT_CENTERS = ((-0.5, 1.25, -0.5), (0.5, 1.25, -0.5), (-0.5, 1.25, 0.5), (0.5, 1.25, 0.5))
+
+
+def soup(num_triangles: int = 1_000_000, seed: int = 1) -> Mesh:
+    """Triangle soup: centre c ~ U[-1,1]^3, size s = exp(U[ln 1e-3, ln 5e-2]),
+    vertices c + s * d_k with d_k uniform on the unit sphere, albedo U[0,1]^3;
+    plus four 0.5 x 0.5 quad lights of radiance 15 at y = 1.25 facing -y
+    (8 triangles, appended after the soup)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = int(num_triangles)
+    c = rng.uniform(-1.0, 1.0, size=(n, 1, 3))
+    s = np.exp(rng.uniform(np.log(1e-3), np.log(5e-2), size=(n, 1, 1)))
+    d = rng.normal(size=(n, 3, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    tri = (c + s * d).astype(F32)
+    alb = rng.uniform(0.0, 1.0, size=(n, 3)).astype(F32)
+    b = _Builder()
+    b.triangles(tri, alb, np.zeros((n, 3), dtype=F32))
+    for cx, cy, cz in SOUP_LIGHT_CENTERS:
+        b.quad((cx - 0.25, cy, cz - 0.25), (0.5, 0, 0), (0, 0, 0.5), (0.0, 0.0, 0.0), LIGHT_E)
+    return b.mesh(SOUP_CAMERA, f"soup{n}")
+
+
+def camera_args(mesh: Mesh) -> dict:
+    return dict(mesh.camera)

@@ -1,0 +1,161 @@
+"""GPU parity: libquasi_hip.so (through the C ABI) against the FP64 oracle on
+the same samples.
+
+Bars (written here, checked below):
+- beauty and AOV images: relative L1 error <= 1e-4 (see tests/parity.py);
+- primary-hit triangle ids: >= 99.99% agreement, every disagreement a near tie;
+- ray counts: camera rays exact, extension/shadow rays within 0.1% (paths may
+  diverge after a near-tie or a rounding-sensitive decision).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quasi_amd import abi, scenes
+from quasi_amd.plugin import HipRenderer, make_camera
+from tests.parity import check_ids, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+REL_BAR = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def oracle_render(mesh, w, h, spp, **kw):
+    s = O.Scene(mesh)
+    return s, s.render(w, h, spp, precision=64, **kw)
+
+
+def compare(r, ref, mesh, oscene, params, label, check_aovs=True):
+    aov = r.readback_aov()
+    ids = r.primary_ids()
+    errs = {"beauty": rel_l1(aov["beauty"], ref.mean("beauty"))}
+    if check_aovs:
+        for k in ("albedo", "normal", "depth"):
+            errs[k] = rel_l1(aov[k], ref.mean(k))
+    frac, nbad, not_ties = check_ids(ids, ref.ids, mesh, oscene, params)
+    say(f"[{label}] rel-L1 {errs} ids agree {frac:.6f} ({nbad} differ, {len(not_ties)} not near ties)")
+    for k, e in errs.items():
+        assert e <= REL_BAR, (label, k, e)
+    assert np.allclose(aov["beauty"][..., 3], 1.0)
+    assert frac >= 0.9999, (label, frac)
+    assert not not_ties, (label, not_ties[:5])
+    return aov, ids
+
+
+def cam_of(mesh):
+    c = mesh.camera
+    return make_camera(c["position"], c["target"], c["up"], c["fov"])
+
+
+def test_reference_mode_default_scene():
+    """The plugin's own default scene and defaults (plugin.mm:556-592): 1 spp
+    per Q_plugin_render, reference integrator and seeding, depth 5."""
+    w, h, frames = 128, 128, 16
+    say(f"[ref-default] {w}x{h} frames={frames}")
+    with HipRenderer(w, h) as r:
+        p = r.get_params()
+        assert (p.spp_per_render, p.max_depth, p.rr_start, p.rng_mode, p.integrator) == (1, 5, 3, 0, 0)
+        r.set_params(flags=abi.Q_FLAG_RECORD_IDS, id_sample=0)
+        r.render(frames)
+        mesh = scenes.cornell_ref()
+        params = dict(width=w, height=h, rng_mode=0, camera=mesh.camera)
+        oscene, ref = oracle_render(mesh, w, h, frames, max_depth=5, rr_start=3, rng_mode=0, integrator=0)
+        compare(r, ref, mesh, oscene, params, "ref-default")
+        st = r.stats()
+        assert st.sample_count == frames and st.num_triangles == 32 and st.num_lights == 2
+        c = ref.counts
+        assert st.rays_camera == c.camera_rays == w * h * frames
+        assert abs(st.rays_extension - c.extension_rays) <= 1e-3 * c.extension_rays
+        assert st.rays_shadow == 0
+
+
+def test_nee_cornell36_config():
+    """36-triangle Cornell-style scene, 256x256, 16 spp, depth 4, NEE, hashed seeding."""
+    w = h = 256
+    mesh = scenes.cornell36(0)
+    say(f"[nee-cornell36] {w}x{h} spp=16")
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        r.set_params(spp_per_render=16, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=0,
+                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=0)
+        r.render(1)
+        params = dict(width=w, height=h, rng_mode=1, seed=0, camera=mesh.camera)
+        oscene, ref = oracle_render(mesh, w, h, 16, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=0)
+        compare(r, ref, mesh, oscene, params, "nee-cornell36")
+        st = r.stats()
+        c = ref.counts
+        assert st.rays_camera == c.camera_rays
+        assert abs(st.rays_extension - c.extension_rays) <= 1e-3 * c.extension_rays
+        assert abs(st.rays_shadow - c.shadow_rays) <= 1e-3 * c.shadow_rays
+
+
+@pytest.mark.parametrize("ntri,w,h,spp", [(200_000, 160, 90, 4)])
+def test_nee_soup_bvh(ntri, w, h, spp):
+    """Triangle soup (the 1M-triangle workload's generator at reduced size),
+    depth 8 with Russian roulette from bounce 3: exercises the BVH."""
+    mesh = scenes.soup(ntri, seed=1)
+    say(f"[nee-soup] ntri={ntri} {w}x{h} spp={spp}")
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        r.set_params(spp_per_render=spp, max_depth=8, rr_start=3, rng_mode=1, integrator=1, seed=0,
+                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=0)
+        r.render(1)
+        params = dict(width=w, height=h, rng_mode=1, seed=0, camera=mesh.camera)
+        oscene, ref = oracle_render(mesh, w, h, spp, max_depth=8, rr_start=3, rng_mode=1, integrator=1, seed=0)
+        compare(r, ref, mesh, oscene, params, "nee-soup")
+        st = r.stats()
+        say(f"[nee-soup] bvh nodes={st.bvh_nodes} depth={st.bvh_depth} build_ms={st.bvh_build_ms:.1f}")
+        c = ref.counts
+        assert st.rays_camera == c.camera_rays
+        assert abs(st.rays_extension - c.extension_rays) <= 1e-3 * c.extension_rays
+        assert abs(st.rays_shadow - c.shadow_rays) <= 1e-3 * c.shadow_rays
+
+
+def test_render_is_deterministic_and_passes_compose():
+    """Two identical renders are bitwise equal; 8 calls of 1 spp match one call
+    of 8 spp (same sample indices) to FP32 summation-order rounding."""
+    w = h = 64
+    mesh = scenes.cornell36(0)
+    say("[determinism]")
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        r.set_params(spp_per_render=8, max_depth=4, rng_mode=1, integrator=1)
+        r.render(1)
+        a = r.readback()
+        r.set_params(spp_per_render=8, max_depth=4, rng_mode=1, integrator=1)
+        r.render(1)
+        b = r.readback()
+        assert np.array_equal(a, b)
+        r.set_params(spp_per_render=1, max_depth=4, rng_mode=1, integrator=1)
+        r.render(8)
+        c = r.readback()
+        assert np.allclose(a, c, rtol=2e-6, atol=1e-6)
+
+
+def test_camera_dirty_resets_and_resize():
+    """camera_dirty restarts the accumulation (plugin.mm:643-646); a frame of
+    another size reallocates it (plugin.mm:638-640)."""
+    say("[camera-dirty/resize]")
+    with HipRenderer(48, 40) as r:
+        assert r.readback() is None               # nothing rendered yet
+        r.render(3)
+        r.set_camera((0.3, 1.1, 3.2), (0.0, 1.0, 0.0))
+        r.render(2)
+        moved = r.readback()
+        assert r.stats().sample_count == 2
+        with HipRenderer(48, 40) as fresh:
+            fresh.set_camera((0.3, 1.1, 3.2), (0.0, 1.0, 0.0))
+            fresh.render(2)
+            assert np.array_equal(moved, fresh.readback())
+        r.resize(32, 24)
+        r.render(1)
+        img = r.readback()
+        assert img.shape == (24, 32, 4)
+        assert r.stats().sample_count == 1

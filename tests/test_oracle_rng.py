@@ -1,0 +1,98 @@
+"""Pins the oracle's RNG (restating backends/metal/plugin.mm:98-107, :295).
+
+Known answers were computed from the formula by hand-independent means (the
+values listed in the reference-path survey); the float conversion is checked
+against numpy's own u32 -> f32 rounding for a large random set of states.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+KAT_HASH = {0: 129708002, 1: 2831084092, 2: 2055130248, 12345: 4099845390, 0xFFFFFFFF: 3861530882}
+
+KAT_FLOATS = {
+    (0, 0, 0): [0.190399364, 0.499476701, 0.551548481, 0.214354739],
+    (128, 128, 0): [0.487917453, 0.95989126, 0.785252154, 0.167519778],
+    (5, 7, 3): [0.983665764, 0.189699858, 0.685519755, 0.9283939],
+}
+
+
+def pcg_hash_np(x):
+    """Independent vectorised restatement used to cross-check the C oracle."""
+    x = np.asarray(x, dtype=np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    state = (x * np.uint64(747796405) + np.uint64(2891336453)) & m
+    shift = (state >> np.uint64(28)) + np.uint64(4)
+    word = (((state >> shift) ^ state) * np.uint64(277803737)) & m
+    return ((word >> np.uint64(22)) ^ word).astype(np.uint32)
+
+
+@pytest.mark.parametrize("x,h", sorted(KAT_HASH.items()))
+def test_pcg_hash_known_answers(x, h):
+    assert O.pcg_hash(x) == h
+
+
+def test_pcg_hash_matches_vectorised_restatement():
+    xs = np.random.default_rng(3).integers(0, 2**32, size=20000, dtype=np.uint64).astype(np.uint32)
+    ref = pcg_hash_np(xs)
+    got = np.array([O.pcg_hash(int(x)) for x in xs[:2000]], dtype=np.uint32)
+    assert np.array_equal(got, ref[:2000])
+
+
+def test_pcg_hash_is_a_bijection_on_a_window():
+    # a permutation of u32 restricted to 2^20 consecutive inputs has no collisions
+    xs = np.arange(1 << 20, dtype=np.uint64)
+    assert np.unique(pcg_hash_np(xs)).size == xs.size
+
+
+@pytest.mark.parametrize("key", sorted(KAT_FLOATS))
+def test_random_float_known_answers(key):
+    x, y, f = key
+    st = O.seed(0, x, y, 256, 256, f)
+    assert st == O.pcg_hash(x + y * 1920 + f * 1920 * 1080)
+    got = O.random_floats(st, 4)
+    np.testing.assert_allclose(got, KAT_FLOATS[key], rtol=0, atol=5e-9)
+
+
+def test_random_float_is_exact_scaling_of_rounded_u32():
+    # float(state) / float(0xFFFFFFFF): the divisor rounds to 2^32, so the
+    # result equals float32(state) * 2^-32 bit for bit.
+    rng = np.random.default_rng(11)
+    for start in rng.integers(0, 2**32, size=50, dtype=np.uint64):
+        seq = O.random_floats(int(start), 64)
+        states = []
+        s = int(start)
+        for _ in range(64):
+            s = int(pcg_hash_np(s))
+            states.append(s)
+        expect = (np.array(states, dtype=np.uint32).astype(np.float32) * np.float32(2.0 ** -32)).astype(np.float32)
+        assert np.array_equal(seq.view(np.uint32), expect.view(np.uint32))
+
+
+def test_random_float_can_reach_one():
+    # states >= 2^32 - 128 round up to 2^32 -> exactly 1.0 (plugin.mm:106)
+    s = 0xFFFFFFC0
+    assert np.float32(s) / np.float32(0xFFFFFFFF) == 1.0
+    # find an input whose hash lands there and check the oracle returns 1.0
+    found = None
+    for chunk in range(64):   # ~0.5 expected hits per 2^24 inputs
+        xs = np.arange(chunk << 24, (chunk + 1) << 24, dtype=np.uint64)
+        idx = np.nonzero(pcg_hash_np(xs) >= np.uint32(0xFFFFFF80))[0]
+        if idx.size:
+            found = int(xs[idx[0]])
+            break
+    assert found is not None
+    assert O.random_floats(found, 1)[0] == 1.0
+
+
+def test_hashed_seed_distinct_per_path():
+    w, h = 64, 32
+    seeds = {O.seed(1, x, y, w, h, s, 5) for s in range(4) for y in range(h) for x in range(w)}
+    assert len(seeds) == w * h * 4
+
+
+def test_reference_seed_uses_1920_stride():
+    # plugin.mm:295 seeds with x + y*1920 + frame*1920*1080 regardless of width
+    assert O.seed(0, 3, 2, 256, 256, 1) == O.pcg_hash(3 + 2 * 1920 + 1920 * 1080)
+    assert O.seed(0, 3, 2, 4096, 4096, 1) == O.seed(0, 3, 2, 16, 16, 1)
