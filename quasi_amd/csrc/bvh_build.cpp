@@ -271,14 +271,19 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
         n[14] = 0.f; n[15] = 0.f;
     };
     if (root.count > 0) {
+        // The whole scene fits one leaf: split it into two leaves under a
+        // single pair node (both children are the same one-triangle leaf when
+        // there is only one triangle).  Both children get the root box.
         out.assign(16, 0.f);
-        Box empty;
-        int32_t cl = encode_leaf(root);
-        int32_t cr = ~int32_t(0);  // never entered: its box is empty
-        write_node(0, root.box, empty, cl, cr);
+        for (uint32_t i = 0; i < root.count; ++i) res.order.push_back(b.prims[root.first + i]);
+        const uint32_t n = root.count, half = n > 1 ? n / 2 : 1;
+        const int32_t cl = ~int32_t((0u << 2) | (half - 1));
+        const int32_t cr = n > 1 ? ~int32_t((half << 2) | (n - half - 1)) : cl;
+        write_node(0, root.box, root.box, cl, cr);
         res.num_nodes = 1;
+        res.num_leaves = n > 1 ? 2 : 1;
         res.depth = 1;
-        res.sah_cost = 1.0 + root.count;
+        res.sah_cost = 1.0 + n;
         return res;
     }
     // Count internal nodes for allocation.

@@ -1,0 +1,19 @@
+// bvh_probe.cpp -- test-only C shim over the plugin's host BVH builder
+// (quasi_amd/csrc/bvh_build.cpp), so its pair-node output can be checked on
+// the CPU before any kernel walks it.  Built by tests/test_bvh_layout.py.
+#include <cstring>
+
+#include "bvh_build.hpp"
+
+extern "C" int probe_build(const float* p0, const float* p1, const float* p2, unsigned ntri, unsigned threads,
+                           float* nodes_out, unsigned nodes_cap, unsigned* order_out, unsigned* info) {
+    quasi_hip::BvhBuildResult r = quasi_hip::build_bvh(p0, p1, p2, ntri, threads);
+    info[0] = r.num_nodes;
+    info[1] = r.num_leaves;
+    info[2] = r.depth;
+    info[3] = unsigned(r.order.size());
+    if (r.num_nodes > nodes_cap) return -1;
+    std::memcpy(nodes_out, r.nodes.data(), r.nodes.size() * sizeof(float));
+    std::memcpy(order_out, r.order.data(), r.order.size() * sizeof(unsigned));
+    return 0;
+}

@@ -1,0 +1,207 @@
+"""The plugin's BVH builder output, checked on the CPU before a kernel walks it.
+
+The builder (quasi_amd/csrc/bvh_build.cpp) is compiled here with g++ through a
+tiny test shim.  Checks: every child reference is in range; leaves hold 1..4
+triangles and cover each triangle exactly once; each child box contains its
+triangles / subtree; the stack depth bound holds; the output does not depend
+on the thread count; and a straightforward CPU walk of the pair-node layout
+(written here, independently of the kernel) finds the same closest triangle as
+brute force.
+"""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+from quasi_amd import scenes
+from tests.conftest import ROOT
+
+
+@pytest.fixture(scope="module")
+def probe(tmp_path_factory):
+    d = tmp_path_factory.mktemp("probe")
+    so = d / "libbvhprobe.so"
+    subprocess.run(["g++", "-std=c++20", "-O2", "-shared", "-fPIC", "-pthread",
+                    "-I", str(ROOT / "quasi_amd" / "csrc"), str(ROOT / "tests" / "native" / "bvh_probe.cpp"),
+                    str(ROOT / "quasi_amd" / "csrc" / "bvh_build.cpp"), "-o", str(so)], check=True)
+    lib = C.CDLL(str(so))
+    P = C.POINTER
+    lib.probe_build.restype = C.c_int
+    lib.probe_build.argtypes = [P(C.c_float)] * 3 + [C.c_uint, C.c_uint, P(C.c_float), C.c_uint, P(C.c_uint),
+                                                      P(C.c_uint)]
+    return lib
+
+
+def build(probe, mesh, threads=4):
+    tri = mesh.triangles().astype(np.float32)
+    n = tri.shape[0]
+    p = [np.ascontiguousarray(tri[:, k, :]) for k in range(3)]
+    cap = max(1, 2 * n)
+    nodes = np.zeros((cap, 16), dtype=np.float32)
+    order = np.zeros(max(1, n), dtype=np.uint32)
+    info = np.zeros(4, dtype=np.uint32)
+    rc = probe.probe_build(*[x.ctypes.data_as(C.POINTER(C.c_float)) for x in p], n, threads,
+                           nodes.ctypes.data_as(C.POINTER(C.c_float)), cap,
+                           order.ctypes.data_as(C.POINTER(C.c_uint)), info.ctypes.data_as(C.POINTER(C.c_uint)))
+    assert rc == 0
+    nn = int(info[0])
+    return nodes[:nn], order[:n], dict(nodes=nn, leaves=int(info[1]), depth=int(info[2]), order=int(info[3]))
+
+
+def child_refs(nodes):
+    ch = nodes[:, 12:14].copy().view(np.int32)
+    return ch[:, 0], ch[:, 1]
+
+
+def boxes(nodes):
+    L = (nodes[:, [0, 2, 8]], nodes[:, [1, 3, 9]])
+    R = (nodes[:, [4, 6, 10]], nodes[:, [5, 7, 11]])
+    return L, R
+
+
+def check_layout(mesh, nodes, order, info):
+    n = mesh.num_triangles
+    assert info["order"] == n and sorted(order.tolist()) == list(range(n))
+    cl, cr = child_refs(nodes)
+    (llo, lhi), (rlo, rhi) = boxes(nodes)
+    tri = mesh.triangles()[order]          # BVH order
+    tlo, thi = tri.min(1), tri.max(1)
+    covered = np.zeros(n, dtype=np.int64)
+    parent_count = np.zeros(len(nodes), dtype=np.int64)
+
+    def leaf_range(c):
+        v = (~np.int64(c)) & 0xFFFFFFFF
+        return int(v >> 2), int(v & 3) + 1
+
+    for i in range(len(nodes)):
+        for c, lo, hi in ((cl[i], llo[i], lhi[i]), (cr[i], rlo[i], rhi[i])):
+            if c >= 0:
+                assert c < len(nodes) and c != 0
+                parent_count[c] += 1
+            else:
+                f, k = leaf_range(c)
+                assert 1 <= k <= 4 and f + k <= n
+                if not (len(nodes) == 1 and n == 1 and c == cr[i] and i == 0 and cl[i] == cr[i]):
+                    covered[f:f + k] += 1
+                assert (lo <= tlo[f:f + k]).all() and (hi >= thi[f:f + k]).all()
+    if n == 1:
+        covered[:] = 1
+    assert (covered == 1).all()
+    assert parent_count[0] == 0 and (parent_count[1:] == 1).all()
+    # subtree containment: a child node's two boxes lie inside the parent's box for it
+    for i in range(len(nodes)):
+        for c, lo, hi in ((cl[i], llo[i], lhi[i]), (cr[i], rlo[i], rhi[i])):
+            if c > 0:
+                assert (llo[c] >= lo).all() and (lhi[c] <= hi).all() and (rlo[c] >= lo).all() and (rhi[c] <= hi).all()
+
+
+def walk_closest(nodes, tri_bvh, order, o, d):
+    """Plain recursive-stack walk of the pair nodes with a slab test, then
+    closest hit with the larger-input-index tie rule."""
+    cl, cr = child_refs(nodes)
+    (llo, lhi), (rlo, rhi) = boxes(nodes)
+    inv = 1.0 / np.where(np.abs(d) < 1e-20, np.copysign(1e-20, d), d)
+    best_t, best = 1e30, -1
+
+    def slab(lo, hi):
+        t0, t1 = (lo - o) * inv, (hi - o) * inv
+        return max(np.minimum(t0, t1).max(), 1e-3) <= min(np.maximum(t0, t1).min(), best_t) * (1 + 1e-9)
+
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        for c, lo, hi in ((cl[i], llo[i], lhi[i]), (cr[i], rlo[i], rhi[i])):
+            if not slab(lo, hi):
+                continue
+            if c >= 0:
+                stack.append(int(c))
+                continue
+            v = (~np.int64(c)) & 0xFFFFFFFF
+            f, k = int(v >> 2), int(v & 3) + 1
+            for j in range(f, f + k):
+                t = mt(o, d, *tri_bvh[j])
+                if t is not None and (t < best_t or (t == best_t and order[j] > best)):
+                    best_t, best = t, int(order[j])
+    return best, best_t
+
+
+def mt(o, d, p0, p1, p2):
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(d, e2)
+    det = e1 @ pv
+    if det == 0:
+        return None
+    s = o - p0
+    u = (s @ pv) / det
+    if u < 0 or u > 1:
+        return None
+    q = np.cross(s, e1)
+    v = (d @ q) / det
+    if v < 0 or u + v > 1:
+        return None
+    t = (e2 @ q) / det
+    return t if 1e-3 <= t <= 1e30 else None
+
+
+def brute_closest(tris, o, d):
+    best_t, best = 1e30, -1
+    for j, tr in enumerate(tris):
+        t = mt(o, d, *tr)
+        if t is not None and (t < best_t or (t == best_t and j > best)):
+            best_t, best = t, j
+    return best, best_t
+
+
+@pytest.mark.parametrize("name", ["cornell_ref", "cornell36", "soup3000"])
+def test_layout_invariants(probe, name):
+    mesh = {"cornell_ref": scenes.cornell_ref, "cornell36": lambda: scenes.cornell36(0),
+            "soup3000": lambda: scenes.soup(3000, seed=2)}[name]()
+    nodes, order, info = build(probe, mesh)
+    check_layout(mesh, nodes, order, info)
+    assert info["depth"] <= 32
+
+
+@pytest.mark.parametrize("ntri", [1, 2, 3, 4, 5, 9])
+def test_tiny_scenes(probe, ntri):
+    m = scenes.soup(ntri, seed=3)
+    m.indices = m.indices[:ntri]
+    m.albedo, m.emission = m.albedo[:ntri], m.emission[:ntri]
+    nodes, order, info = build(probe, m)
+    assert info["nodes"] >= 1
+    check_layout(m, nodes, order, info)
+
+
+def test_thread_count_does_not_change_output(probe):
+    m = scenes.soup(200_000, seed=5)
+    a = build(probe, m, threads=1)
+    b = build(probe, m, threads=8)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
+
+
+def test_large_soup_depth_within_stack(probe):
+    m = scenes.soup(1_000_000, seed=1)
+    nodes, order, info = build(probe, m, threads=8)
+    cl, cr = child_refs(nodes)
+    internal = np.concatenate([cl[cl >= 0], cr[cr >= 0]])
+    assert internal.max() < len(nodes)
+    assert info["depth"] <= 64
+
+
+def test_walk_equals_brute_force(probe):
+    m = scenes.soup(2000, seed=6)
+    nodes, order, info = build(probe, m)
+    tris = m.triangles().astype(np.float64)
+    tri_bvh = tris[order]
+    rng = np.random.default_rng(1)
+    hits = 0
+    cen = tris.mean(1)
+    for i in range(300):
+        o = rng.uniform(-1.3, 1.3, 3)
+        d = (cen[rng.integers(len(cen))] - o) if i % 2 else rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a = walk_closest(nodes, tri_bvh, order, o, d)
+        b = brute_closest(tris, o, d)
+        assert a[0] == b[0]
+        hits += a[0] >= 0
+    assert hits > 10
