@@ -40,21 +40,29 @@ def cpu_baseline(mesh, args, seconds: float):
     from oracle import oracle as O
     threads = min(16, O.hardware_concurrency())
     s = O.Scene(mesh)
-    common = dict(max_depth=args.depth, rr_start=3, integrator=1, rng_mode=1, seed=0, precision=32,
-                  threads=threads)
-    npix = args.width * args.height
-    stride = 997
-    r = s.render(args.width, args.height, 1, pixel_stride=stride, **common)
-    per_path = r.counts.seconds / max(1, r.counts.camera_rays)
-    # choose the pixel stride so that one pass over samples 0..3 takes about `seconds`
-    target_paths = max(1000, seconds / max(per_path, 1e-9) / 4)
-    stride = max(1, int(npix / target_paths))
-    r = s.render(args.width, args.height, 4, pixel_stride=stride, **common)
-    c = r.counts
-    rays = c.camera_rays + c.extension_rays + c.shadow_rays
-    return dict(value=rays / c.seconds / 1e6, unit="Mrays/s", cores=int(c.threads), kind="port",
-                sample=(f"oracle FP32, {args.width}x{args.height} soup, samples 0..3 of every {stride}th pixel "
-                        f"({c.camera_rays} paths, {rays} rays, {c.seconds:.1f} s, BVH build excluded)"))
+    common = dict(max_depth=args.depth, rr_start=3, integrator=1, rng_mode=1, seed=0, precision=32)
+    W, H = args.width, args.height
+
+    def rays_of(c):
+        return c.camera_rays + c.extension_rays + c.shadow_rays
+
+    # one core: samples 0..3 of every 61st pixel
+    c1 = s.render(W, H, 4, pixel_stride=61, threads=1, **common).counts
+    one_core = rays_of(c1) / c1.seconds / 1e6
+    # all cores: passes of 4 samples over every 3rd pixel until the budget is spent
+    paths = rays = 0
+    secs = 0.0
+    offset = 0
+    while offset < args.spp and (secs < seconds or offset == 0):
+        c = s.render(W, H, 4, sample_offset=offset, pixel_stride=3, threads=threads, **common).counts
+        paths += c.camera_rays
+        rays += rays_of(c)
+        secs += c.seconds
+        offset += 4
+    return dict(value=rays / secs / 1e6, unit="Mrays/s", cores=threads, kind="port", value_1core=one_core,
+                sample=(f"oracle FP32, {W}x{H} soup, samples 0..{offset - 1} of every 3rd pixel "
+                        f"({paths} paths, {rays} rays, {secs:.1f} s on {threads} threads; 1-core figure from "
+                        f"samples 0..3 of every 61st pixel; BVH build excluded)"))
 
 
 def main():
