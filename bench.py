@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--ntri", type=int, default=1_000_000)
     ap.add_argument("--spp-per-launch", type=int, default=8)
+    ap.add_argument("--engine", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--samples-per-pass", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -104,8 +106,10 @@ def main():
         build_s = time.perf_counter() - t0
         c = mesh.camera
         r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+        engine = abi.Q_ENGINE_WAVEFRONT if args.engine == "wavefront" else abi.Q_ENGINE_MEGAKERNEL
         r.set_params(spp_per_render=args.spp_per_launch, max_depth=args.depth, rr_start=3, rng_mode=1,
-                     integrator=1, seed=0, flags=abi.Q_FLAG_TIME_KERNELS)
+                     integrator=1, seed=0, flags=abi.Q_FLAG_TIME_KERNELS, engine=engine,
+                     samples_per_pass=args.samples_per_pass)
         st = r.stats()
         log(f"BVH: {st.bvh_nodes} nodes, depth {st.bvh_depth}, build+upload {st.bvh_build_ms:.0f} ms "
             f"(set_scene {build_s:.2f} s)")
@@ -132,18 +136,24 @@ def main():
     rays = {"camera": st.rays_camera, "extension": st.rays_extension, "shadow": st.rays_shadow}
     total_rays = sum(rays.values())
     value = total_rays / elapsed / 1e6
-    k = st.kernels[0]
+    kern = {st.kernels[i].name.decode(): st.kernels[i] for i in range(st.num_kernels)}
+    dominant = "megakernel" if args.engine == "megakernel" else "wf_extend"
+    k = kern[dominant]
     launches = int(k.launches)
     avg_ms = k.total_ms / max(1, launches)
-    bytes_rays = sum(rays[t] * alg[t]["bytes_per_ray"] for t in rays)
-    bytes_fb = 32.0 * W * H * args.spp * args.steps
+    if dominant == "megakernel":
+        bytes_rays = sum(rays[t] * alg[t]["bytes_per_ray"] for t in rays)
+        bytes_fb = 32.0 * W * H * args.spp * args.steps
+    else:   # the extend kernel traces the camera and extension rays only
+        bytes_rays = sum(rays[t] * alg[t]["bytes_per_ray"] for t in ("camera", "extension"))
+        bytes_fb = 0.0
     per_launch = (bytes_rays + bytes_fb) / max(1, launches)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
     tf = ROOT / "profiles" / "hbm_traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get("megakernel_soup_1m_1080p", {}).get("bytes_per_launch")
+            traffic = json.loads(tf.read_text()).get(f"{dominant}_soup_1m_1080p", {}).get("bytes_per_launch")
         except (ValueError, AttributeError):
             traffic = None
     out = {
@@ -161,14 +171,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain wavefront-config soup: {mesh.num_triangles} triangles, {W}x{H}, "
                                f"{args.spp} spp, depth {args.depth}, RR from 3, NEE",
-                   "engine": "megakernel", "spp_per_launch": args.spp_per_launch, "bvh": "binned-SAH BVH2"},
+                   "engine": args.engine, "spp_per_launch": args.spp_per_launch, "bvh": "binned-SAH BVH2"},
         "rays": rays,
         "rays_per_path": round(total_rays / (W * H * args.spp * args.steps), 4),
         "bvh_build_ms": round(st.bvh_build_ms, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "megakernel", "avg_launch_ms": round(avg_ms, 3), "launches": launches,
+                     "kernel": dominant, "avg_launch_ms": round(avg_ms, 3), "launches": launches,
                      "alg_bytes_per_launch": round(per_launch)},
+        "kernel_ms": {n: round(v.total_ms, 3) for n, v in kern.items() if v.launches},
     }
     if not args.no_cpu:
         log("CPU baseline (oracle) ...")

@@ -34,6 +34,7 @@ HOSTONLY = ["-x", "c++", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__", "-I/op
 
 PLUGIN_SOURCES = [
     ("kernels_mega.hip", DEVICE),
+    ("kernels_wave.hip", DEVICE),
     ("kernels_util.hip", DEVICE),
     ("plugin.cpp", HOSTONLY),
     ("bvh_build.cpp", HOSTONLY),

@@ -49,6 +49,29 @@ struct Counters {
     unsigned long long pad;
 };
 
+constexpr uint32_t kMaxDepth = 16;
+
+// Wavefront path state.  Per-path arrays are indexed by the path id
+// p = sample_in_pass * W*H + pixel; per-slot arrays by the position in the
+// compacted queue of the current bounce (double buffered by bounce parity).
+struct WaveState {
+    void* ray_o[2];        // float4 per slot: origin.xyz, w = path id (bits)
+    void* ray_d[2];        // float4 per slot: direction.xyz
+    void* hit;             // uint2 per slot: (t bits, BVH triangle index or -1)
+    void* thr;             // float4 per path: throughput
+    void* radiance;        // float4 per path: radiance
+    uint32_t* rng;         // per path
+    void* aov0;            // float4 per path: first-hit albedo.xyz, w = first-hit t
+    void* aov1;            // float4 per path: first-hit normal*0.5+0.5, w = first-hit input id (bits)
+    void* sh_o;            // float4 per shadow slot: origin.xyz, w = t max
+    void* sh_d;            // float4 per shadow slot: direction.xyz, w = path id (bits)
+    void* sh_c;            // float4 per shadow slot: unoccluded contribution
+    uint32_t* counts;      // [kMaxDepth + 1] extension queue length per bounce
+    uint32_t* shadow_counts;  // [kMaxDepth] shadow queue length per bounce
+    uint32_t* fetch;       // [2 * kMaxDepth] work-fetch cursors (extend, shadow) per bounce
+    uint32_t paths;        // paths in this pass
+};
+
 // Everything one sample pass needs (passed by value as a kernel argument).
 struct PassParams {
     const void* tris;      // float4*

@@ -247,11 +247,161 @@ __device__ __forceinline__ bool trace_any(const float4* __restrict__ nodes, cons
     }
 }
 
+// ---------------------------------------------------------------- shading
+// What one path vertex produces.  The shadow ray (NEE) is returned instead of
+// traced so that the megakernel can trace it in place and the wavefront
+// engine can queue it.
+struct ShadeOut {
+    bool cont;       // the path continues with `next`
+    bool shadow;     // a shadow ray towards a light sample is requested
+    Ray next;
+    Ray sray;
+    float stmax;
+    f3 scontrib;     // added to the path radiance when the shadow ray is unoccluded
+};
+
+struct FirstHit {
+    f3 albedo, normal;
+    float t;
+    uint32_t id;
+};
+
+// One vertex of either integrator at bounce b for a ray that hit BVH triangle
+// `tri` at distance t.  Updates throughput, radiance and the random stream.
+__device__ __forceinline__ void shade_hit(const PassParams& P, const float4* __restrict__ tris,
+                                          const float4* __restrict__ mats, uint32_t b, const Ray& ray,
+                                          float t, int tri, f3& thr, f3& L, uint32_t& rng, FirstHit* first,
+                                          ShadeOut& out) {
+    out.cont = false;
+    out.shadow = false;
+    const float4 t0 = tris[3 * tri + 0];
+    const float4 t1 = tris[3 * tri + 1];
+    const float4 t2 = tris[3 * tri + 2];
+    const f3 n = normalize(cross(xyz(t1), xyz(t2)));
+    const f3 nf = dot(n, ray.d) < 0.f ? n : -n;           // plugin.mm:192
+    const f3 pt = ray.o + ray.d * t;                        // plugin.mm:179
+    const float4 m0 = mats[2 * tri + 0];
+    const float4 m1 = mats[2 * tri + 1];
+    const f3 alb = xyz(m0), Le = xyz(m1);
+    const float emit_str = m0.w;
+    if (first) {                                            // plugin.mm:242-249
+        first->albedo = emit_str > 0.1f ? Le : alb;
+        first->normal = nf * 0.5f + mk(0.5f, 0.5f, 0.5f);
+        first->t = t;
+        first->id = __float_as_uint(t0.w);
+    }
+    if (P.integrator == 0u) {
+        L = L + thr * Le;                                   // plugin.mm:252
+        if (emit_str > 0.1f) return;                        // plugin.mm:255-258
+        thr = thr * alb;                                    // plugin.mm:261
+        if (b + 1 >= P.max_depth) return;                   // no further trace
+        if (b >= P.rr_start) {                              // plugin.mm:264-268
+            const float q = fmaxf(0.05f, max3(thr));
+            if (random_float(rng) > q) return;
+            thr = thr * (1.0f / q);
+        }
+        out.next.o = pt + nf * kOffset;                     // plugin.mm:271-272
+        out.next.d = cosine_sample(nf, rng);
+        out.cont = true;
+        return;
+    }
+    if (b == 0) L = L + Le;
+    if (emit_str > 0.1f) return;
+    if (b + 1 >= P.max_depth) return;
+    const f3 org = pt + nf * kOffset;
+    if (P.num_lights) {
+        // light choice: first entry with r < cdf (the last when none)
+        const float rs = random_float(rng);
+        uint32_t lo = 0, hi = P.num_lights;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (rs < P.lights[mid].cdf) hi = mid; else lo = mid + 1;
+        }
+        if (lo >= P.num_lights) lo = P.num_lights - 1;
+        const LightEntry le = P.lights[lo];
+        const float r1 = random_float(rng);
+        const float r2 = random_float(rng);
+        const float4 l0 = tris[3 * le.tri + 0];
+        const float4 l1 = tris[3 * le.tri + 1];
+        const float4 l2 = tris[3 * le.tri + 2];
+        const float su = sqrtf(r1);
+        const f3 xl = xyz(l0) + xyz(l1) * (su * (1.0f - r2)) + xyz(l2) * (su * r2);
+        // geometry term at the hit point; the offset origin is used only for
+        // the visibility ray
+        const f3 w = xl - pt;
+        const float dist2 = dot(w, w);
+        const f3 wi = w * (1.0f / sqrtf(dist2));
+        const f3 nl = normalize(cross(xyz(l1), xyz(l2)));
+        const float cos_x = dot(nf, wi);
+        const float cos_l = fabsf(dot(nl, wi));
+        if (cos_x > 0.f && cos_l > 0.f) {
+            const f3 sv = xl - org;
+            const float sl = sqrtf(dot(sv, sv));
+            out.shadow = true;
+            out.sray.o = org;
+            out.sray.d = sv * (1.0f / sl);
+            out.stmax = sl * 0.999f;
+            const f3 Ll = xyz(mats[2 * le.tri + 1]);
+            const float g = cos_x * cos_l * le.area / (kPi * dist2 * le.psel);
+            out.scontrib = thr * alb * Ll * g;
+        }
+    }
+    thr = thr * alb;
+    if (b >= P.rr_start) {
+        const float q = fmaxf(0.05f, max3(thr));
+        if (random_float(rng) > q) return;
+        thr = thr * (1.0f / q);
+    }
+    out.next.o = org;
+    out.next.d = cosine_sample(nf, rng);
+    out.cont = true;
+}
+
+// Primary ray of pixel (x, y) for one sample (plugin.mm:129-153 with the
+// camera basis evaluated on the host; x jitter drawn first, plugin.mm:131).
+__device__ __forceinline__ Ray camera_ray(const PassParams& P, uint32_t x, uint32_t y, uint32_t& rng) {
+    const f3 llo = mk(P.cam.llo[0], P.cam.llo[1], P.cam.llo[2]);
+    const f3 hor = mk(P.cam.horizontal[0], P.cam.horizontal[1], P.cam.horizontal[2]);
+    const f3 ver = mk(P.cam.vertical[0], P.cam.vertical[1], P.cam.vertical[2]);
+    const float u0 = (float(x) + 0.5f) / float(P.width);
+    const float v0 = 1.0f - (float(y) + 0.5f) / float(P.height);
+    const float jx = random_float(rng);
+    const float jy = random_float(rng);
+    const float u = u0 + (jx - 0.5f) * 0.001f;
+    const float v = v0 + (jy - 0.5f) * 0.001f;
+    return Ray{mk(P.cam.origin[0], P.cam.origin[1], P.cam.origin[2]), normalize(llo + hor * u + ver * v)};
+}
+
 // Sum over the 64 lanes of a wave (all lanes must be present).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+// Lanes below this one in `mask` (wave64).
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Wave-wide stream compaction: every lane of the wave calls this with its
+// predicate; lanes with pred get consecutive slots reserved by one atomic per
+// wave on *counter.  Returns the lane's slot (meaningful only when pred).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t total = uint32_t(__popcll(mask));
+    uint32_t base = 0;
+    if ((threadIdx.x & 63u) == 0u && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    return base + lanes_below(mask);
+}
+
+// Wave-uniform work fetch: lane 0 reserves the next 64 items of a queue of n.
+__device__ __forceinline__ bool fetch_chunk(uint32_t* cursor, uint32_t n, uint32_t& base) {
+    uint32_t b = 0;
+    if ((threadIdx.x & 63u) == 0u) b = atomicAdd(cursor, 64u);
+    base = __shfl(b, 0, 64);
+    return base < n;
 }
 
 }  // namespace quasi_hip

@@ -17,6 +17,32 @@ uint32_t megakernel_max_depth();
 // One sample pass of the megakernel engine over the whole framebuffer.
 hipError_t launch_megakernel(const PassParams& p, uint32_t bvh_depth, hipStream_t stream);
 
+// Wavefront engine kernels, for timing hooks.
+enum WaveKernel : uint32_t {
+    WAVE_K_RAYGEN = 0,
+    WAVE_K_EXTEND,
+    WAVE_K_SHADE,
+    WAVE_K_SHADOW,
+    WAVE_K_ACCUMULATE,
+    WAVE_K_COUNT
+};
+
+// Optional callbacks around every wavefront launch (used to record hipEvents).
+struct WaveTimers {
+    void* user;
+    void (*before)(void* user, uint32_t kernel, hipStream_t stream);
+    void (*after)(void* user, uint32_t kernel, hipStream_t stream);
+};
+
+// Blocks of the persistent extend/shadow kernels that fill the device once.
+uint32_t wavefront_persistent_blocks(int device, uint32_t bvh_depth);
+
+// One sample pass of the wavefront engine over s.paths = W*H*S paths (samples
+// p.sample_begin .. p.sample_begin + S - 1): a fixed, host-synchronisation-free
+// launch sequence.
+hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, uint32_t bvh_depth,
+                                 uint32_t persistent_blocks, hipStream_t stream, WaveTimers* timers);
+
 // out[i] = sum[i] / n for npix RGBA pixels (the running mean of plugin.mm:326-327).
 hipError_t launch_resolve(const float4* sum, float4* out, size_t npix, uint32_t n, hipStream_t stream);
 

@@ -42,8 +42,21 @@ struct DeviceBuffer {
     size_t bytes = 0;
 };
 
-enum KernelSlot : uint32_t { K_MEGAKERNEL = 0, K_RESOLVE, K_FILL, K_COUNT };
-const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve", "fill"};
+enum KernelSlot : uint32_t {
+    K_MEGAKERNEL = 0, K_RESOLVE, K_FILL,
+    K_RAYGEN, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUMULATE,  // wavefront, in WaveKernel order
+    K_COUNT
+};
+const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve", "fill", "wf_raygen", "wf_extend",
+                                           "wf_shade", "wf_shadow", "wf_accumulate"};
+constexpr bool kRenderKernel[K_COUNT] = {true, false, false, true, true, true, true, true};
+
+// Wavefront path-state buffers (see WaveState in common.hpp).
+struct WaveBuffers {
+    DeviceBuffer ray_o[2], ray_d[2], hit, thr, radiance, rng, aov0, aov1, sh_o, sh_d, sh_c;
+    DeviceBuffer counts, shadow_counts, fetch;
+    uint32_t capacity = 0;
+};
 
 struct TimedLaunch {
     uint32_t slot;
@@ -76,6 +89,11 @@ struct plugin_state {
     DeviceBuffer ids;
     DeviceBuffer resolved;
     DeviceBuffer counters;
+
+    // wavefront engine
+    WaveBuffers wave;
+    uint32_t persistent_blocks = 0;
+    hipEvent_t pending_start = nullptr;
 
     // stats
     std::vector<hipEvent_t> event_pool;
@@ -299,8 +317,86 @@ int32_t upload_scene(plugin_state* st) {
     st->num_nodes = bvh.num_nodes;
     st->bvh_depth = bvh.depth;
     st->num_lights = uint32_t(lights.size());
+    st->persistent_blocks = wavefront_persistent_blocks(st->device, bvh.depth);
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return Q_EXT_OK;
+}
+
+void free_wave(WaveBuffers& w) {
+    for (auto* b : {&w.ray_o[0], &w.ray_o[1], &w.ray_d[0], &w.ray_d[1], &w.hit, &w.thr, &w.radiance, &w.rng,
+                    &w.aov0, &w.aov1, &w.sh_o, &w.sh_d, &w.sh_c, &w.counts, &w.shadow_counts, &w.fetch})
+        free_buffer(*b);
+    w.capacity = 0;
+}
+
+// Path-state buffers for `paths` paths (grown, never shrunk).
+bool ensure_wave(plugin_state* st, uint32_t paths) {
+    WaveBuffers& w = st->wave;
+    if (w.capacity >= paths) return true;
+    const size_t n = paths;
+    bool ok = true;
+    for (int k = 0; k < 2; ++k) {
+        ok = ok && ensure_buffer(st, w.ray_o[k], 16 * n, "wavefront rays");
+        ok = ok && ensure_buffer(st, w.ray_d[k], 16 * n, "wavefront rays");
+    }
+    ok = ok && ensure_buffer(st, w.hit, 8 * n, "wavefront hits") && ensure_buffer(st, w.thr, 16 * n, "throughput") &&
+         ensure_buffer(st, w.radiance, 16 * n, "radiance") && ensure_buffer(st, w.rng, 4 * n, "rng states") &&
+         ensure_buffer(st, w.aov0, 16 * n, "first-hit data") && ensure_buffer(st, w.aov1, 16 * n, "first-hit data") &&
+         ensure_buffer(st, w.sh_o, 16 * n, "shadow rays") && ensure_buffer(st, w.sh_d, 16 * n, "shadow rays") &&
+         ensure_buffer(st, w.sh_c, 16 * n, "shadow contributions") &&
+         ensure_buffer(st, w.counts, 4 * (kMaxDepth + 1), "queue lengths") &&
+         ensure_buffer(st, w.shadow_counts, 4 * kMaxDepth, "queue lengths") &&
+         ensure_buffer(st, w.fetch, 8 * kMaxDepth, "work cursors");
+    if (!ok) { free_wave(w); return false; }
+    w.capacity = paths;
+    return true;
+}
+
+WaveState wave_state(const WaveBuffers& w, uint32_t paths) {
+    WaveState s{};
+    for (int k = 0; k < 2; ++k) { s.ray_o[k] = w.ray_o[k].ptr; s.ray_d[k] = w.ray_d[k].ptr; }
+    s.hit = w.hit.ptr;
+    s.thr = w.thr.ptr;
+    s.radiance = w.radiance.ptr;
+    s.rng = static_cast<uint32_t*>(w.rng.ptr);
+    s.aov0 = w.aov0.ptr;
+    s.aov1 = w.aov1.ptr;
+    s.sh_o = w.sh_o.ptr;
+    s.sh_d = w.sh_d.ptr;
+    s.sh_c = w.sh_c.ptr;
+    s.counts = static_cast<uint32_t*>(w.counts.ptr);
+    s.shadow_counts = static_cast<uint32_t*>(w.shadow_counts.ptr);
+    s.fetch = static_cast<uint32_t*>(w.fetch.ptr);
+    s.paths = paths;
+    return s;
+}
+
+void wave_timer_before(void* user, uint32_t kernel, hipStream_t s) {
+    auto* st = static_cast<plugin_state*>(user);
+    st->launches[K_RAYGEN + kernel]++;
+    if (!(st->params.flags & Q_FLAG_TIME_KERNELS)) return;
+    st->pending_start = take_event(st);
+    if (st->pending_start) (void)hipEventRecord(st->pending_start, s);
+}
+
+void wave_timer_after(void* user, uint32_t kernel, hipStream_t s) {
+    auto* st = static_cast<plugin_state*>(user);
+    if (!st->pending_start) return;
+    hipEvent_t stop = take_event(st);
+    if (!stop) { st->event_pool.push_back(st->pending_start); st->pending_start = nullptr; return; }
+    (void)hipEventRecord(stop, s);
+    st->timed.push_back({K_RAYGEN + kernel, st->pending_start, stop});
+    st->pending_start = nullptr;
+}
+
+// Samples per wavefront pass: about 16M paths in flight unless the caller set it.
+uint32_t wave_samples_per_pass(const plugin_state* st) {
+    const uint32_t npix = st->width * st->height;
+    uint32_t s = st->params.samples_per_pass;
+    if (s == 0) s = std::max<uint32_t>(1u, (16u << 20) / std::max<uint32_t>(1u, npix));
+    s = std::min(s, st->params.spp_per_render);
+    while (s > 1 && uint64_t(s) * npix > 0xFFFFFFFFull / 2) s /= 2;
+    return std::max(1u, s);
 }
 
 // Host camera packing (plugin.mm:649-674) and the basis of get_camera_ray
@@ -351,6 +447,7 @@ void destroy_state(plugin_state* st) {
     free_buffer(st->ids);
     free_buffer(st->resolved);
     free_buffer(st->counters);
+    free_wave(st->wave);
     if (st->own_stream && st->stream) (void)hipStreamDestroy(st->stream);
     delete st;
 }
@@ -517,7 +614,21 @@ Q_EXPORT void Q_plugin_render(Q_plugin_handle* handle, Q_render_frame* frame) {
         p.num_nodes = st->num_nodes;
         p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
         p.id_sample = rp.id_sample;
-        {
+        if (rp.engine == Q_ENGINE_WAVEFRONT) {
+            const uint32_t per_pass = wave_samples_per_pass(st);
+            const uint32_t npix = st->width * st->height;
+            if (!ensure_wave(st, per_pass * npix)) return;
+            WaveTimers tm{st, wave_timer_before, wave_timer_after};
+            for (uint32_t done = 0; done < rp.spp_per_render; done += per_pass) {
+                const uint32_t n = std::min(per_pass, rp.spp_per_render - done);
+                p.sample_begin = st->sample_count + done;
+                p.sample_count = n;
+                const WaveState ws = wave_state(st->wave, n * npix);
+                if (!hip_ok(st, launch_wavefront_pass(p, ws, st->bvh_depth, st->persistent_blocks, s, &tm),
+                            "wavefront launch"))
+                    return;
+            }
+        } else {
             LaunchTimer lt(st, s, K_MEGAKERNEL);
             if (!hip_ok(st, launch_megakernel(p, st->bvh_depth, s), "megakernel launch")) return;
         }
@@ -626,7 +737,7 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
         Q_render_params p = *in;
         p.struct_size = sizeof(Q_render_params);
         if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > 16 ||
-            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_MEGAKERNEL ||
+            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT ||
             (p.bvh_width != 0 && p.bvh_width != 2)) {
             log_msg(st, "set_params: unsupported parameter combination");
             return Q_EXT_ERR_UNSUPPORTED;
@@ -679,7 +790,9 @@ Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out
     s.rays_extension = c.rays_extension;
     s.rays_shadow = c.rays_shadow;
     s.bvh_build_ms = st->bvh_build_ms;
-    s.render_ms = st->kernel_ms[K_MEGAKERNEL];
+    s.render_ms = 0;
+    for (uint32_t k = 0; k < K_COUNT; ++k)
+        if (kRenderKernel[k]) s.render_ms += st->kernel_ms[k];
     s.num_kernels = K_COUNT;
     for (uint32_t k = 0; k < K_COUNT; ++k) {
         std::snprintf(s.kernels[k].name, sizeof s.kernels[k].name, "%s", kKernelNames[k]);

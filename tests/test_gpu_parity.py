@@ -51,15 +51,20 @@ def cam_of(mesh):
     return make_camera(c["position"], c["target"], c["up"], c["fov"])
 
 
-def test_reference_mode_default_scene():
+ENGINES = [abi.Q_ENGINE_MEGAKERNEL, abi.Q_ENGINE_WAVEFRONT]
+ENGINE_IDS = ["megakernel", "wavefront"]
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+def test_reference_mode_default_scene(engine):
     """The plugin's own default scene and defaults (plugin.mm:556-592): 1 spp
     per Q_plugin_render, reference integrator and seeding, depth 5."""
     w, h, frames = 128, 128, 16
-    say(f"[ref-default] {w}x{h} frames={frames}")
+    say(f"[ref-default] engine={engine} {w}x{h} frames={frames}")
     with HipRenderer(w, h) as r:
         p = r.get_params()
         assert (p.spp_per_render, p.max_depth, p.rr_start, p.rng_mode, p.integrator) == (1, 5, 3, 0, 0)
-        r.set_params(flags=abi.Q_FLAG_RECORD_IDS, id_sample=0)
+        r.set_params(flags=abi.Q_FLAG_RECORD_IDS, id_sample=0, engine=engine)
         r.render(frames)
         mesh = scenes.cornell_ref()
         params = dict(width=w, height=h, rng_mode=0, camera=mesh.camera)
@@ -73,16 +78,17 @@ def test_reference_mode_default_scene():
         assert st.rays_shadow == 0
 
 
-def test_nee_cornell36_config():
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+def test_nee_cornell36_config(engine):
     """36-triangle Cornell-style scene, 256x256, 16 spp, depth 4, NEE, hashed seeding."""
     w = h = 256
     mesh = scenes.cornell36(0)
-    say(f"[nee-cornell36] {w}x{h} spp=16")
+    say(f"[nee-cornell36] engine={engine} {w}x{h} spp=16")
     with HipRenderer(w, h) as r:
         r.set_scene(mesh)
         r.camera = cam_of(mesh)
         r.set_params(spp_per_render=16, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=0,
-                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=0)
+                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=0, engine=engine)
         r.render(1)
         params = dict(width=w, height=h, rng_mode=1, seed=0, camera=mesh.camera)
         oscene, ref = oracle_render(mesh, w, h, 16, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=0)
@@ -94,17 +100,18 @@ def test_nee_cornell36_config():
         assert abs(st.rays_shadow - c.shadow_rays) <= 1e-3 * c.shadow_rays
 
 
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
 @pytest.mark.parametrize("ntri,w,h,spp", [(200_000, 160, 90, 4)])
-def test_nee_soup_bvh(ntri, w, h, spp):
+def test_nee_soup_bvh(ntri, w, h, spp, engine):
     """Triangle soup (the 1M-triangle workload's generator at reduced size),
     depth 8 with Russian roulette from bounce 3: exercises the BVH."""
     mesh = scenes.soup(ntri, seed=1)
-    say(f"[nee-soup] ntri={ntri} {w}x{h} spp={spp}")
+    say(f"[nee-soup] engine={engine} ntri={ntri} {w}x{h} spp={spp}")
     with HipRenderer(w, h) as r:
         r.set_scene(mesh)
         r.camera = cam_of(mesh)
         r.set_params(spp_per_render=spp, max_depth=8, rr_start=3, rng_mode=1, integrator=1, seed=0,
-                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=0)
+                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=0, engine=engine)
         r.render(1)
         params = dict(width=w, height=h, rng_mode=1, seed=0, camera=mesh.camera)
         oscene, ref = oracle_render(mesh, w, h, spp, max_depth=8, rr_start=3, rng_mode=1, integrator=1, seed=0)
@@ -159,3 +166,48 @@ def test_camera_dirty_resets_and_resize():
         img = r.readback()
         assert img.shape == (24, 32, 4)
         assert r.stats().sample_count == 1
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_wavefront_matches_megakernel(integrator):
+    """Same samples through both engines: FP32 images within 1e-6 relative L1,
+    identical primary ids and ray counts (except rare rounding-order divergence)."""
+    mesh = scenes.soup(100_000, seed=3)
+    w, h, spp = 128, 72, 6
+    say(f"[wave-vs-mega] integrator={integrator}")
+    out = {}
+    for engine in ENGINES:
+        with HipRenderer(w, h) as r:
+            r.set_scene(mesh)
+            r.camera = cam_of(mesh)
+            r.set_params(spp_per_render=spp, max_depth=8, rr_start=3, rng_mode=1, integrator=integrator, seed=2,
+                         flags=abi.Q_FLAG_RECORD_IDS, id_sample=3, engine=engine, samples_per_pass=2)
+            r.render(1)
+            out[engine] = (r.readback_aov(), r.primary_ids(), r.stats())
+    (am, im, sm), (aw, iw, sw) = out[ENGINES[0]], out[ENGINES[1]]
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert rel_l1(aw[k], am[k]) <= 1e-6, k
+    assert (im == iw).mean() >= 0.9999
+    assert sm.rays_camera == sw.rays_camera == w * h * spp
+    assert abs(int(sm.rays_extension) - int(sw.rays_extension)) <= 1e-4 * sm.rays_extension
+    assert abs(int(sm.rays_shadow) - int(sw.rays_shadow)) <= 1e-4 * max(1, sm.rays_shadow)
+
+
+def test_wavefront_pass_split_and_determinism():
+    """The wavefront result does not depend on how the samples are split into
+    passes (up to FP32 summation order) and repeats bit for bit."""
+    mesh = scenes.cornell36(0)
+    w, h = 96, 64
+    say("[wave-passes]")
+    imgs = []
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        for per_pass in (1, 3, 8, 8):
+            r.set_params(spp_per_render=8, max_depth=4, rng_mode=1, integrator=1, engine=abi.Q_ENGINE_WAVEFRONT,
+                         samples_per_pass=per_pass)
+            r.render(1)
+            imgs.append(r.readback())
+    assert np.array_equal(imgs[2], imgs[3])
+    for im in imgs[:2]:
+        assert np.allclose(im, imgs[2], rtol=2e-6, atol=1e-6)
