@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--spp-per-launch", type=int, default=8)
     ap.add_argument("--engine", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--samples-per-pass", type=int, default=0)
+    ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=4)
+    ap.add_argument("--block", type=int, choices=[64, 128, 256], default=256)
+    ap.add_argument("--chunk", type=int, choices=[1, 64, 256], default=64)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -109,7 +112,8 @@ def main():
         engine = abi.Q_ENGINE_WAVEFRONT if args.engine == "wavefront" else abi.Q_ENGINE_MEGAKERNEL
         r.set_params(spp_per_render=args.spp_per_launch, max_depth=args.depth, rr_start=3, rng_mode=1,
                      integrator=1, seed=0, flags=abi.Q_FLAG_TIME_KERNELS, engine=engine,
-                     samples_per_pass=args.samples_per_pass)
+                     samples_per_pass=args.samples_per_pass, bvh_width=args.bvh_width, block_size=args.block,
+                     queue_chunk=args.chunk)
         st = r.stats()
         log(f"BVH: {st.bvh_nodes} nodes, depth {st.bvh_depth}, build+upload {st.bvh_build_ms:.0f} ms "
             f"(set_scene {build_s:.2f} s)")
@@ -171,7 +175,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain wavefront-config soup: {mesh.num_triangles} triangles, {W}x{H}, "
                                f"{args.spp} spp, depth {args.depth}, RR from 3, NEE",
-                   "engine": args.engine, "spp_per_launch": args.spp_per_launch, "bvh": "binned-SAH BVH2"},
+                   "engine": args.engine, "spp_per_launch": args.spp_per_launch,
+                   "bvh": f"binned-SAH BVH{args.bvh_width if args.engine == 'wavefront' else 2}",
+                   "trace_block": args.block, "queue_chunk": args.chunk},
         "rays": rays,
         "rays_per_path": round(total_rays / (W * H * args.spp * args.steps), 4),
         "bvh_build_ms": round(st.bvh_build_ms, 1),

@@ -82,11 +82,13 @@ typedef struct Q_render_params {
     uint32_t rng_mode;         /* Q_RNG_* */
     uint32_t integrator;       /* Q_INTEGRATOR_* */
     uint32_t engine;           /* Q_ENGINE_* */
-    uint32_t bvh_width;        /* 2 (only value accepted so far) */
+    uint32_t bvh_width;        /* wavefront traversal: 2 or 4 (0 = 2); the megakernel walks BVH2 */
     uint32_t samples_per_pass; /* wavefront: samples per pixel in flight at once (0 = auto) */
-    uint32_t block_size;       /* threads per block for the path kernels (0 = auto) */
+    uint32_t block_size;       /* wavefront traversal threads per block: 64, 128, 256 (0 = 256) */
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
+    uint32_t queue_chunk;      /* wavefront: rays a wave takes per queue access, 64 or 256 (0 = 64);
+                                  1 = each lane fetches its next ray as soon as it is idle */
 } Q_render_params;
 
 #define Q_STATS_MAX_KERNELS 16

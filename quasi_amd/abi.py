@@ -86,7 +86,7 @@ class Q_scene_desc(C.Structure):
 class Q_render_params(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "struct_size", "spp_per_render", "max_depth", "rr_start", "seed", "rng_mode", "integrator", "engine",
-        "bvh_width", "samples_per_pass", "block_size", "flags", "id_sample")]
+        "bvh_width", "samples_per_pass", "block_size", "flags", "id_sample", "queue_chunk")]
 
 
 class Q_kernel_stat(C.Structure):

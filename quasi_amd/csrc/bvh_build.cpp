@@ -250,16 +250,19 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
     b.prepare(threads);
     b.build(threads);
 
-    // Depth-first conversion to pair nodes.  A root that is itself a leaf gets
-    // a pair node whose right child is an empty box (never entered).
+    // Depth-first conversion to pair nodes.
     const TmpNode& root = b.nodes[0];
     res.order.reserve(ntri);
     std::vector<float>& out = res.nodes;
-    auto encode_leaf = [&](const TmpNode& lf) -> int32_t {
+    // leaf reference of every temporary leaf node, shared by both layouts
+    std::vector<int32_t> leaf_ref(b.next_node.load(), 0);
+    auto encode_leaf = [&](uint32_t tmp) -> int32_t {
+        const TmpNode& lf = b.nodes[tmp];
         uint32_t first = uint32_t(res.order.size());
         for (uint32_t i = 0; i < lf.count; ++i) res.order.push_back(b.prims[lf.first + i]);
         res.num_leaves++;
-        return ~int32_t((first << 2) | (lf.count - 1));
+        leaf_ref[tmp] = ~int32_t((first << 2) | (lf.count - 1));
+        return leaf_ref[tmp];
     };
     auto write_node = [&](uint32_t idx, const Box& L, const Box& R, int32_t cl, int32_t cr) {
         float* n = &out[16 * size_t(idx)];
@@ -284,6 +287,18 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
         res.num_leaves = n > 1 ? 2 : 1;
         res.depth = 1;
         res.sah_cost = 1.0 + n;
+        res.nodes4.assign(32, 0.f);
+        const int32_t refs[4] = {cl, n > 1 ? cr : kEmptyChild, kEmptyChild, kEmptyChild};
+        for (int k = 0; k < 4; ++k) {
+            const bool used = refs[k] != kEmptyChild;
+            for (int a = 0; a < 3; ++a) {
+                res.nodes4[8 * a + k] = used ? root.box.lo[a] : 0.f;
+                res.nodes4[8 * a + 4 + k] = used ? root.box.hi[a] : 0.f;
+            }
+            std::memcpy(&res.nodes4[24 + k], &refs[k], 4);
+        }
+        res.num_nodes4 = 1;
+        res.depth4 = 1;
         return res;
     }
     // Count internal nodes for allocation.
@@ -319,14 +334,63 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
         const TmpNode& L = b.nodes[nd.left];
         const TmpNode& R = b.nodes[nd.right];
         int32_t cl = 0, cr = 0;
-        if (L.count) { cl = encode_leaf(L); cost += L.box.area() / root_area * L.count; }
-        if (R.count) { cr = encode_leaf(R); cost += R.box.area() / root_area * R.count; }
+        if (L.count) { cl = encode_leaf(nd.left); cost += L.box.area() / root_area * L.count; }
+        if (R.count) { cr = encode_leaf(nd.right); cost += R.box.area() / root_area * R.count; }
         write_node(idx, L.box, R.box, cl, cr);
         if (!R.count) st.push_back({nd.right, int64_t(idx), 1, it.depth + 1});
         if (!L.count) st.push_back({nd.left, int64_t(idx), 0, it.depth + 1});
     }
     res.num_nodes = internal;
     res.sah_cost = cost;
+
+    // BVH4: collapse the binary tree.  Each 4-wide node takes its binary
+    // node's two children and repeatedly opens the internal child with the
+    // largest box until it has four children (or only leaves remain).  Leaves
+    // keep the references assigned above, so both layouts share `order`.
+    struct Walk4 { uint32_t tmp; int64_t parent; uint32_t slot; uint32_t depth; };
+    std::vector<Walk4> st4{{0, -1, 0, 1}};
+    std::vector<float>& o4 = res.nodes4;
+    o4.reserve(size_t(internal) * 32 / 2);
+    uint32_t next4 = 0;
+    while (!st4.empty()) {
+        Walk4 it = st4.back(); st4.pop_back();
+        const uint32_t idx = next4++;
+        o4.resize(32 * size_t(next4), 0.f);
+        if (it.parent >= 0) {
+            int32_t c = int32_t(idx);
+            std::memcpy(&o4[32 * size_t(it.parent) + 24 + it.slot], &c, 4);
+        }
+        res.depth4 = std::max(res.depth4, it.depth);
+        uint32_t kids[4] = {b.nodes[it.tmp].left, b.nodes[it.tmp].right, 0, 0};
+        uint32_t nk = 2;
+        while (nk < 4) {
+            int pick = -1;
+            double best_area = -1.0;
+            for (uint32_t k = 0; k < nk; ++k) {
+                const TmpNode& c = b.nodes[kids[k]];
+                if (c.count == 0 && c.box.area() > best_area) { best_area = c.box.area(); pick = int(k); }
+            }
+            if (pick < 0) break;
+            const TmpNode& c = b.nodes[kids[pick]];
+            for (uint32_t k = nk; k > uint32_t(pick) + 1; --k) kids[k] = kids[k - 1];
+            kids[pick + 1] = c.right;
+            kids[pick] = c.left;
+            ++nk;
+        }
+        float* n = &o4[32 * size_t(idx)];
+        for (uint32_t k = 0; k < 4; ++k) {
+            int32_t ref = kEmptyChild;
+            if (k < nk) {
+                const TmpNode& c = b.nodes[kids[k]];
+                for (int a = 0; a < 3; ++a) { n[8 * a + k] = c.box.lo[a]; n[8 * a + 4 + k] = c.box.hi[a]; }
+                ref = c.count ? leaf_ref[kids[k]] : 0;
+            }
+            std::memcpy(&n[24 + k], &ref, 4);
+        }
+        for (int k = int(nk) - 1; k >= 0; --k)
+            if (b.nodes[kids[k]].count == 0) st4.push_back({kids[k], int64_t(idx), uint32_t(k), it.depth + 1});
+    }
+    res.num_nodes4 = next4;
     return res;
 }
 

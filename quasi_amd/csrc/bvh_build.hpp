@@ -2,8 +2,9 @@
 //
 // No reference counterpart: the reference intersects at most 32 quads by brute
 // force (backends/metal/plugin.mm:199-212).  32 centroid bins, at most 4
-// triangles per leaf, output in the 64-byte pair-node layout described in
-// common.hpp.  Deterministic for a given input whatever the thread count.
+// triangles per leaf, output in the 64-byte pair-node layout and the 128-byte
+// 4-wide layout described in common.hpp.  Deterministic for a given input
+// whatever the thread count.
 #pragma once
 
 #include <cstdint>
@@ -18,6 +19,10 @@ struct BvhBuildResult {
     uint32_t num_leaves = 0;
     uint32_t depth = 0;             // deepest internal node, root = 1
     double sah_cost = 0;            // SAH cost of the tree (traversal cost 1, triangle cost 1)
+    // 4-wide layout over the same leaves and triangle order (see common.hpp)
+    std::vector<float> nodes4;      // 32 floats (8 float4) per node
+    uint32_t num_nodes4 = 0;
+    uint32_t depth4 = 0;
 };
 
 // p0/p1/p2: vertex positions of each triangle, 3 floats each, in input order.

@@ -17,6 +17,14 @@
 //              [3] = (child L, child R, 0, 0) as int bits
 //            child >= 0: internal node index; child < 0: leaf, ~child =
 //            (first triangle << 2) | (count - 1), count 1..4.
+//   nodes4 : BVH4 nodes, 8 float4 (128 B) per node, boxes stored per component
+//            so the four slab tests vectorise:
+//              [0] = lo.x of children 0..3   [1] = hi.x of children 0..3
+//              [2] = lo.y                    [3] = hi.y
+//              [4] = lo.z                    [5] = hi.z
+//              [6] = child references (int bits, same encoding as BVH2;
+//                    kEmptyChild marks an unused slot)
+//              [7] = unused
 //   lights : one LightEntry per emissive triangle
 #pragma once
 
@@ -27,6 +35,7 @@ namespace quasi_hip {
 constexpr int kLeafMax = 4;
 constexpr int kSahBins = 32;
 constexpr uint32_t kMissId = 0xFFFFFFFFu;
+constexpr int32_t kEmptyChild = 0x7FFFFFFF;
 
 struct LightEntry {
     uint32_t tri;    // BVH-order triangle index
@@ -92,6 +101,23 @@ struct PassParams {
     uint32_t num_lights;
     uint32_t num_nodes;
     uint32_t record_ids, id_sample;
+    // 4-wide BVH over the same triangles, and the traversal-stack overflow
+    // area of the persistent kernels (gstack_entries words per thread,
+    // lane-interleaved; sized by the host for the deepest possible stack)
+    const void* nodes4;
+    uint32_t num_nodes4;
+    uint32_t gstack_entries;
+    int* gstack;
 };
+
+// Launch shape of the persistent traversal kernels (autotuned).
+struct TraceConfig {
+    uint32_t width;   // BVH width: 2 or 4
+    uint32_t block;   // threads per block: 64, 128 or 256
+    uint32_t chunk;   // rays fetched per wave per queue access: 64 or 256; 1 = per-lane dynamic fetch
+};
+
+constexpr int kShortStack = 16;       // LDS entries per lane in the persistent kernels
+constexpr uint32_t kMaxThreadsPerCU = 2048;
 
 }  // namespace quasi_hip

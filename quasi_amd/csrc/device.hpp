@@ -93,18 +93,86 @@ struct TraceHit {
 // few ulps).
 constexpr float kSlabWiden = 1.000001f;
 
-// Stack-based closest-hit traversal of the pair-node BVH2.  The per-lane stack
-// lives in LDS, interleaved by lane (entry i of lane l at stk[i * STRIDE]), so a
-// wave's pushes and pops hit 64 distinct banks.  Children are visited near
-// first.  On equal t the larger input index wins (the reference's linear loop,
-// plugin.mm:204-209 with the inclusive `t > t_max` test of plugin.mm:177).
+// Traversal stacks.  Entry i of a lane lives at base[i * stride], so the 64
+// lanes of a wave touch 64 consecutive words (distinct LDS banks, coalesced
+// global lines).
+//
+// LdsStack: the whole stack in LDS (bounded by the BVH depth).
 template <int STRIDE>
-__device__ __forceinline__ TraceHit trace_closest(const float4* __restrict__ nodes,
-                                                  const float4* __restrict__ tris, Ray r, float tmin,
-                                                  float tmax, int* __restrict__ stk) {
+struct LdsStack {
+    int* lds;
+    int sp;
+    __device__ __forceinline__ void push(int v) { lds[sp * STRIDE] = v; ++sp; }
+    __device__ __forceinline__ int pop() { --sp; return lds[sp * STRIDE]; }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+};
+
+// ShortStack: the top LDS_N entries in LDS, deeper entries in a per-thread
+// global overflow area (lane-interleaved with stride gstride).  Keeps the LDS
+// footprint small enough for full occupancy; the overflow is rarely touched.
+template <int LDS_N, int STRIDE>
+struct ShortStack {
+    int* lds;
+    int* glob;
+    uint32_t gstride;
+    int sp;
+    __device__ __forceinline__ void push(int v) {
+        if (sp < LDS_N) lds[sp * STRIDE] = v;
+        else glob[size_t(sp - LDS_N) * gstride] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop() {
+        --sp;
+        return sp < LDS_N ? lds[sp * STRIDE] : glob[size_t(sp - LDS_N) * gstride];
+    }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+};
+
+// Moller-Trumbore against BVH-order triangle i; true with t when the hit lies
+// in [tmin, tmax].  Inclusive edges.
+__device__ __forceinline__ bool hit_triangle(const float4* __restrict__ tris, int i, const Ray& r, float tmin,
+                                             float tmax, float& t_out, uint32_t& id) {
+    const float4 t0 = tris[3 * i + 0];
+    const float4 t1 = tris[3 * i + 1];
+    const float4 t2 = tris[3 * i + 2];
+    const f3 e1 = xyz(t1), e2 = xyz(t2);
+    const f3 pv = cross(r.d, e2);
+    const float det = dot(e1, pv);
+    if (det == 0.0f) return false;
+    const float idet = 1.0f / det;
+    const f3 s = r.o - xyz(t0);
+    const float u = dot(s, pv) * idet;
+    if (u < 0.0f || u > 1.0f) return false;
+    const f3 q = cross(s, e1);
+    const float vv = dot(r.d, q) * idet;
+    if (vv < 0.0f || u + vv > 1.0f) return false;
+    const float t = dot(e2, q) * idet;
+    if (t < tmin || t > tmax) return false;
+    t_out = t;
+    id = __float_as_uint(t0.w);
+    return true;
+}
+
+// Slab distances of one box: returns (t_near, widened t_far) clipped to
+// [tmin, tmax].
+__device__ __forceinline__ void slab(float lx, float hx, float ly, float hy, float lz, float hz, const Ray& r,
+                                     const f3& inv, float tmin, float tmax, float& tn, float& tf) {
+    const float x0 = (lx - r.o.x) * inv.x, x1 = (hx - r.o.x) * inv.x;
+    const float y0 = (ly - r.o.y) * inv.y, y1 = (hy - r.o.y) * inv.y;
+    const float z0 = (lz - r.o.z) * inv.z, z1 = (hz - r.o.z) * inv.z;
+    tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * kSlabWiden;
+}
+
+// Closest hit over the pair-node BVH2.  Children are visited near first.  On
+// equal t the larger input index wins (the reference's linear loop,
+// plugin.mm:204-209 with the inclusive `t > t_max` test of plugin.mm:177).
+// ANY: return at the first hit in [tmin, tmax] (shadow rays).
+template <bool ANY, class Stack>
+__device__ __forceinline__ TraceHit traverse2(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                              const Ray& r, float tmin, float tmax, Stack& st) {
     TraceHit best{tmax, -1, 0u};
     const f3 inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
-    int sp = 0;
     int node = 0;
     for (;;) {
         while (node >= 0) {
@@ -114,137 +182,230 @@ __device__ __forceinline__ TraceHit trace_closest(const float4* __restrict__ nod
             const float4 dd = nodes[4 * node + 3];
             const int cl = __float_as_int(dd.x);
             const int cr = __float_as_int(dd.y);
-            // left child
-            float lx0 = (a.x - r.o.x) * inv.x, lx1 = (a.y - r.o.x) * inv.x;
-            float ly0 = (a.z - r.o.y) * inv.y, ly1 = (a.w - r.o.y) * inv.y;
-            float lz0 = (c.x - r.o.z) * inv.z, lz1 = (c.y - r.o.z) * inv.z;
-            float rx0 = (b.x - r.o.x) * inv.x, rx1 = (b.y - r.o.x) * inv.x;
-            float ry0 = (b.z - r.o.y) * inv.y, ry1 = (b.w - r.o.y) * inv.y;
-            float rz0 = (c.z - r.o.z) * inv.z, rz1 = (c.w - r.o.z) * inv.z;
-            float tnl = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
-            float tfl = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), best.t)) * kSlabWiden;
-            float tnr = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
-            float tfr = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), best.t)) * kSlabWiden;
+            float tnl, tfl, tnr, tfr;
+            slab(a.x, a.y, a.z, a.w, c.x, c.y, r, inv, tmin, best.t, tnl, tfl);
+            slab(b.x, b.y, b.z, b.w, c.z, c.w, r, inv, tmin, best.t, tnr, tfr);
             const bool hl = tnl <= tfl;
             const bool hr = tnr <= tfr;
             if (hl && hr) {
                 const bool left_first = tnl <= tnr;
-                stk[sp * STRIDE] = left_first ? cr : cl;
-                ++sp;
+                st.push(left_first ? cr : cl);
                 node = left_first ? cl : cr;
             } else if (hl) {
                 node = cl;
             } else if (hr) {
                 node = cr;
             } else {
-                if (sp == 0) return best;
-                --sp;
-                node = stk[sp * STRIDE];
+                if (st.empty()) return best;
+                node = st.pop();
             }
         }
-        // leaf
         const uint32_t v = ~uint32_t(node);
         const int first = int(v >> 2);
         const int count = int(v & 3u) + 1;
         for (int i = first; i < first + count; ++i) {
-            const float4 t0 = tris[3 * i + 0];
-            const float4 t1 = tris[3 * i + 1];
-            const float4 t2 = tris[3 * i + 2];
-            const f3 e1 = xyz(t1), e2 = xyz(t2);
-            const f3 pv = cross(r.d, e2);
-            const float det = dot(e1, pv);
-            if (det == 0.0f) continue;
-            const float idet = 1.0f / det;
-            const f3 s = r.o - xyz(t0);
-            const float u = dot(s, pv) * idet;
-            if (u < 0.0f || u > 1.0f) continue;
-            const f3 q = cross(s, e1);
-            const float vv = dot(r.d, q) * idet;
-            if (vv < 0.0f || u + vv > 1.0f) continue;
-            const float t = dot(e2, q) * idet;
-            if (t < tmin || t > best.t) continue;
-            const uint32_t id = __float_as_uint(t0.w);
+            float t;
+            uint32_t id;
+            if (!hit_triangle(tris, i, r, tmin, best.t, t, id)) continue;
+            if (ANY) { best.t = t; best.tri = i; best.id = id; return best; }
             if (best.tri < 0 || t < best.t || id > best.id) {
                 best.t = t;
                 best.tri = i;
                 best.id = id;
             }
         }
-        if (sp == 0) return best;
-        --sp;
-        node = stk[sp * STRIDE];
+        if (st.empty()) return best;
+        node = st.pop();
     }
 }
 
-// Any-hit traversal for shadow rays over [tmin, tmax]: returns at the first
-// intersection found.
-template <int STRIDE>
-__device__ __forceinline__ bool trace_any(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          Ray r, float tmin, float tmax, int* __restrict__ stk) {
+// Compare-exchange on (key, child) pairs, smaller key first.
+__device__ __forceinline__ void cswap(float& ka, int& ca, float& kb, int& cb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    const int c = sw ? cb : ca;
+    kb = sw ? ka : kb;
+    cb = sw ? ca : cb;
+    ka = k;
+    ca = c;
+}
+
+// Closest (or any) hit over the 4-wide BVH.  The four slab tests are
+// independent; hit children are ordered by entry distance with a 5-comparator
+// sorting network, the nearest is visited next and the others are pushed far
+// to near.  Same tie rule as traverse2.
+template <bool ANY, class Stack>
+__device__ __forceinline__ TraceHit traverse4(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                              const Ray& r, float tmin, float tmax, Stack& st) {
+    TraceHit best{tmax, -1, 0u};
     const f3 inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
-    int sp = 0;
+    constexpr float kMissKey = 3.0e38f;
     int node = 0;
     for (;;) {
         while (node >= 0) {
-            const float4 a = nodes[4 * node + 0];
-            const float4 b = nodes[4 * node + 1];
-            const float4 c = nodes[4 * node + 2];
-            const float4 dd = nodes[4 * node + 3];
-            const int cl = __float_as_int(dd.x);
-            const int cr = __float_as_int(dd.y);
-            float lx0 = (a.x - r.o.x) * inv.x, lx1 = (a.y - r.o.x) * inv.x;
-            float ly0 = (a.z - r.o.y) * inv.y, ly1 = (a.w - r.o.y) * inv.y;
-            float lz0 = (c.x - r.o.z) * inv.z, lz1 = (c.y - r.o.z) * inv.z;
-            float rx0 = (b.x - r.o.x) * inv.x, rx1 = (b.y - r.o.x) * inv.x;
-            float ry0 = (b.z - r.o.y) * inv.y, ry1 = (b.w - r.o.y) * inv.y;
-            float rz0 = (c.z - r.o.z) * inv.z, rz1 = (c.w - r.o.z) * inv.z;
-            float tnl = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), tmin));
-            float tfl = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax)) * kSlabWiden;
-            float tnr = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), tmin));
-            float tfr = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax)) * kSlabWiden;
-            const bool hl = tnl <= tfl;
-            const bool hr = tnr <= tfr;
-            if (hl && hr) {
-                const bool left_first = tnl <= tnr;
-                stk[sp * STRIDE] = left_first ? cr : cl;
-                ++sp;
-                node = left_first ? cl : cr;
-            } else if (hl) {
-                node = cl;
-            } else if (hr) {
-                node = cr;
-            } else {
-                if (sp == 0) return false;
-                --sp;
-                node = stk[sp * STRIDE];
+            const float4* n = nodes + 8 * node;
+            const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+            const int4 ch = *reinterpret_cast<const int4*>(n + 6);
+            float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
+            slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, inv, tmin, best.t, t0n, t0f);
+            slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, inv, tmin, best.t, t1n, t1f);
+            slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, inv, tmin, best.t, t2n, t2f);
+            slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, inv, tmin, best.t, t3n, t3f);
+            float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
+            float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
+            float k2 = (t2n <= t2f && ch.z != kEmptyChild) ? t2n : kMissKey;
+            float k3 = (t3n <= t3f && ch.w != kEmptyChild) ? t3n : kMissKey;
+            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+            cswap(k0, c0, k1, c1);
+            cswap(k2, c2, k3, c3);
+            cswap(k0, c0, k2, c2);
+            cswap(k1, c1, k3, c3);
+            cswap(k1, c1, k2, c2);
+            if (k0 == kMissKey) {
+                if (st.empty()) return best;
+                node = st.pop();
+                continue;
             }
+            if (k3 != kMissKey) st.push(c3);
+            if (k2 != kMissKey) st.push(c2);
+            if (k1 != kMissKey) st.push(c1);
+            node = c0;
         }
         const uint32_t v = ~uint32_t(node);
         const int first = int(v >> 2);
         const int count = int(v & 3u) + 1;
         for (int i = first; i < first + count; ++i) {
-            const float4 t0 = tris[3 * i + 0];
-            const float4 t1 = tris[3 * i + 1];
-            const float4 t2 = tris[3 * i + 2];
-            const f3 e1 = xyz(t1), e2 = xyz(t2);
-            const f3 pv = cross(r.d, e2);
-            const float det = dot(e1, pv);
-            if (det == 0.0f) continue;
-            const float idet = 1.0f / det;
-            const f3 s = r.o - xyz(t0);
-            const float u = dot(s, pv) * idet;
-            if (u < 0.0f || u > 1.0f) continue;
-            const f3 q = cross(s, e1);
-            const float vv = dot(r.d, q) * idet;
-            if (vv < 0.0f || u + vv > 1.0f) continue;
-            const float t = dot(e2, q) * idet;
-            if (t < tmin || t > tmax) continue;
-            return true;
+            float t;
+            uint32_t id;
+            if (!hit_triangle(tris, i, r, tmin, best.t, t, id)) continue;
+            if (ANY) { best.t = t; best.tri = i; best.id = id; return best; }
+            if (best.tri < 0 || t < best.t || id > best.id) {
+                best.t = t;
+                best.tri = i;
+                best.id = id;
+            }
         }
-        if (sp == 0) return false;
-        --sp;
-        node = stk[sp * STRIDE];
+        if (st.empty()) return best;
+        node = st.pop();
     }
+}
+
+// Resumable traversal for the dynamic-fetch kernels: one call of trav_step
+// processes one node (internal or leaf) of the lane's current ray, so a wave
+// can interleave rays that started at different times.  Same visit order and
+// tie rule as traverse2 / traverse4.
+struct TravState {
+    Ray r;
+    f3 inv;
+    float tmin;
+    float best_t;
+    int best_tri;
+    uint32_t best_id;
+    int node;
+};
+
+__device__ __forceinline__ void trav_begin(TravState& s, const Ray& r, float tmin, float tmax) {
+    s.r = r;
+    s.inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+    s.tmin = tmin;
+    s.best_t = tmax;
+    s.best_tri = -1;
+    s.best_id = 0u;
+    s.node = 0;
+}
+
+// Returns true when the ray's traversal has finished (result in s.best_*).
+template <int WIDTH, bool ANY, class Stack>
+__device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          TravState& s, Stack& st) {
+    if (s.node >= 0) {
+        if (WIDTH == 4) {
+            const float4* n = nodes + 8 * s.node;
+            const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+            const int4 ch = *reinterpret_cast<const int4*>(n + 6);
+            float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
+            slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.r, s.inv, s.tmin, s.best_t, t0n, t0f);
+            slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, s.r, s.inv, s.tmin, s.best_t, t1n, t1f);
+            slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, s.r, s.inv, s.tmin, s.best_t, t2n, t2f);
+            slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, s.r, s.inv, s.tmin, s.best_t, t3n, t3f);
+            constexpr float kMissKey = 3.0e38f;
+            float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
+            float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
+            float k2 = (t2n <= t2f && ch.z != kEmptyChild) ? t2n : kMissKey;
+            float k3 = (t3n <= t3f && ch.w != kEmptyChild) ? t3n : kMissKey;
+            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+            cswap(k0, c0, k1, c1);
+            cswap(k2, c2, k3, c3);
+            cswap(k0, c0, k2, c2);
+            cswap(k1, c1, k3, c3);
+            cswap(k1, c1, k2, c2);
+            if (k0 == kMissKey) {
+                if (st.empty()) return true;
+                s.node = st.pop();
+                return false;
+            }
+            if (k3 != kMissKey) st.push(c3);
+            if (k2 != kMissKey) st.push(c2);
+            if (k1 != kMissKey) st.push(c1);
+            s.node = c0;
+            return false;
+        } else {
+            const float4 a = nodes[4 * s.node + 0];
+            const float4 b = nodes[4 * s.node + 1];
+            const float4 c = nodes[4 * s.node + 2];
+            const float4 dd = nodes[4 * s.node + 3];
+            const int cl = __float_as_int(dd.x);
+            const int cr = __float_as_int(dd.y);
+            float tnl, tfl, tnr, tfr;
+            slab(a.x, a.y, a.z, a.w, c.x, c.y, s.r, s.inv, s.tmin, s.best_t, tnl, tfl);
+            slab(b.x, b.y, b.z, b.w, c.z, c.w, s.r, s.inv, s.tmin, s.best_t, tnr, tfr);
+            const bool hl = tnl <= tfl;
+            const bool hr = tnr <= tfr;
+            if (hl && hr) {
+                const bool left_first = tnl <= tnr;
+                st.push(left_first ? cr : cl);
+                s.node = left_first ? cl : cr;
+            } else if (hl || hr) {
+                s.node = hl ? cl : cr;
+            } else {
+                if (st.empty()) return true;
+                s.node = st.pop();
+            }
+            return false;
+        }
+    }
+    const uint32_t v = ~uint32_t(s.node);
+    const int first = int(v >> 2);
+    const int count = int(v & 3u) + 1;
+    for (int i = first; i < first + count; ++i) {
+        float t;
+        uint32_t id;
+        if (!hit_triangle(tris, i, s.r, s.tmin, s.best_t, t, id)) continue;
+        if (ANY) { s.best_t = t; s.best_tri = i; s.best_id = id; return true; }
+        if (s.best_tri < 0 || t < s.best_t || id > s.best_id) {
+            s.best_t = t;
+            s.best_tri = i;
+            s.best_id = id;
+        }
+    }
+    if (st.empty()) return true;
+    s.node = st.pop();
+    return false;
+}
+
+// Convenience wrappers with an LDS stack (megakernel).
+template <int STRIDE>
+__device__ __forceinline__ TraceHit trace_closest(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                                  Ray r, float tmin, float tmax, int* __restrict__ stk) {
+    LdsStack<STRIDE> st{stk, 0};
+    return traverse2<false>(nodes, tris, r, tmin, tmax, st);
+}
+
+template <int STRIDE>
+__device__ __forceinline__ bool trace_any(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          Ray r, float tmin, float tmax, int* __restrict__ stk) {
+    LdsStack<STRIDE> st{stk, 0};
+    return traverse2<true>(nodes, tris, r, tmin, tmax, st).tri >= 0;
 }
 
 // ---------------------------------------------------------------- shading
@@ -396,10 +557,12 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     return base + lanes_below(mask);
 }
 
-// Wave-uniform work fetch: lane 0 reserves the next 64 items of a queue of n.
+// Wave-uniform work fetch: lane 0 reserves the next CHUNK items of a queue
+// of n; the wave then walks them 64 at a time.
+template <int CHUNK>
 __device__ __forceinline__ bool fetch_chunk(uint32_t* cursor, uint32_t n, uint32_t& base) {
     uint32_t b = 0;
-    if ((threadIdx.x & 63u) == 0u) b = atomicAdd(cursor, 64u);
+    if ((threadIdx.x & 63u) == 0u) b = atomicAdd(cursor, uint32_t(CHUNK));
     base = __shfl(b, 0, 64);
     return base < n;
 }

@@ -34,14 +34,22 @@ struct WaveTimers {
     void (*after)(void* user, uint32_t kernel, hipStream_t stream);
 };
 
-// Blocks of the persistent extend/shadow kernels that fill the device once.
-uint32_t wavefront_persistent_blocks(int device, uint32_t bvh_depth);
+// Compiled launch shapes of the persistent traversal kernels.
+bool trace_config_supported(const TraceConfig& c);
+uint32_t trace_config_count();
+TraceConfig trace_config_at(uint32_t i);
+
+// Blocks of the persistent extend/shadow kernels that fill the device once
+// (never more than kMaxThreadsPerCU threads per CU).
+uint32_t persistent_blocks(int device, const TraceConfig& c);
 
 // One sample pass of the wavefront engine over s.paths = W*H*S paths (samples
 // p.sample_begin .. p.sample_begin + S - 1): a fixed, host-synchronisation-free
-// launch sequence.
-hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, uint32_t bvh_depth,
-                                 uint32_t persistent_blocks, hipStream_t stream, WaveTimers* timers);
+// launch sequence.  p.gstack must hold p.gstack_entries words for each of
+// trace_blocks * cfg.block threads, with kShortStack + p.gstack_entries at
+// least the deepest traversal stack of the chosen BVH width.
+hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const TraceConfig& cfg,
+                                 uint32_t trace_blocks, hipStream_t stream, WaveTimers* timers);
 
 // out[i] = sum[i] / n for npix RGBA pixels (the running mean of plugin.mm:326-327).
 hipError_t launch_resolve(const float4* sum, float4* out, size_t npix, uint32_t n, hipStream_t stream);

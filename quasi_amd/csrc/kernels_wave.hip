@@ -56,27 +56,36 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     }
 }
 
-template <int STACK, int BLOCK>
+// Persistent closest-hit traversal: each wave reserves CHUNK rays of the
+// bounce's queue at a time until the queue is drained.
+template <int WIDTH, int BLOCK, int CHUNK>
 __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uint32_t b) {
-    __shared__ int s_stack[STACK * BLOCK];
-    int* stk = s_stack + threadIdx.x;
+    __shared__ int s_stack[kShortStack * BLOCK];
+    ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
+                                      gridDim.x * BLOCK, 0};
     const uint32_t n = S.counts[b];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n)
         atomicAdd(b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension, (unsigned long long)n);
-    const float4* __restrict__ nodes = static_cast<const float4*>(P.nodes);
+    const float4* __restrict__ nodes = static_cast<const float4*>(WIDTH == 4 ? P.nodes4 : P.nodes);
     const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
     const float4* __restrict__ ro = static_cast<const float4*>(S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(S.ray_d[b & 1u]);
     uint2* hit = static_cast<uint2*>(S.hit);
     uint32_t base;
-    while (fetch_chunk(&S.fetch[2 * b], n, base)) {
-        const uint32_t i = base + lane_id();
-        if (i < n) {
-            const float4 o = ro[i], d = rd[i];
-            TraceHit h{kTFar, -1, 0u};
-            if (P.num_nodes)
-                h = trace_closest<BLOCK>(nodes, tris, Ray{xyz(o), xyz(d)}, kTMin, kTFar, stk);
-            hit[i] = make_uint2(__float_as_uint(h.t), uint32_t(h.tri));
+    while (fetch_chunk<CHUNK>(&S.fetch[2 * b], n, base)) {
+        for (uint32_t j = 0; j < uint32_t(CHUNK); j += 64) {
+            const uint32_t i = base + j + lane_id();
+            if (i < n) {
+                const float4 o = ro[i], d = rd[i];
+                TraceHit h{kTFar, -1, 0u};
+                st.sp = 0;
+                if (P.num_nodes) {
+                    const Ray ray{xyz(o), xyz(d)};
+                    h = WIDTH == 4 ? traverse4<false>(nodes, tris, ray, kTMin, kTFar, st)
+                                   : traverse2<false>(nodes, tris, ray, kTMin, kTFar, st);
+                }
+                hit[i] = make_uint2(__float_as_uint(h.t), uint32_t(h.tri));
+            }
         }
     }
 }
@@ -142,31 +151,131 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
     }
 }
 
-template <int STACK, int BLOCK>
+// Persistent any-hit traversal of the bounce's shadow queue; an unoccluded
+// ray adds its contribution to the path's radiance slot (one shadow ray per
+// path and bounce, so no two lanes write the same slot).
+template <int WIDTH, int BLOCK, int CHUNK>
 __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uint32_t b) {
-    __shared__ int s_stack[STACK * BLOCK];
-    int* stk = s_stack + threadIdx.x;
+    __shared__ int s_stack[kShortStack * BLOCK];
+    ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
+                                      gridDim.x * BLOCK, 0};
     const uint32_t n = S.shadow_counts[b];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n)
         atomicAdd(&P.counters->rays_shadow, (unsigned long long)n);
-    const float4* __restrict__ nodes = static_cast<const float4*>(P.nodes);
+    const float4* __restrict__ nodes = static_cast<const float4*>(WIDTH == 4 ? P.nodes4 : P.nodes);
     const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
     const float4* __restrict__ so_a = static_cast<const float4*>(S.sh_o);
     const float4* __restrict__ sd_a = static_cast<const float4*>(S.sh_d);
     const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
     float4* rad = static_cast<float4*>(S.radiance);
     uint32_t base;
-    while (fetch_chunk(&S.fetch[2 * b + 1], n, base)) {
-        const uint32_t k = base + lane_id();
-        if (k < n) {
-            const float4 o = so_a[k], d = sd_a[k];
-            if (!trace_any<BLOCK>(nodes, tris, Ray{xyz(o), xyz(d)}, kTMin, o.w, stk)) {
-                const uint32_t p = __float_as_uint(d.w);
-                const float4 c = sc_a[k];
-                float4 l = rad[p];
-                l.x += c.x; l.y += c.y; l.z += c.z;
-                rad[p] = l;
+    while (fetch_chunk<CHUNK>(&S.fetch[2 * b + 1], n, base)) {
+        for (uint32_t j = 0; j < uint32_t(CHUNK); j += 64) {
+            const uint32_t k = base + j + lane_id();
+            if (k < n) {
+                const float4 o = so_a[k], d = sd_a[k];
+                const Ray ray{xyz(o), xyz(d)};
+                st.sp = 0;
+                const bool occ = WIDTH == 4 ? traverse4<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0
+                                            : traverse2<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0;
+                if (!occ) {
+                    const uint32_t p = __float_as_uint(d.w);
+                    const float4 c = sc_a[k];
+                    float4 l = rad[p];
+                    l.x += c.x; l.y += c.y; l.z += c.z;
+                    rad[p] = l;
+                }
             }
+        }
+    }
+}
+
+// Persistent traversal with per-lane dynamic ray fetch: a lane whose ray has
+// finished takes the next one from the queue as soon as fewer than kRefill
+// lanes of its wave are still busy, instead of the whole wave waiting for its
+// slowest ray.  Every fetch is one atomic per wave on the bounce's cursor.
+// SHADOW selects the any-hit shadow queue; otherwise the closest-hit queue.
+constexpr uint32_t kRefill = 40;
+
+template <int WIDTH, int BLOCK, bool SHADOW>
+__global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
+    __shared__ int s_stack[kShortStack * BLOCK];
+    ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
+                                      gridDim.x * BLOCK, 0};
+    const uint32_t n = SHADOW ? S.shadow_counts[b] : S.counts[b];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n)
+        atomicAdd(SHADOW ? &P.counters->rays_shadow
+                         : (b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension),
+                  (unsigned long long)n);
+    const float4* __restrict__ nodes = static_cast<const float4*>(WIDTH == 4 ? P.nodes4 : P.nodes);
+    const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
+    const float4* __restrict__ ro = static_cast<const float4*>(SHADOW ? S.sh_o : S.ray_o[b & 1u]);
+    const float4* __restrict__ rd = static_cast<const float4*>(SHADOW ? S.sh_d : S.ray_d[b & 1u]);
+    const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
+    uint2* hit = static_cast<uint2*>(S.hit);
+    float4* rad = static_cast<float4*>(S.radiance);
+    uint32_t* cursor = &S.fetch[2 * b + (SHADOW ? 1 : 0)];
+    const bool have_bvh = P.num_nodes != 0;
+
+    TravState ts;
+    bool busy = false, exhausted = false;
+    uint32_t idx = 0;
+    float4 dray = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (;;) {
+        // refill idle lanes (wave-uniform: every lane reaches the ballot)
+        const unsigned long long need = __ballot(!busy && !exhausted);
+        if (need) {
+            uint32_t base = 0;
+            if ((threadIdx.x & 63u) == 0u) base = atomicAdd(cursor, uint32_t(__popcll(need)));
+            base = __shfl(base, 0, 64);
+            if (!busy && !exhausted) {
+                idx = base + lanes_below(need);
+                if (idx < n) {
+                    const float4 o = ro[idx];
+                    dray = rd[idx];
+                    trav_begin(ts, Ray{xyz(o), xyz(dray)}, kTMin, SHADOW ? o.w : kTFar);
+                    st.sp = 0;
+                    busy = true;
+                    if (!have_bvh) {
+                        // nothing to intersect: the ray is finished already
+                        if (SHADOW) {
+                            const uint32_t p = __float_as_uint(dray.w);
+                            const float4 c = sc_a[idx];
+                            float4 l = rad[p];
+                            l.x += c.x; l.y += c.y; l.z += c.z;
+                            rad[p] = l;
+                        } else {
+                            hit[idx] = make_uint2(__float_as_uint(kTFar), uint32_t(-1));
+                        }
+                        busy = false;
+                    }
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(busy) == 0ull) {
+            if (__ballot(!exhausted) == 0ull) break;
+            continue;
+        }
+        // traverse until too few lanes are left busy
+        for (;;) {
+            if (busy && trav_step<WIDTH, SHADOW>(nodes, tris, ts, st)) {
+                busy = false;
+                if (SHADOW) {
+                    if (ts.best_tri < 0) {
+                        const uint32_t p = __float_as_uint(dray.w);
+                        const float4 c = sc_a[idx];
+                        float4 l = rad[p];
+                        l.x += c.x; l.y += c.y; l.z += c.z;
+                        rad[p] = l;
+                    }
+                } else {
+                    hit[idx] = make_uint2(__float_as_uint(ts.best_t), uint32_t(ts.best_tri));
+                }
+            }
+            const uint32_t active = uint32_t(__popcll(__ballot(busy)));
+            if (active < kRefill) break;
         }
     }
 }
@@ -207,58 +316,128 @@ uint32_t grid_for(size_t n, uint32_t block, uint32_t cap) {
     return uint32_t(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-constexpr int kTraceBlock = 256;
 constexpr int kShadeBlock = 256;
 
-template <int STACK>
-hipError_t launch_trace_pair(const PassParams& p, const WaveState& s, uint32_t b, uint32_t persistent_blocks,
-                             uint32_t shade_blocks, bool shadows, hipStream_t stream, WaveTimers* tm) {
-    if (tm && tm->before) tm->before(tm->user, WAVE_K_EXTEND, stream);
-    hipLaunchKernelGGL((k_extend<STACK, kTraceBlock>), dim3(persistent_blocks), dim3(kTraceBlock), 0, stream, p, s, b);
-    if (tm && tm->after) tm->after(tm->user, WAVE_K_EXTEND, stream);
-    if (tm && tm->before) tm->before(tm->user, WAVE_K_SHADE, stream);
-    hipLaunchKernelGGL((k_shade<kShadeBlock>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
-    if (tm && tm->after) tm->after(tm->user, WAVE_K_SHADE, stream);
-    if (shadows) {
-        if (tm && tm->before) tm->before(tm->user, WAVE_K_SHADOW, stream);
-        hipLaunchKernelGGL((k_shadow<STACK, kTraceBlock>), dim3(persistent_blocks), dim3(kTraceBlock), 0, stream, p, s, b);
-        if (tm && tm->after) tm->after(tm->user, WAVE_K_SHADOW, stream);
+template <int WIDTH, int BLOCK, int CHUNK>
+struct TraceVariant {
+    static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        hipLaunchKernelGGL((k_extend<WIDTH, BLOCK, CHUNK>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+    static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        hipLaunchKernelGGL((k_shadow<WIDTH, BLOCK, CHUNK>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        return hipGetLastError();
+    }
+    static int occupancy() {
+        int a = 0, c = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<WIDTH, BLOCK, CHUNK>, BLOCK, 0) != hipSuccess) a = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<WIDTH, BLOCK, CHUNK>, BLOCK, 0) != hipSuccess) c = 1;
+        return a < c ? a : c;
+    }
+};
+
+// queue_chunk == 1: per-lane dynamic fetch.
+template <int WIDTH, int BLOCK>
+struct DynamicVariant {
+    static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        return hipGetLastError();
+    }
+    static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        return hipGetLastError();
+    }
+    static int occupancy() {
+        int a = 0, c = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace_dynamic<WIDTH, BLOCK, false>, BLOCK, 0) != hipSuccess) a = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_trace_dynamic<WIDTH, BLOCK, true>, BLOCK, 0) != hipSuccess) c = 1;
+        return a < c ? a : c;
+    }
+};
+
+using ExtendFn = hipError_t (*)(const PassParams&, const WaveState&, uint32_t, uint32_t, hipStream_t);
+using OccFn = int (*)();
+struct VariantEntry {
+    uint32_t width, block, chunk;
+    ExtendFn extend, shadow;
+    OccFn occupancy;
+};
+
+#define QH_VARIANT(W, B, C) {W, B, C, &TraceVariant<W, B, C>::extend, &TraceVariant<W, B, C>::shadow, \
+                             &TraceVariant<W, B, C>::occupancy}
+#define QH_DYNAMIC(W, B) {W, B, 1, &DynamicVariant<W, B>::extend, &DynamicVariant<W, B>::shadow, \
+                         &DynamicVariant<W, B>::occupancy}
+const VariantEntry kVariants[] = {
+    QH_VARIANT(2, 64, 64),  QH_VARIANT(2, 128, 64),  QH_VARIANT(2, 256, 64),
+    QH_VARIANT(2, 64, 256), QH_VARIANT(2, 128, 256), QH_VARIANT(2, 256, 256),
+    QH_VARIANT(4, 64, 64),  QH_VARIANT(4, 128, 64),  QH_VARIANT(4, 256, 64),
+    QH_VARIANT(4, 64, 256), QH_VARIANT(4, 128, 256), QH_VARIANT(4, 256, 256),
+    QH_DYNAMIC(2, 64), QH_DYNAMIC(2, 128), QH_DYNAMIC(2, 256),
+    QH_DYNAMIC(4, 64), QH_DYNAMIC(4, 128), QH_DYNAMIC(4, 256),
+};
+#undef QH_VARIANT
+#undef QH_DYNAMIC
+
+const VariantEntry* find_variant(const TraceConfig& c) {
+    for (const VariantEntry& v : kVariants)
+        if (v.width == c.width && v.block == c.block && v.chunk == c.chunk) return &v;
+    return nullptr;
 }
 
 }  // namespace
 
-uint32_t wavefront_persistent_blocks(int device, uint32_t bvh_depth) {
+bool trace_config_supported(const TraceConfig& c) { return find_variant(c) != nullptr; }
+
+uint32_t trace_config_count() { return uint32_t(sizeof(kVariants) / sizeof(kVariants[0])); }
+
+TraceConfig trace_config_at(uint32_t i) {
+    const VariantEntry& v = kVariants[i % trace_config_count()];
+    return TraceConfig{v.width, v.block, v.chunk};
+}
+
+uint32_t persistent_blocks(int device, const TraceConfig& c) {
+    const VariantEntry* v = find_variant(c);
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
-    int per_cu = 0;
-    hipError_t e = bvh_depth <= 32
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<32, kTraceBlock>, kTraceBlock, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<64, kTraceBlock>, kTraceBlock, 0);
-    if (e != hipSuccess || per_cu <= 0) per_cu = 2;
+    int per_cu = v ? v->occupancy() : 1;
+    if (per_cu <= 0) per_cu = 1;
+    // never more threads than the stack-overflow area is sized for
+    const int max_per_cu = int(kMaxThreadsPerCU / c.block);
+    if (per_cu > max_per_cu) per_cu = max_per_cu;
     return uint32_t(cus * per_cu);
 }
 
-hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, uint32_t bvh_depth,
-                                 uint32_t persistent_blocks, hipStream_t stream, WaveTimers* tm) {
+hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const TraceConfig& cfg,
+                                 uint32_t trace_blocks, hipStream_t stream, WaveTimers* tm) {
     if (s.paths == 0) return hipSuccess;
-    if (p.max_depth > kMaxDepth || bvh_depth > 64) return hipErrorInvalidValue;
+    const VariantEntry* v = find_variant(cfg);
+    if (!v || p.max_depth > kMaxDepth) return hipErrorInvalidValue;
     const uint32_t gen_blocks = grid_for(s.paths, 256, 16384);
     const uint32_t shade_blocks = grid_for(s.paths, kShadeBlock, 8192);
-    if (tm && tm->before) tm->before(tm->user, WAVE_K_RAYGEN, stream);
+    auto before = [&](uint32_t k) { if (tm && tm->before) tm->before(tm->user, k, stream); };
+    auto after = [&](uint32_t k) { if (tm && tm->after) tm->after(tm->user, k, stream); };
+    before(WAVE_K_RAYGEN);
     hipLaunchKernelGGL(k_raygen, dim3(gen_blocks), dim3(256), 0, stream, p, s);
-    if (tm && tm->after) tm->after(tm->user, WAVE_K_RAYGEN, stream);
+    after(WAVE_K_RAYGEN);
     const bool shadows = p.integrator != 0u && p.num_lights != 0u;
     for (uint32_t b = 0; b < p.max_depth; ++b) {
-        hipError_t e = bvh_depth <= 32
-            ? launch_trace_pair<32>(p, s, b, persistent_blocks, shade_blocks, shadows, stream, tm)
-            : launch_trace_pair<64>(p, s, b, persistent_blocks, shade_blocks, shadows, stream, tm);
+        before(WAVE_K_EXTEND);
+        hipError_t e = v->extend(p, s, b, trace_blocks, stream);
+        after(WAVE_K_EXTEND);
         if (e != hipSuccess) return e;
+        before(WAVE_K_SHADE);
+        hipLaunchKernelGGL((k_shade<kShadeBlock>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+        after(WAVE_K_SHADE);
+        if (shadows) {
+            before(WAVE_K_SHADOW);
+            e = v->shadow(p, s, b, trace_blocks, stream);
+            after(WAVE_K_SHADOW);
+            if (e != hipSuccess) return e;
+        }
     }
-    if (tm && tm->before) tm->before(tm->user, WAVE_K_ACCUMULATE, stream);
+    before(WAVE_K_ACCUMULATE);
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(size_t(p.width) * p.height, 256, 16384)), dim3(256), 0, stream, p, s);
-    if (tm && tm->after) tm->after(tm->user, WAVE_K_ACCUMULATE, stream);
+    after(WAVE_K_ACCUMULATE);
     return hipGetLastError();
 }
 

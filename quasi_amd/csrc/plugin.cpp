@@ -78,8 +78,9 @@ struct plugin_state {
     std::vector<float> albedo, emission;
 
     // device scene
-    DeviceBuffer tris, mats, nodes, lights;
+    DeviceBuffer tris, mats, nodes, nodes4, lights, gstack;
     uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
+    uint32_t num_nodes4 = 0, bvh_depth4 = 0, gstack_entries = 0;
     double bvh_build_ms = 0;
 
     // framebuffer
@@ -92,8 +93,10 @@ struct plugin_state {
 
     // wavefront engine
     WaveBuffers wave;
-    uint32_t persistent_blocks = 0;
+    TraceConfig trace_cfg{2, 256, 64};
+    uint32_t trace_blocks = 0;
     hipEvent_t pending_start = nullptr;
+    int num_cus = 256;
 
     // stats
     std::vector<hipEvent_t> event_pool;
@@ -204,6 +207,7 @@ Q_render_params default_params() {
     p.block_size = 0;
     p.flags = 0;
     p.id_sample = 0;
+    p.queue_chunk = 0;
     return p;
 }
 
@@ -301,9 +305,16 @@ int32_t upload_scene(plugin_state* st) {
     }
     if (!lights.empty()) lights.back().cdf = 1.0f;
 
+    // traversal-stack overflow area of the persistent kernels: the BVH2 walk
+    // keeps at most one entry per level, the BVH4 walk at most three
+    const uint32_t need = std::max(bvh.depth, 3 * bvh.depth4);
+    const uint32_t gentries = need > uint32_t(kShortStack) ? need - uint32_t(kShortStack) : 0u;
+    const size_t gthreads = size_t(st->num_cus) * kMaxThreadsPerCU;
     if (!ensure_buffer(st, st->tris, std::max<size_t>(16, tri4.size() * 4), "triangles") ||
         !ensure_buffer(st, st->mats, std::max<size_t>(16, mat4.size() * 4), "materials") ||
         !ensure_buffer(st, st->nodes, std::max<size_t>(64, bvh.nodes.size() * 4), "BVH nodes") ||
+        !ensure_buffer(st, st->nodes4, std::max<size_t>(128, bvh.nodes4.size() * 4), "BVH4 nodes") ||
+        !ensure_buffer(st, st->gstack, std::max<size_t>(4, size_t(gentries) * gthreads * 4), "traversal stacks") ||
         !ensure_buffer(st, st->lights, std::max<size_t>(16, lights.size() * sizeof(LightEntry)), "lights"))
         return Q_EXT_ERR_OUT_OF_MEMORY;
     hipStream_t s = st->stream;
@@ -311,13 +322,16 @@ int32_t upload_scene(plugin_state* st) {
         (nt && !hip_ok(st, hipMemcpy(st->tris.ptr, tri4.data(), tri4.size() * 4, hipMemcpyHostToDevice), "triangle upload")) ||
         (nt && !hip_ok(st, hipMemcpy(st->mats.ptr, mat4.data(), mat4.size() * 4, hipMemcpyHostToDevice), "material upload")) ||
         (!bvh.nodes.empty() && !hip_ok(st, hipMemcpy(st->nodes.ptr, bvh.nodes.data(), bvh.nodes.size() * 4, hipMemcpyHostToDevice), "BVH upload")) ||
+        (!bvh.nodes4.empty() && !hip_ok(st, hipMemcpy(st->nodes4.ptr, bvh.nodes4.data(), bvh.nodes4.size() * 4, hipMemcpyHostToDevice), "BVH4 upload")) ||
         (!lights.empty() && !hip_ok(st, hipMemcpy(st->lights.ptr, lights.data(), lights.size() * sizeof(LightEntry), hipMemcpyHostToDevice), "light upload")))
         return Q_EXT_ERR_DEVICE;
     st->num_tris = nt;
     st->num_nodes = bvh.num_nodes;
     st->bvh_depth = bvh.depth;
     st->num_lights = uint32_t(lights.size());
-    st->persistent_blocks = wavefront_persistent_blocks(st->device, bvh.depth);
+    st->num_nodes4 = bvh.num_nodes4;
+    st->bvh_depth4 = bvh.depth4;
+    st->gstack_entries = gentries;
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return Q_EXT_OK;
 }
@@ -442,7 +456,9 @@ void destroy_state(plugin_state* st) {
     free_buffer(st->tris);
     free_buffer(st->mats);
     free_buffer(st->nodes);
+    free_buffer(st->nodes4);
     free_buffer(st->lights);
+    free_buffer(st->gstack);
     for (auto& b : st->accum) free_buffer(b);
     free_buffer(st->ids);
     free_buffer(st->resolved);
@@ -523,6 +539,11 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
         st->device = dev;
         st->params = default_params();
         if (!hip_ok(st, hipSetDevice(dev), "hipSetDevice")) { delete st; return nullptr; }
+        if (hipDeviceGetAttribute(&st->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            st->num_cus <= 0)
+            st->num_cus = 256;
+        st->trace_cfg = TraceConfig{2, 256, 64};
+        st->trace_blocks = persistent_blocks(dev, st->trace_cfg);
         if (ctx->gpu->queue) {
             st->stream = static_cast<hipStream_t>(ctx->gpu->queue);
         } else {
@@ -614,6 +635,10 @@ Q_EXPORT void Q_plugin_render(Q_plugin_handle* handle, Q_render_frame* frame) {
         p.num_nodes = st->num_nodes;
         p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
         p.id_sample = rp.id_sample;
+        p.nodes4 = st->nodes4.ptr;
+        p.num_nodes4 = st->num_nodes4;
+        p.gstack = static_cast<int*>(st->gstack.ptr);
+        p.gstack_entries = st->gstack_entries;
         if (rp.engine == Q_ENGINE_WAVEFRONT) {
             const uint32_t per_pass = wave_samples_per_pass(st);
             const uint32_t npix = st->width * st->height;
@@ -624,7 +649,7 @@ Q_EXPORT void Q_plugin_render(Q_plugin_handle* handle, Q_render_frame* frame) {
                 p.sample_begin = st->sample_count + done;
                 p.sample_count = n;
                 const WaveState ws = wave_state(st->wave, n * npix);
-                if (!hip_ok(st, launch_wavefront_pass(p, ws, st->bvh_depth, st->persistent_blocks, s, &tm),
+                if (!hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, &tm),
                             "wavefront launch"))
                     return;
             }
@@ -737,15 +762,22 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
         Q_render_params p = *in;
         p.struct_size = sizeof(Q_render_params);
         if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > 16 ||
-            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT ||
-            (p.bvh_width != 0 && p.bvh_width != 2)) {
+            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT) {
             log_msg(st, "set_params: unsupported parameter combination");
             return Q_EXT_ERR_UNSUPPORTED;
         }
         if (p.bvh_width == 0) p.bvh_width = 2;
+        const TraceConfig cfg{p.bvh_width, p.block_size ? p.block_size : 256u, p.queue_chunk ? p.queue_chunk : 64u};
+        if (!trace_config_supported(cfg)) {
+            log_msg(st, "set_params: no traversal kernel for width %u, block %u, chunk %u", cfg.width, cfg.block,
+                    cfg.chunk);
+            return Q_EXT_ERR_UNSUPPORTED;
+        }
         (void)hipSetDevice(st->device);
         if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
         st->params = p;
+        st->trace_cfg = cfg;
+        st->trace_blocks = persistent_blocks(st->device, cfg);
         if (!clear_accumulation(st, st->stream)) return Q_EXT_ERR_DEVICE;
         return Q_EXT_OK;
     } catch (...) {

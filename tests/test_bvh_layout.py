@@ -29,24 +29,30 @@ def probe(tmp_path_factory):
     P = C.POINTER
     lib.probe_build.restype = C.c_int
     lib.probe_build.argtypes = [P(C.c_float)] * 3 + [C.c_uint, C.c_uint, P(C.c_float), C.c_uint, P(C.c_uint),
-                                                      P(C.c_uint)]
+                                                      P(C.c_uint), P(C.c_float)]
     return lib
 
 
-def build(probe, mesh, threads=4):
+def build(probe, mesh, threads=4, with4=False):
     tri = mesh.triangles().astype(np.float32)
     n = tri.shape[0]
     p = [np.ascontiguousarray(tri[:, k, :]) for k in range(3)]
     cap = max(1, 2 * n)
     nodes = np.zeros((cap, 16), dtype=np.float32)
+    nodes4 = np.zeros((cap, 32), dtype=np.float32)
     order = np.zeros(max(1, n), dtype=np.uint32)
-    info = np.zeros(4, dtype=np.uint32)
+    info = np.zeros(6, dtype=np.uint32)
     rc = probe.probe_build(*[x.ctypes.data_as(C.POINTER(C.c_float)) for x in p], n, threads,
                            nodes.ctypes.data_as(C.POINTER(C.c_float)), cap,
-                           order.ctypes.data_as(C.POINTER(C.c_uint)), info.ctypes.data_as(C.POINTER(C.c_uint)))
+                           order.ctypes.data_as(C.POINTER(C.c_uint)), info.ctypes.data_as(C.POINTER(C.c_uint)),
+                           nodes4.ctypes.data_as(C.POINTER(C.c_float)))
     assert rc == 0
     nn = int(info[0])
-    return nodes[:nn], order[:n], dict(nodes=nn, leaves=int(info[1]), depth=int(info[2]), order=int(info[3]))
+    meta = dict(nodes=nn, leaves=int(info[1]), depth=int(info[2]), order=int(info[3]), nodes4=int(info[4]),
+                depth4=int(info[5]))
+    if with4:
+        return nodes[:nn], order[:n], meta, nodes4[:meta["nodes4"]]
+    return nodes[:nn], order[:n], meta
 
 
 def child_refs(nodes):
@@ -205,3 +211,111 @@ def test_walk_equals_brute_force(probe):
         assert a[0] == b[0]
         hits += a[0] >= 0
     assert hits > 10
+
+
+EMPTY = 0x7FFFFFFF
+
+
+def check_layout4(mesh, nodes4, order):
+    """Every BVH4 reference is in range, every triangle is in exactly one leaf,
+    and each child box contains its leaf's triangles / its subtree's boxes."""
+    n = mesh.num_triangles
+    refs = nodes4[:, 24:28].copy().view(np.int32)
+    lo = np.stack([nodes4[:, 0:4], nodes4[:, 8:12], nodes4[:, 16:20]], axis=2)    # (N, 4, 3)
+    hi = np.stack([nodes4[:, 4:8], nodes4[:, 12:16], nodes4[:, 20:24]], axis=2)
+    tri = mesh.triangles()[order]
+    tlo, thi = tri.min(1), tri.max(1)
+    covered = np.zeros(n, dtype=np.int64)
+    parents = np.zeros(len(nodes4), dtype=np.int64)
+    for i in range(len(nodes4)):
+        used = refs[i] != EMPTY
+        assert used.sum() >= (2 if n > 1 else 1)
+        assert (~used[np.argmax(~used):]).all() if (~used).any() else True   # empty slots are trailing
+        for k in range(4):
+            c = int(refs[i, k])
+            if c == EMPTY:
+                continue
+            if c >= 0:
+                assert 0 < c < len(nodes4)
+                parents[c] += 1
+                assert (lo[c][refs[c] != EMPTY] >= lo[i, k]).all() and (hi[c][refs[c] != EMPTY] <= hi[i, k]).all()
+            else:
+                v = (~np.int64(c)) & 0xFFFFFFFF
+                f, cnt = int(v >> 2), int(v & 3) + 1
+                assert f + cnt <= n
+                covered[f:f + cnt] += 1
+                assert (lo[i, k] <= tlo[f:f + cnt]).all() and (hi[i, k] >= thi[f:f + cnt]).all()
+    if n > 1:
+        assert (covered == 1).all()
+    assert parents[0] == 0 and (parents[1:] == 1).all()
+    return refs, lo, hi
+
+
+def walk4_closest(nodes4, tri_bvh, order, o, d):
+    refs = nodes4[:, 24:28].copy().view(np.int32)
+    lo = np.stack([nodes4[:, 0:4], nodes4[:, 8:12], nodes4[:, 16:20]], axis=2)
+    hi = np.stack([nodes4[:, 4:8], nodes4[:, 12:16], nodes4[:, 20:24]], axis=2)
+    inv = 1.0 / np.where(np.abs(d) < 1e-20, np.copysign(1e-20, d), d)
+    best_t, best = 1e30, -1
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        for k in range(4):
+            c = int(refs[i, k])
+            if c == EMPTY:
+                continue
+            t0, t1 = (lo[i, k] - o) * inv, (hi[i, k] - o) * inv
+            if max(np.minimum(t0, t1).max(), 1e-3) > min(np.maximum(t0, t1).min(), best_t) * (1 + 1e-9):
+                continue
+            if c >= 0:
+                stack.append(c)
+                continue
+            v = (~np.int64(c)) & 0xFFFFFFFF
+            for j in range(int(v >> 2), int(v >> 2) + int(v & 3) + 1):
+                t = mt(o, d, *tri_bvh[j])
+                if t is not None and (t < best_t or (t == best_t and order[j] > best)):
+                    best_t, best = t, int(order[j])
+    return best, best_t
+
+
+@pytest.mark.parametrize("name", ["cornell_ref", "cornell36", "soup3000"])
+def test_layout4_invariants(probe, name):
+    mesh = {"cornell_ref": scenes.cornell_ref, "cornell36": lambda: scenes.cornell36(0),
+            "soup3000": lambda: scenes.soup(3000, seed=2)}[name]()
+    nodes, order, meta, nodes4 = build(probe, mesh, with4=True)
+    check_layout4(mesh, nodes4, order)
+    assert meta["depth4"] <= meta["depth"]
+
+
+@pytest.mark.parametrize("ntri", [1, 2, 3, 4, 5, 9])
+def test_layout4_tiny_scenes(probe, ntri):
+    m = scenes.soup(ntri, seed=3)
+    m.indices = m.indices[:ntri]
+    m.albedo, m.emission = m.albedo[:ntri], m.emission[:ntri]
+    nodes, order, meta, nodes4 = build(probe, m, with4=True)
+    assert meta["nodes4"] >= 1
+    check_layout4(m, nodes4, order)
+
+
+def test_walk4_equals_brute_force(probe):
+    m = scenes.soup(2000, seed=6)
+    nodes, order, meta, nodes4 = build(probe, m, with4=True)
+    tris = m.triangles().astype(np.float64)
+    tri_bvh = tris[order]
+    rng = np.random.default_rng(2)
+    cen = tris.mean(1)
+    for i in range(300):
+        o = rng.uniform(-1.3, 1.3, 3)
+        d = (cen[rng.integers(len(cen))] - o) if i % 2 else rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        assert walk4_closest(nodes4, tri_bvh, order, o, d)[0] == brute_closest(tris, o, d)[0]
+
+
+def test_layout4_large_soup(probe):
+    m = scenes.soup(1_000_000, seed=1)
+    nodes, order, meta, nodes4 = build(probe, m, threads=8, with4=True)
+    refs = nodes4[:, 24:28].copy().view(np.int32)
+    internal = refs[(refs >= 0) & (refs != EMPTY)]
+    assert internal.max() < len(nodes4)
+    assert meta["nodes4"] < meta["nodes"]
+    assert 3 * meta["depth4"] <= 16 + 64     # LDS short stack + global overflow capacity

@@ -211,3 +211,58 @@ def test_wavefront_pass_split_and_determinism():
     assert np.array_equal(imgs[2], imgs[3])
     for im in imgs[:2]:
         assert np.allclose(im, imgs[2], rtol=2e-6, atol=1e-6)
+
+
+TRACE_CONFIGS = [(w, b, c) for w in (2, 4) for b in (64, 128, 256) for c in (1, 64, 256)]
+
+
+def test_all_trace_configs_render_identically():
+    """Every compiled traversal variant (BVH width 2/4, block 64/128/256, queue
+    chunk 64/256) yields the same image bit for bit: the closest hit is unique
+    under the tie rule and occlusion does not depend on which blocker is found."""
+    mesh = scenes.soup(150_000, seed=8)
+    w, h = 96, 64
+    say("[trace-configs]")
+    ref = None
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        for (bw, blk, ch) in TRACE_CONFIGS:
+            r.set_params(spp_per_render=4, max_depth=6, rr_start=3, rng_mode=1, integrator=1,
+                         engine=abi.Q_ENGINE_WAVEFRONT, bvh_width=bw, block_size=blk, queue_chunk=ch)
+            r.render(1)
+            img = r.readback()
+            if ref is None:
+                ref = img
+            assert np.array_equal(img, ref), (bw, blk, ch)
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_bvh4_parity_with_oracle(width):
+    mesh = scenes.soup(200_000, seed=1)
+    w, h, spp = 120, 80, 4
+    say(f"[bvh-width] width={width}")
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        r.set_params(spp_per_render=spp, max_depth=8, rr_start=3, rng_mode=1, integrator=1, seed=5,
+                     flags=abi.Q_FLAG_RECORD_IDS, id_sample=1, engine=abi.Q_ENGINE_WAVEFRONT, bvh_width=width)
+        r.render(1)
+        params = dict(width=w, height=h, rng_mode=1, seed=5, camera=mesh.camera)
+        oscene = O.Scene(mesh)
+        ref = oscene.render(w, h, spp, max_depth=8, rr_start=3, rng_mode=1, integrator=1, seed=5, id_sample=1)
+        aov = r.readback_aov()
+        assert rel_l1(aov["beauty"], ref.mean()) <= REL_BAR
+        frac, nbad, not_ties = check_ids(r.primary_ids(), ref.ids, mesh, oscene, params, sample=1)
+        assert frac >= 0.9999 and not not_ties
+
+
+def test_unsupported_trace_config_is_rejected():
+    with HipRenderer(16, 16) as r:
+        with pytest.raises(RuntimeError):
+            r.set_params(bvh_width=3)
+        with pytest.raises(RuntimeError):
+            r.set_params(block_size=96)
+        with pytest.raises(RuntimeError):
+            r.set_params(queue_chunk=100)
+        r.set_params(bvh_width=4, block_size=128, queue_chunk=256)
