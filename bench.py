@@ -65,6 +65,91 @@ def cpu_baseline(mesh, args, seconds: float):
                         f"samples 0..3 of every 61st pixel; BVH build excluded)"))
 
 
+def bench_batch(args):
+    """64 scenes of uneven size (room + soup, 1k..500k triangles, 128^2..1024^2),
+    32 spp, depth 6, NEE: every scene rendered one after another with the
+    plain wavefront engine, then all of them as one batch (shared queues,
+    captured pass graph, autotuned trace configuration).  Scene preparation
+    (BVH builds, uploads) is outside both timed regions; every scene's batched
+    beauty image is compared with its plain one."""
+    import numpy as np
+    from quasi_amd import abi, scenes
+    from quasi_amd.plugin import HipRenderer, make_camera
+
+    t0 = time.perf_counter()
+    batch = scenes.batch_scenes(num_scenes=args.batch_scenes)
+    ntri = sum(m.num_triangles for m, _, _, _ in batch)
+    npix = sum(w * h for _, w, h, _ in batch)
+    log(f"batch: {len(batch)} scenes, {ntri} triangles, {npix} pixels generated in {time.perf_counter() - t0:.1f} s")
+    spp, depth = args.batch_spp, 6
+    common = dict(max_depth=depth, rr_start=3, rng_mode=1, integrator=1, engine=abi.Q_ENGINE_WAVEFRONT,
+                  bvh_width=args.bvh_width, block_size=args.block, queue_chunk=args.chunk)
+
+    def rays_of(st):
+        return st.rays_camera + st.rays_extension + st.rays_shadow
+
+    plain_s = 0.0
+    plain_rays = 0
+    plain_imgs = []
+    with HipRenderer(64, 64, device=0) as r:
+        for i, (mesh, w, h, seed) in enumerate(batch):
+            r.set_scene(mesh)
+            r.resize(w, h)
+            c = mesh.camera
+            r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+            r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **common)
+            if i == 0:          # untimed warm-up of the kernels, then a cleared accumulation
+                r.render(1)
+                r.synchronize()
+                r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **common)
+            r.reset_stats()
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.render(1)
+            r.synchronize()
+            plain_s += time.perf_counter() - t0
+            plain_rays += rays_of(r.stats())
+            plain_imgs.append(r.readback_aov()["beauty"])
+        log(f"batch: plain sequential {plain_s:.3f} s, {plain_rays / plain_s / 1e6:.1f} Mrays/s")
+
+    with HipRenderer(64, 64, device=0) as r:
+        r.set_params(spp_per_render=1, seed=0, samples_per_pass=args.batch_samples_per_pass, flags=0, **common)
+        t0 = time.perf_counter()
+        r.set_batch(batch)
+        prep_s = time.perf_counter() - t0
+        tune = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=args.batch_samples_per_pass, flags=abi.Q_AUTOTUNE_NO_STORE)
+        best = tune.entries[tune.best]
+        log(f"batch: prepared in {prep_s:.1f} s; autotuned width {best.bvh_width} block {best.block_size} "
+            f"chunk {best.queue_chunk} ({best.ms:.2f} ms per probe)")
+        # the tuner's probes warmed the kernels; it leaves the chosen pass graph
+        # captured and the accumulation empty
+        r.reset_stats()
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.render_batch(spp)
+        r.synchronize()
+        batched_s = time.perf_counter() - t0
+        batched_rays = rays_of(r.stats())
+        err = 0.0
+        for i in range(len(batch)):
+            img = r.readback_batch(i)["beauty"]
+            ref = plain_imgs[i]
+            err = max(err, float(np.abs(img - ref).sum() / max(np.abs(ref).sum(), 1e-30)))
+    plain = plain_rays / plain_s / 1e6
+    batched = batched_rays / batched_s / 1e6
+    log(f"batch: batched {batched_s:.3f} s, {batched:.1f} Mrays/s, speedup {batched / plain:.3f}, "
+        f"max rel-L1 vs plain {err:.2e}")
+    return {"workload": f"{len(batch)} scenes (room + soup, {ntri} triangles, {npix} pixels), {spp} spp, "
+                        f"depth {depth}, RR from 3, NEE",
+            "plain_mrays": round(plain, 3), "batched_mrays": round(batched, 3),
+            "speedup": round(batched / plain, 4), "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4),
+            "plain_rays": int(plain_rays), "batched_rays": int(batched_rays),
+            "samples_per_pass": args.batch_samples_per_pass,
+            "autotuned": {"bvh_width": best.bvh_width, "block_size": best.block_size,
+                          "queue_chunk": best.queue_chunk},
+            "max_rel_l1_vs_plain": err}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +168,10 @@ def main():
     ap.add_argument("--chunk", type=int, choices=[1, 64, 256], default=64)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-spp", type=int, default=32)
+    ap.add_argument("--batch-samples-per-pass", type=int, default=1)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -187,6 +276,7 @@ def main():
                      "alg_bytes_per_launch": round(per_launch)},
         "kernel_ms": {n: round(v.total_ms, 3) for n, v in kern.items() if v.launches},
     }
+    out["batch"] = None if args.no_batch else bench_batch(args)
     if not args.no_cpu:
         log("CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(mesh, args, args.cpu_seconds)

@@ -137,6 +137,78 @@ int32_t Q_plugin_reset_stats(Q_plugin_handle* handle);
 /* Waits until every queued render has finished. */
 int32_t Q_plugin_synchronize(Q_plugin_handle* handle);
 
+/* ---- Batched scenes --------------------------------------------------------
+ * Many scenes of uneven size (and resolution) traced together: every sample
+ * pass covers all scenes with shared, compacted ray queues, and the pass is
+ * captured once in a hipGraph and replayed per pass.  The render parameters
+ * (depth, integrator, seeding mode, traversal shape, samples_per_pass: samples
+ * of every scene per replayed pass, 0 = 1) are those of Q_plugin_set_params;
+ * each scene brings its own camera, resolution and seed.  With the same
+ * parameters, scene i of a batch renders exactly what a single-scene render
+ * of it with seed = scenes[i].seed and the same samples_per_pass gives. */
+typedef struct Q_batch_scene {
+    uint32_t width;
+    uint32_t height;
+    uint32_t seed;             /* seed of this scene's Q_RNG_HASHED streams */
+    uint32_t reserved;
+    Q_camera camera;
+    Q_scene_desc mesh;
+} Q_batch_scene;
+
+typedef struct Q_batch_desc {
+    uint32_t struct_size;
+    uint32_t num_scenes;
+    const Q_batch_scene* scenes;
+} Q_batch_desc;
+
+/* Builds every scene's BVH (host threads), uploads all scenes and clears the
+ * batch accumulation.  Replaces any previous batch. */
+int32_t Q_plugin_set_batch(Q_plugin_handle* handle, const Q_batch_desc* batch);
+
+/* Adds spp samples to every scene of the batch, asynchronously on `stream`
+ * (a hipStream_t; NULL: the plugin's stream).  spp must be a multiple of the
+ * samples per pass. */
+int32_t Q_plugin_render_batch(Q_plugin_handle* handle, uint32_t spp, void* stream);
+
+/* The mean AOVs of one scene (malloc'd, free with Q_plugin_readback_aov_free). */
+int32_t Q_plugin_readback_batch(Q_plugin_handle* handle, uint32_t scene, Q_readback_aov_result* out);
+
+/* Primary ids of one scene's sample id_sample (see Q_plugin_readback_ids). */
+int32_t Q_plugin_readback_batch_ids(Q_plugin_handle* handle, uint32_t scene, uint32_t* out, uint64_t count);
+
+/* ---- Autotuner --------------------------------------------------------------
+ * Times every compiled traversal shape (BVH width x block size x queue chunk)
+ * on probe_spp samples of the current single scene (Q_AUTOTUNE_SINGLE, with
+ * the last camera given to Q_plugin_render) or of the current batch
+ * (Q_AUTOTUNE_BATCH), keeps the fastest in the render parameters and clears
+ * the accumulation.  Results are cached, one JSON object per line, in
+ * $QUASI_HIP_TUNE_CACHE (default ~/.cache/quasi_hip/autotune.jsonl), keyed by
+ * GPU architecture, target and workload signature. */
+#define Q_AUTOTUNE_SINGLE 0u
+#define Q_AUTOTUNE_BATCH  1u
+#define Q_AUTOTUNE_USE_CACHE 1u   /* flags: take a cached choice without timing */
+#define Q_AUTOTUNE_NO_STORE  2u   /* flags: do not write the cache */
+#define Q_AUTOTUNE_MAX_ENTRIES 32
+
+typedef struct Q_autotune_entry {
+    uint32_t bvh_width;
+    uint32_t block_size;
+    uint32_t queue_chunk;
+    uint32_t reserved;
+    double ms;                 /* hipEvent time of the probe */
+} Q_autotune_entry;
+
+typedef struct Q_autotune_result {
+    uint32_t struct_size;
+    uint32_t num_entries;
+    uint32_t best;             /* index into entries */
+    uint32_t from_cache;
+    Q_autotune_entry entries[Q_AUTOTUNE_MAX_ENTRIES];
+} Q_autotune_result;
+
+int32_t Q_plugin_autotune(Q_plugin_handle* handle, uint32_t target, uint32_t probe_spp, uint32_t flags,
+                          Q_autotune_result* out);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -102,6 +102,30 @@ class Q_render_stats(C.Structure):
                 ("kernels", Q_kernel_stat * Q_STATS_MAX_KERNELS)]
 
 
+class Q_batch_scene(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("seed", C.c_uint32), ("reserved", C.c_uint32),
+                ("camera", Q_camera), ("mesh", Q_scene_desc)]
+
+
+class Q_batch_desc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("num_scenes", C.c_uint32), ("scenes", C.POINTER(Q_batch_scene))]
+
+
+Q_AUTOTUNE_SINGLE, Q_AUTOTUNE_BATCH = 0, 1
+Q_AUTOTUNE_USE_CACHE, Q_AUTOTUNE_NO_STORE = 1, 2
+Q_AUTOTUNE_MAX_ENTRIES = 32
+
+
+class Q_autotune_entry(C.Structure):
+    _fields_ = [("bvh_width", C.c_uint32), ("block_size", C.c_uint32), ("queue_chunk", C.c_uint32),
+                ("reserved", C.c_uint32), ("ms", C.c_double)]
+
+
+class Q_autotune_result(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("num_entries", C.c_uint32), ("best", C.c_uint32),
+                ("from_cache", C.c_uint32), ("entries", Q_autotune_entry * Q_AUTOTUNE_MAX_ENTRIES)]
+
+
 # Symbol names, plugin_interface.hpp:170-179
 K_SYMBOL_ABI_VERSION = "Q_plugin_abi_version"
 K_SYMBOL_GET_INFO = "Q_plugin_get_info"
@@ -120,4 +144,5 @@ OPTIONAL_SYMBOLS = (K_SYMBOL_READBACK, K_SYMBOL_READBACK_FREE, K_SYMBOL_READBACK
                     K_SYMBOL_READBACK_AOV_FREE)
 EXTENSION_SYMBOLS = ("Q_plugin_set_scene", "Q_plugin_get_params", "Q_plugin_set_params",
                      "Q_plugin_readback_ids", "Q_plugin_get_stats", "Q_plugin_reset_stats",
-                     "Q_plugin_synchronize")
+                     "Q_plugin_synchronize", "Q_plugin_set_batch", "Q_plugin_render_batch",
+                     "Q_plugin_readback_batch", "Q_plugin_readback_batch_ids", "Q_plugin_autotune")

@@ -37,6 +37,7 @@ PLUGIN_SOURCES = [
     ("kernels_wave.hip", DEVICE),
     ("kernels_util.hip", DEVICE),
     ("plugin.cpp", HOSTONLY),
+    ("plugin_batch.cpp", HOSTONLY),
     ("bvh_build.cpp", HOSTONLY),
     ("scene_cornell.cpp", HOSTONLY),
 ]

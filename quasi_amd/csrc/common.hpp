@@ -81,6 +81,25 @@ struct WaveState {
     uint32_t paths;        // paths in this pass
 };
 
+// One scene of a wavefront pass.  A single-scene pass carries it inline
+// (PassParams::scene0); a batched pass reads a device table of them.
+// Scene i owns paths [path_begin, path_begin + width*height*samples) of the
+// pass and pixels [pix_begin, pix_begin + width*height) of the framebuffer
+// arrays; path p_local = s*width*height + y*width + x.
+struct SceneDesc {
+    const void* tris;
+    const void* mats;
+    const void* nodes;
+    const void* nodes4;
+    const LightEntry* lights;
+    uint32_t num_nodes, num_lights;
+    CameraBasis cam;
+    uint32_t width, height;
+    uint32_t path_begin, pix_begin;
+    uint32_t seed;
+    uint32_t pad;
+};
+
 // Everything one sample pass needs (passed by value as a kernel argument).
 struct PassParams {
     const void* tris;      // float4*
@@ -108,6 +127,12 @@ struct PassParams {
     uint32_t num_nodes4;
     uint32_t gstack_entries;
     int* gstack;
+    // wavefront scene(s): scene0 when num_scenes == 0, else scenes[0..num_scenes)
+    SceneDesc scene0;
+    const SceneDesc* scenes;
+    uint32_t num_scenes;
+    uint32_t total_pixels;             // framebuffer pixels over all scenes
+    const uint32_t* dyn_sample_begin;  // when set, the pass's first sample index is read here
 };
 
 // Launch shape of the persistent traversal kernels (autotuned).

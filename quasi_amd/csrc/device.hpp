@@ -430,7 +430,8 @@ struct FirstHit {
 // One vertex of either integrator at bounce b for a ray that hit BVH triangle
 // `tri` at distance t.  Updates throughput, radiance and the random stream.
 __device__ __forceinline__ void shade_hit(const PassParams& P, const float4* __restrict__ tris,
-                                          const float4* __restrict__ mats, uint32_t b, const Ray& ray,
+                                          const float4* __restrict__ mats, const LightEntry* __restrict__ lights,
+                                          uint32_t num_lights, uint32_t b, const Ray& ray,
                                           float t, int tri, f3& thr, f3& L, uint32_t& rng, FirstHit* first,
                                           ShadeOut& out) {
     out.cont = false;
@@ -470,16 +471,16 @@ __device__ __forceinline__ void shade_hit(const PassParams& P, const float4* __r
     if (emit_str > 0.1f) return;
     if (b + 1 >= P.max_depth) return;
     const f3 org = pt + nf * kOffset;
-    if (P.num_lights) {
+    if (num_lights) {
         // light choice: first entry with r < cdf (the last when none)
         const float rs = random_float(rng);
-        uint32_t lo = 0, hi = P.num_lights;
+        uint32_t lo = 0, hi = num_lights;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (rs < P.lights[mid].cdf) hi = mid; else lo = mid + 1;
+            if (rs < lights[mid].cdf) hi = mid; else lo = mid + 1;
         }
-        if (lo >= P.num_lights) lo = P.num_lights - 1;
-        const LightEntry le = P.lights[lo];
+        if (lo >= num_lights) lo = num_lights - 1;
+        const LightEntry le = lights[lo];
         const float r1 = random_float(rng);
         const float r2 = random_float(rng);
         const float4 l0 = tris[3 * le.tri + 0];
@@ -520,17 +521,18 @@ __device__ __forceinline__ void shade_hit(const PassParams& P, const float4* __r
 
 // Primary ray of pixel (x, y) for one sample (plugin.mm:129-153 with the
 // camera basis evaluated on the host; x jitter drawn first, plugin.mm:131).
-__device__ __forceinline__ Ray camera_ray(const PassParams& P, uint32_t x, uint32_t y, uint32_t& rng) {
-    const f3 llo = mk(P.cam.llo[0], P.cam.llo[1], P.cam.llo[2]);
-    const f3 hor = mk(P.cam.horizontal[0], P.cam.horizontal[1], P.cam.horizontal[2]);
-    const f3 ver = mk(P.cam.vertical[0], P.cam.vertical[1], P.cam.vertical[2]);
-    const float u0 = (float(x) + 0.5f) / float(P.width);
-    const float v0 = 1.0f - (float(y) + 0.5f) / float(P.height);
+__device__ __forceinline__ Ray camera_ray(const CameraBasis& cam, uint32_t width, uint32_t height, uint32_t x,
+                                          uint32_t y, uint32_t& rng) {
+    const f3 llo = mk(cam.llo[0], cam.llo[1], cam.llo[2]);
+    const f3 hor = mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2]);
+    const f3 ver = mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]);
+    const float u0 = (float(x) + 0.5f) / float(width);
+    const float v0 = 1.0f - (float(y) + 0.5f) / float(height);
     const float jx = random_float(rng);
     const float jy = random_float(rng);
     const float u = u0 + (jx - 0.5f) * 0.001f;
     const float v = v0 + (jy - 0.5f) * 0.001f;
-    return Ray{mk(P.cam.origin[0], P.cam.origin[1], P.cam.origin[2]), normalize(llo + hor * u + ver * v)};
+    return Ray{mk(cam.origin[0], cam.origin[1], cam.origin[2]), normalize(llo + hor * u + ver * v)};
 }
 
 // Sum over the 64 lanes of a wave (all lanes must be present).

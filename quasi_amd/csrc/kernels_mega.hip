@@ -37,7 +37,7 @@ __global__ __launch_bounds__(BLOCK) void k_megakernel(PassParams P) {
         float4 acc_b = {0, 0, 0, 0}, acc_a = {0, 0, 0, 0}, acc_n = {0, 0, 0, 0}, acc_d = {0, 0, 0, 0};
         for (uint32_t s = P.sample_begin; s < P.sample_begin + P.sample_count; ++s) {
             uint32_t rng = path_seed(P.rng_mode, x, y, P.width, P.height, s, P.seed);
-            Ray ray = camera_ray(P, x, y, rng);
+            Ray ray = camera_ray(P.cam, P.width, P.height, x, y, rng);
             f3 thr = mk(1.f, 1.f, 1.f);
             f3 L = mk(0.f, 0.f, 0.f);
             FirstHit first{mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f), 0.f, kMissId};
@@ -47,7 +47,8 @@ __global__ __launch_bounds__(BLOCK) void k_megakernel(PassParams P) {
                 if (b == 0) ++n_cam; else ++n_ext;
                 if (h.tri < 0) break;
                 ShadeOut so;
-                shade_hit(P, tris, mats, b, ray, h.t, h.tri, thr, L, rng, b == 0 ? &first : nullptr, so);
+                shade_hit(P, tris, mats, P.lights, P.num_lights, b, ray, h.t, h.tri, thr, L, rng,
+                          b == 0 ? &first : nullptr, so);
                 if (so.shadow) {
                     ++n_shadow;
                     if (!trace_any<BLOCK>(nodes, tris, so.sray, kTMin, so.stmax, stk)) L = L + so.scontrib;

@@ -1,15 +1,22 @@
-// kernels_wave.hip -- the wavefront engine.
+// kernels_wave.hip -- the wavefront engine, for one scene or a batch of scenes.
 //
-// One sample pass over N = W*H*S paths runs as
+// One sample pass over N paths runs as
 //   raygen -> (extend -> shade -> shadow) x max_depth -> accumulate
 // with every queue length kept on the device, so the host enqueues a fixed
-// launch sequence without synchronising (and the sequence can be captured in
-// a hipGraph).  extend and shadow are persistent kernels: waves fetch 64-ray
-// chunks from a per-bounce cursor until the queue is drained.  shade compacts
-// the surviving paths and the requested shadow rays with one ballot and one
-// atomic per wave.  Radiance lives in a per-path slot and is reduced per pixel
-// in sample order by accumulate, so the result does not depend on the order in
-// which queues were filled.
+// launch sequence without synchronising, and the sequence can be captured in
+// a hipGraph and replayed (the pass's first sample index can be read from
+// device memory, and k_advance moves it on).  extend and shadow are
+// persistent kernels: waves fetch chunks of rays from a per-bounce cursor until
+// the queue is drained (or, in the dynamic variant, each lane fetches its next
+// ray as soon as it is idle).  shade compacts the surviving paths and the
+// requested shadow rays with one ballot and one atomic per wave.  Radiance
+// lives in a per-path slot and is reduced per pixel in sample order by
+// accumulate, so the result does not depend on the order in which queues were
+// filled.
+//
+// BATCH = true: several scenes of different sizes share the queues of one
+// pass; each ray carries its scene index and every kernel reads the scene's
+// geometry, camera and framebuffer offsets from the scene table.
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -19,22 +26,43 @@ namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
-// Path id p -> (pixel, sample within the pass).
-__device__ __forceinline__ void path_pixel(const PassParams& P, uint32_t p, uint32_t& x, uint32_t& y,
-                                           uint32_t& s) {
-    const uint32_t npix = P.width * P.height;
-    s = p / npix;
-    const uint32_t pix = p - s * npix;
-    y = pix / P.width;
-    x = pix - y * P.width;
+template <bool BATCH>
+__device__ __forceinline__ const SceneDesc& scene_at(const PassParams& P, uint32_t sid) {
+    if (BATCH) return P.scenes[sid];
+    return P.scene0;
 }
 
+// Scene owning path id p (BATCH): the last scene whose path_begin <= p.
+__device__ __forceinline__ uint32_t scene_of_path(const PassParams& P, uint32_t p) {
+    uint32_t lo = 0, hi = P.num_scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.scenes[mid].path_begin <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scene_of_pixel(const PassParams& P, uint32_t pix) {
+    uint32_t lo = 0, hi = P.num_scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.scenes[mid].pix_begin <= pix) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t pass_sample_begin(const PassParams& P) {
+    return P.dyn_sample_begin ? *P.dyn_sample_begin : P.sample_begin;
+}
+
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     if (blockIdx.x == 0 && threadIdx.x < 2 * kMaxDepth + 1) {
         if (threadIdx.x <= kMaxDepth) S.counts[threadIdx.x] = threadIdx.x == 0 ? S.paths : 0u;
         if (threadIdx.x < kMaxDepth) S.shadow_counts[threadIdx.x] = 0u;
         if (threadIdx.x < 2 * kMaxDepth) S.fetch[threadIdx.x] = 0u;
     }
+    const uint32_t s0 = pass_sample_begin(P);
     float4* ro = static_cast<float4*>(S.ray_o[0]);
     float4* rd = static_cast<float4*>(S.ray_d[0]);
     float4* thr = static_cast<float4*>(S.thr);
@@ -42,12 +70,18 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     float4* a0 = static_cast<float4*>(S.aov0);
     float4* a1 = static_cast<float4*>(S.aov1);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < S.paths; p += gridDim.x * blockDim.x) {
-        uint32_t x, y, s;
-        path_pixel(P, p, x, y, s);
-        uint32_t rng = path_seed(P.rng_mode, x, y, P.width, P.height, P.sample_begin + s, P.seed);
-        const Ray r = camera_ray(P, x, y, rng);
+        const uint32_t sid = BATCH ? scene_of_path(P, p) : 0u;
+        const SceneDesc& sc = scene_at<BATCH>(P, sid);
+        const uint32_t npix = sc.width * sc.height;
+        const uint32_t pl = p - sc.path_begin;
+        const uint32_t s = pl / npix;
+        const uint32_t pix = pl - s * npix;
+        const uint32_t y = pix / sc.width;
+        const uint32_t x = pix - y * sc.width;
+        uint32_t rng = path_seed(P.rng_mode, x, y, sc.width, sc.height, s0 + s, sc.seed);
+        const Ray r = camera_ray(sc.cam, sc.width, sc.height, x, y, rng);
         ro[p] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p));
-        rd[p] = make_float4(r.d.x, r.d.y, r.d.z, 0.f);
+        rd[p] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(sid));
         thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
         rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         a0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -56,9 +90,15 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     }
 }
 
+template <int WIDTH, bool BATCH>
+__device__ __forceinline__ const float4* scene_nodes(const PassParams& P, uint32_t sid) {
+    const SceneDesc& sc = scene_at<BATCH>(P, sid);
+    return static_cast<const float4*>(WIDTH == 4 ? sc.nodes4 : sc.nodes);
+}
+
 // Persistent closest-hit traversal: each wave reserves CHUNK rays of the
 // bounce's queue at a time until the queue is drained.
-template <int WIDTH, int BLOCK, int CHUNK>
+template <int WIDTH, int BLOCK, int CHUNK, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uint32_t b) {
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
@@ -66,8 +106,6 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uin
     const uint32_t n = S.counts[b];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n)
         atomicAdd(b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension, (unsigned long long)n);
-    const float4* __restrict__ nodes = static_cast<const float4*>(WIDTH == 4 ? P.nodes4 : P.nodes);
-    const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
     const float4* __restrict__ ro = static_cast<const float4*>(S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(S.ray_d[b & 1u]);
     uint2* hit = static_cast<uint2*>(S.hit);
@@ -77,9 +115,13 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uin
             const uint32_t i = base + j + lane_id();
             if (i < n) {
                 const float4 o = ro[i], d = rd[i];
+                const uint32_t sid = BATCH ? __float_as_uint(d.w) : 0u;
+                const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 TraceHit h{kTFar, -1, 0u};
                 st.sp = 0;
-                if (P.num_nodes) {
+                if (sc.num_nodes) {
+                    const float4* nodes = scene_nodes<WIDTH, BATCH>(P, sid);
+                    const float4* tris = static_cast<const float4*>(sc.tris);
                     const Ray ray{xyz(o), xyz(d)};
                     h = WIDTH == 4 ? traverse4<false>(nodes, tris, ray, kTMin, kTFar, st)
                                    : traverse2<false>(nodes, tris, ray, kTMin, kTFar, st);
@@ -90,11 +132,9 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uin
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint32_t b) {
     const uint32_t n = S.counts[b];
-    const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
-    const float4* __restrict__ mats = static_cast<const float4*>(P.mats);
     const float4* __restrict__ ro = static_cast<const float4*>(S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(S.ray_d[b & 1u]);
     float4* no = static_cast<float4*>(S.ray_o[(b + 1) & 1u]);
@@ -113,18 +153,21 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
         const uint32_t i = wb + lane_id();
         bool cont = false, shadow = false;
         ShadeOut so;
-        uint32_t p = 0;
+        uint32_t p = 0, sid = 0;
         if (i < n) {
             const float4 o = ro[i], d = rd[i];
             p = __float_as_uint(o.w);
+            sid = BATCH ? __float_as_uint(d.w) : 0u;
             const uint2 h = hit[i];
             const int tri = int(h.y);
             if (tri >= 0) {
+                const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 float4 t4 = thr_a[p], l4 = rad_a[p];
                 f3 thr = xyz(t4), L = xyz(l4);
                 uint32_t rng = S.rng[p];
                 FirstHit first;
-                shade_hit(P, tris, mats, b, Ray{xyz(o), xyz(d)}, __uint_as_float(h.x), tri, thr, L, rng,
+                shade_hit(P, static_cast<const float4*>(sc.tris), static_cast<const float4*>(sc.mats), sc.lights,
+                          sc.num_lights, b, Ray{xyz(o), xyz(d)}, __uint_as_float(h.x), tri, thr, L, rng,
                           b == 0 ? &first : nullptr, so);
                 if (b == 0) {
                     a0[p] = make_float4(first.albedo.x, first.albedo.y, first.albedo.z, first.t);
@@ -140,21 +183,28 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
         const uint32_t j = wave_append(cont, &S.counts[b + 1]);
         if (cont) {
             no[j] = make_float4(so.next.o.x, so.next.o.y, so.next.o.z, __uint_as_float(p));
-            nd[j] = make_float4(so.next.d.x, so.next.d.y, so.next.d.z, 0.f);
+            nd[j] = make_float4(so.next.d.x, so.next.d.y, so.next.d.z, __uint_as_float(sid));
         }
         const uint32_t k = wave_append(shadow, &S.shadow_counts[b]);
         if (shadow) {
             so_a[k] = make_float4(so.sray.o.x, so.sray.o.y, so.sray.o.z, so.stmax);
             sd_a[k] = make_float4(so.sray.d.x, so.sray.d.y, so.sray.d.z, __uint_as_float(p));
-            sc_a[k] = make_float4(so.scontrib.x, so.scontrib.y, so.scontrib.z, 0.f);
+            sc_a[k] = make_float4(so.scontrib.x, so.scontrib.y, so.scontrib.z, __uint_as_float(sid));
         }
     }
+}
+
+__device__ __forceinline__ void add_shadow(float4* rad, const float4* sc_a, uint32_t k, uint32_t p) {
+    const float4 c = sc_a[k];
+    float4 l = rad[p];
+    l.x += c.x; l.y += c.y; l.z += c.z;
+    rad[p] = l;
 }
 
 // Persistent any-hit traversal of the bounce's shadow queue; an unoccluded
 // ray adds its contribution to the path's radiance slot (one shadow ray per
 // path and bounce, so no two lanes write the same slot).
-template <int WIDTH, int BLOCK, int CHUNK>
+template <int WIDTH, int BLOCK, int CHUNK, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uint32_t b) {
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
@@ -162,8 +212,6 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
     const uint32_t n = S.shadow_counts[b];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n)
         atomicAdd(&P.counters->rays_shadow, (unsigned long long)n);
-    const float4* __restrict__ nodes = static_cast<const float4*>(WIDTH == 4 ? P.nodes4 : P.nodes);
-    const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
     const float4* __restrict__ so_a = static_cast<const float4*>(S.sh_o);
     const float4* __restrict__ sd_a = static_cast<const float4*>(S.sh_d);
     const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
@@ -174,17 +222,16 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
             const uint32_t k = base + j + lane_id();
             if (k < n) {
                 const float4 o = so_a[k], d = sd_a[k];
+                const uint32_t sid = BATCH ? __float_as_uint(sc_a[k].w) : 0u;
+                const SceneDesc& sc = scene_at<BATCH>(P, sid);
+                const float4* nodes = scene_nodes<WIDTH, BATCH>(P, sid);
+                const float4* tris = static_cast<const float4*>(sc.tris);
                 const Ray ray{xyz(o), xyz(d)};
                 st.sp = 0;
-                const bool occ = WIDTH == 4 ? traverse4<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0
-                                            : traverse2<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0;
-                if (!occ) {
-                    const uint32_t p = __float_as_uint(d.w);
-                    const float4 c = sc_a[k];
-                    float4 l = rad[p];
-                    l.x += c.x; l.y += c.y; l.z += c.z;
-                    rad[p] = l;
-                }
+                const bool occ = sc.num_nodes != 0 &&
+                                 (WIDTH == 4 ? traverse4<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0
+                                             : traverse2<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0);
+                if (!occ) add_shadow(rad, sc_a, k, __float_as_uint(d.w));
             }
         }
     }
@@ -197,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // SHADOW selects the any-hit shadow queue; otherwise the closest-hit queue.
 constexpr uint32_t kRefill = 40;
 
-template <int WIDTH, int BLOCK, bool SHADOW>
+template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
@@ -207,20 +254,18 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
         atomicAdd(SHADOW ? &P.counters->rays_shadow
                          : (b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension),
                   (unsigned long long)n);
-    const float4* __restrict__ nodes = static_cast<const float4*>(WIDTH == 4 ? P.nodes4 : P.nodes);
-    const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
     const float4* __restrict__ ro = static_cast<const float4*>(SHADOW ? S.sh_o : S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(SHADOW ? S.sh_d : S.ray_d[b & 1u]);
     const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
     uint2* hit = static_cast<uint2*>(S.hit);
     float4* rad = static_cast<float4*>(S.radiance);
     uint32_t* cursor = &S.fetch[2 * b + (SHADOW ? 1 : 0)];
-    const bool have_bvh = P.num_nodes != 0;
 
     TravState ts;
     bool busy = false, exhausted = false;
-    uint32_t idx = 0;
-    float4 dray = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t idx = 0, sid = 0, path = 0;
+    const float4* nodes = nullptr;
+    const float4* tris = nullptr;
     for (;;) {
         // refill idle lanes (wave-uniform: every lane reaches the ballot)
         const unsigned long long need = __ballot(!busy && !exhausted);
@@ -232,21 +277,19 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 idx = base + lanes_below(need);
                 if (idx < n) {
                     const float4 o = ro[idx];
-                    dray = rd[idx];
-                    trav_begin(ts, Ray{xyz(o), xyz(dray)}, kTMin, SHADOW ? o.w : kTFar);
+                    const float4 d = rd[idx];
+                    path = __float_as_uint(d.w);
+                    sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
+                    const SceneDesc& sc = scene_at<BATCH>(P, sid);
+                    nodes = scene_nodes<WIDTH, BATCH>(P, sid);
+                    tris = static_cast<const float4*>(sc.tris);
+                    trav_begin(ts, Ray{xyz(o), xyz(d)}, kTMin, SHADOW ? o.w : kTFar);
                     st.sp = 0;
                     busy = true;
-                    if (!have_bvh) {
+                    if (sc.num_nodes == 0) {
                         // nothing to intersect: the ray is finished already
-                        if (SHADOW) {
-                            const uint32_t p = __float_as_uint(dray.w);
-                            const float4 c = sc_a[idx];
-                            float4 l = rad[p];
-                            l.x += c.x; l.y += c.y; l.z += c.z;
-                            rad[p] = l;
-                        } else {
-                            hit[idx] = make_uint2(__float_as_uint(kTFar), uint32_t(-1));
-                        }
+                        if (SHADOW) add_shadow(rad, sc_a, idx, path);
+                        else hit[idx] = make_uint2(__float_as_uint(kTFar), uint32_t(-1));
                         busy = false;
                     }
                 } else {
@@ -263,13 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             if (busy && trav_step<WIDTH, SHADOW>(nodes, tris, ts, st)) {
                 busy = false;
                 if (SHADOW) {
-                    if (ts.best_tri < 0) {
-                        const uint32_t p = __float_as_uint(dray.w);
-                        const float4 c = sc_a[idx];
-                        float4 l = rad[p];
-                        l.x += c.x; l.y += c.y; l.z += c.z;
-                        rad[p] = l;
-                    }
+                    if (ts.best_tri < 0) add_shadow(rad, sc_a, idx, path);
                 } else {
                     hit[idx] = make_uint2(__float_as_uint(ts.best_t), uint32_t(ts.best_tri));
                 }
@@ -281,9 +318,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
 }
 
 // Per pixel: the pass's samples in sample order, added to the accumulation.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_accumulate(PassParams P, WaveState S) {
-    const uint32_t npix = P.width * P.height;
-    const uint32_t spp = S.paths / npix;
+    const uint32_t spp = P.sample_count;
+    const uint32_t s0 = pass_sample_begin(P);
     const float4* rad = static_cast<const float4*>(S.radiance);
     const float4* a0 = static_cast<const float4*>(S.aov0);
     const float4* a1 = static_cast<const float4*>(S.aov1);
@@ -291,24 +329,34 @@ __global__ __launch_bounds__(256) void k_accumulate(PassParams P, WaveState S) {
     float4* aa = static_cast<float4*>(P.accum_albedo);
     float4* an = static_cast<float4*>(P.accum_normal);
     float4* ad = static_cast<float4*>(P.accum_depth);
-    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < npix; pix += gridDim.x * blockDim.x) {
+    for (uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x; gp < P.total_pixels; gp += gridDim.x * blockDim.x) {
+        const uint32_t sid = BATCH ? scene_of_pixel(P, gp) : 0u;
+        const SceneDesc& sc = scene_at<BATCH>(P, sid);
+        const uint32_t npix = sc.width * sc.height;
+        const uint32_t pix = gp - sc.pix_begin;
         float4 sb = {0, 0, 0, 0}, sa = {0, 0, 0, 0}, sn = {0, 0, 0, 0}, sd = {0, 0, 0, 0};
         for (uint32_t s = 0; s < spp; ++s) {
-            const uint32_t p = s * npix + pix;
+            const uint32_t p = sc.path_begin + s * npix + pix;
             const float4 l = rad[p], x0 = a0[p], x1 = a1[p];
             const float dn = fminf(fmaxf((x0.w - 1.0f) / 4.0f, 0.0f), 1.0f);   // plugin.mm:301
             sb.x += l.x; sb.y += l.y; sb.z += l.z; sb.w += 1.0f;
             sa.x += x0.x; sa.y += x0.y; sa.z += x0.z; sa.w += 1.0f;
             sn.x += x1.x; sn.y += x1.y; sn.z += x1.z; sn.w += 1.0f;
             sd.x += dn; sd.y += dn; sd.z += dn; sd.w += 1.0f;
-            if (P.record_ids && P.sample_begin + s == P.id_sample) P.ids[pix] = __float_as_uint(x1.w);
+            if (P.record_ids && s0 + s == P.id_sample) P.ids[gp] = __float_as_uint(x1.w);
         }
         float4 o;
-        o = ab[pix]; o.x += sb.x; o.y += sb.y; o.z += sb.z; o.w += sb.w; ab[pix] = o;
-        o = aa[pix]; o.x += sa.x; o.y += sa.y; o.z += sa.z; o.w += sa.w; aa[pix] = o;
-        o = an[pix]; o.x += sn.x; o.y += sn.y; o.z += sn.z; o.w += sn.w; an[pix] = o;
-        o = ad[pix]; o.x += sd.x; o.y += sd.y; o.z += sd.z; o.w += sd.w; ad[pix] = o;
+        o = ab[gp]; o.x += sb.x; o.y += sb.y; o.z += sb.z; o.w += sb.w; ab[gp] = o;
+        o = aa[gp]; o.x += sa.x; o.y += sa.y; o.z += sa.z; o.w += sa.w; aa[gp] = o;
+        o = an[gp]; o.x += sn.x; o.y += sn.y; o.z += sn.z; o.w += sn.w; an[gp] = o;
+        o = ad[gp]; o.x += sd.x; o.y += sd.y; o.z += sd.z; o.w += sd.w; ad[gp] = o;
     }
+}
+
+// Moves the device-side sample index on by one pass (last node of a
+// captured pass graph).
+__global__ void k_advance(uint32_t* sample_begin, uint32_t by) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sample_begin += by;
 }
 
 uint32_t grid_for(size_t n, uint32_t block, uint32_t cap) {
@@ -318,65 +366,68 @@ uint32_t grid_for(size_t n, uint32_t block, uint32_t cap) {
 
 constexpr int kShadeBlock = 256;
 
+using TraceFn = hipError_t (*)(const PassParams&, const WaveState&, uint32_t, uint32_t, hipStream_t);
+using OccFn = int (*)();
+
+template <class K>
+int occupancy_of(K kernel, int block) {
+    int a = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, kernel, block, 0) != hipSuccess) a = 1;
+    return a;
+}
+
+// CHUNK 64 / 256: static chunks per wave; CHUNK 1: per-lane dynamic fetch.
 template <int WIDTH, int BLOCK, int CHUNK>
 struct TraceVariant {
+    static constexpr int kStatic = CHUNK == 1 ? 64 : CHUNK;
+    template <bool BATCH>
     static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        hipLaunchKernelGGL((k_extend<WIDTH, BLOCK, CHUNK>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        if (CHUNK == 1)
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, false, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        else
+            hipLaunchKernelGGL((k_extend<WIDTH, BLOCK, kStatic, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         return hipGetLastError();
     }
+    template <bool BATCH>
     static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        hipLaunchKernelGGL((k_shadow<WIDTH, BLOCK, CHUNK>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        if (CHUNK == 1)
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, true, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        else
+            hipLaunchKernelGGL((k_shadow<WIDTH, BLOCK, kStatic, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         return hipGetLastError();
     }
     static int occupancy() {
-        int a = 0, c = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<WIDTH, BLOCK, CHUNK>, BLOCK, 0) != hipSuccess) a = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<WIDTH, BLOCK, CHUNK>, BLOCK, 0) != hipSuccess) c = 1;
+        int a, c;
+        if (CHUNK == 1) {
+            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, true>, BLOCK);
+            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, true>, BLOCK);
+        } else {
+            a = occupancy_of(k_extend<WIDTH, BLOCK, kStatic, true>, BLOCK);
+            c = occupancy_of(k_shadow<WIDTH, BLOCK, kStatic, true>, BLOCK);
+        }
         return a < c ? a : c;
     }
 };
 
-// queue_chunk == 1: per-lane dynamic fetch.
-template <int WIDTH, int BLOCK>
-struct DynamicVariant {
-    static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
-        return hipGetLastError();
-    }
-    static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
-        return hipGetLastError();
-    }
-    static int occupancy() {
-        int a = 0, c = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace_dynamic<WIDTH, BLOCK, false>, BLOCK, 0) != hipSuccess) a = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_trace_dynamic<WIDTH, BLOCK, true>, BLOCK, 0) != hipSuccess) c = 1;
-        return a < c ? a : c;
-    }
-};
-
-using ExtendFn = hipError_t (*)(const PassParams&, const WaveState&, uint32_t, uint32_t, hipStream_t);
-using OccFn = int (*)();
 struct VariantEntry {
     uint32_t width, block, chunk;
-    ExtendFn extend, shadow;
+    TraceFn extend1, shadow1;   // one scene
+    TraceFn extendB, shadowB;   // batch
     OccFn occupancy;
 };
 
-#define QH_VARIANT(W, B, C) {W, B, C, &TraceVariant<W, B, C>::extend, &TraceVariant<W, B, C>::shadow, \
-                             &TraceVariant<W, B, C>::occupancy}
-#define QH_DYNAMIC(W, B) {W, B, 1, &DynamicVariant<W, B>::extend, &DynamicVariant<W, B>::shadow, \
-                         &DynamicVariant<W, B>::occupancy}
+#define QH_VARIANT(W, B, C)                                                                                  \
+    {W, B, C, &TraceVariant<W, B, C>::extend<false>, &TraceVariant<W, B, C>::shadow<false>,                  \
+     &TraceVariant<W, B, C>::extend<true>, &TraceVariant<W, B, C>::shadow<true>, &TraceVariant<W, B, C>::occupancy}
 const VariantEntry kVariants[] = {
     QH_VARIANT(2, 64, 64),  QH_VARIANT(2, 128, 64),  QH_VARIANT(2, 256, 64),
     QH_VARIANT(2, 64, 256), QH_VARIANT(2, 128, 256), QH_VARIANT(2, 256, 256),
     QH_VARIANT(4, 64, 64),  QH_VARIANT(4, 128, 64),  QH_VARIANT(4, 256, 64),
     QH_VARIANT(4, 64, 256), QH_VARIANT(4, 128, 256), QH_VARIANT(4, 256, 256),
-    QH_DYNAMIC(2, 64), QH_DYNAMIC(2, 128), QH_DYNAMIC(2, 256),
-    QH_DYNAMIC(4, 64), QH_DYNAMIC(4, 128), QH_DYNAMIC(4, 256),
+    QH_VARIANT(2, 64, 1),   QH_VARIANT(2, 128, 1),   QH_VARIANT(2, 256, 1),
+    QH_VARIANT(4, 64, 1),   QH_VARIANT(4, 128, 1),   QH_VARIANT(4, 256, 1),
 };
 #undef QH_VARIANT
-#undef QH_DYNAMIC
 
 const VariantEntry* find_variant(const TraceConfig& c) {
     for (const VariantEntry& v : kVariants)
@@ -412,32 +463,42 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
     if (s.paths == 0) return hipSuccess;
     const VariantEntry* v = find_variant(cfg);
     if (!v || p.max_depth > kMaxDepth) return hipErrorInvalidValue;
+    const bool batch = p.num_scenes > 0;
     const uint32_t gen_blocks = grid_for(s.paths, 256, 16384);
     const uint32_t shade_blocks = grid_for(s.paths, kShadeBlock, 8192);
     auto before = [&](uint32_t k) { if (tm && tm->before) tm->before(tm->user, k, stream); };
     auto after = [&](uint32_t k) { if (tm && tm->after) tm->after(tm->user, k, stream); };
     before(WAVE_K_RAYGEN);
-    hipLaunchKernelGGL(k_raygen, dim3(gen_blocks), dim3(256), 0, stream, p, s);
+    if (batch) hipLaunchKernelGGL(k_raygen<true>, dim3(gen_blocks), dim3(256), 0, stream, p, s);
+    else hipLaunchKernelGGL(k_raygen<false>, dim3(gen_blocks), dim3(256), 0, stream, p, s);
     after(WAVE_K_RAYGEN);
-    const bool shadows = p.integrator != 0u && p.num_lights != 0u;
+    // shadow kernels run whenever next-event estimation is on (an empty
+    // shadow queue makes them exit at once)
+    const bool shadows = p.integrator != 0u;
     for (uint32_t b = 0; b < p.max_depth; ++b) {
         before(WAVE_K_EXTEND);
-        hipError_t e = v->extend(p, s, b, trace_blocks, stream);
+        hipError_t e = (batch ? v->extendB : v->extend1)(p, s, b, trace_blocks, stream);
         after(WAVE_K_EXTEND);
         if (e != hipSuccess) return e;
         before(WAVE_K_SHADE);
-        hipLaunchKernelGGL((k_shade<kShadeBlock>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+        if (batch) hipLaunchKernelGGL((k_shade<kShadeBlock, true>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+        else hipLaunchKernelGGL((k_shade<kShadeBlock, false>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
         after(WAVE_K_SHADE);
         if (shadows) {
             before(WAVE_K_SHADOW);
-            e = v->shadow(p, s, b, trace_blocks, stream);
+            e = (batch ? v->shadowB : v->shadow1)(p, s, b, trace_blocks, stream);
             after(WAVE_K_SHADOW);
             if (e != hipSuccess) return e;
         }
     }
     before(WAVE_K_ACCUMULATE);
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(size_t(p.width) * p.height, 256, 16384)), dim3(256), 0, stream, p, s);
+    const uint32_t acc_blocks = grid_for(p.total_pixels, 256, 16384);
+    if (batch) hipLaunchKernelGGL(k_accumulate<true>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
+    else hipLaunchKernelGGL(k_accumulate<false>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
     after(WAVE_K_ACCUMULATE);
+    if (p.dyn_sample_begin)
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, const_cast<uint32_t*>(p.dyn_sample_begin),
+                           p.sample_count);
     return hipGetLastError();
 }
 

@@ -2,12 +2,11 @@
 //
 // Replaces the host side of the reference backend, backends/metal/plugin.mm:
 // the 10 exported symbols (plugin.mm:543-902, declared in
-// src/quasi/plugin/plugin_interface.hpp:108-148) plus the optional extension
-// exports declared in include/quasi/quasi_hip_ext.h.  No exception and no HIP
-// error crosses the ABI: failures are logged through Q_plugin_context.log and
+// src/quasi/plugin/plugin_interface.hpp:108-148) plus the single-scene
+// extension exports declared in include/quasi/quasi_hip_ext.h (the batch and
+// autotuner exports live in plugin_batch.cpp).  No exception and no HIP error
+// crosses the ABI: failures are logged through Q_plugin_context.log and
 // reported as NULL handles, NULL readback data or negative return codes.
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -20,91 +19,18 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/quasi/plugin_interface.h"
-#include "../../include/quasi/quasi_hip_ext.h"
 #include "bvh_build.hpp"
-#include "common.hpp"
-#include "kernels.hpp"
+#include "plugin_state.hpp"
 #include "scene_cornell.hpp"
 
 #define Q_EXPORT __attribute__((visibility("default")))
 
-using namespace quasi_hip;
+namespace quasi_hip {
+namespace host {
 
-namespace {
-
-constexpr const char* kName = "HIP Path Tracer";
-constexpr const char* kDescription = "Cornell Box path tracer using HIP on AMD Instinct (gfx950)";
-constexpr const char* kAuthor = "Quasi";
-
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-};
-
-enum KernelSlot : uint32_t {
-    K_MEGAKERNEL = 0, K_RESOLVE, K_FILL,
-    K_RAYGEN, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUMULATE,  // wavefront, in WaveKernel order
-    K_COUNT
-};
-const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve", "fill", "wf_raygen", "wf_extend",
-                                           "wf_shade", "wf_shadow", "wf_accumulate"};
-constexpr bool kRenderKernel[K_COUNT] = {true, false, false, true, true, true, true, true};
-
-// Wavefront path-state buffers (see WaveState in common.hpp).
-struct WaveBuffers {
-    DeviceBuffer ray_o[2], ray_d[2], hit, thr, radiance, rng, aov0, aov1, sh_o, sh_d, sh_c;
-    DeviceBuffer counts, shadow_counts, fetch;
-    uint32_t capacity = 0;
-};
-
-struct TimedLaunch {
-    uint32_t slot;
-    hipEvent_t start, stop;
-};
-
-struct plugin_state {
-    Q_plugin_context* context = nullptr;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    hipStream_t last_stream = nullptr;
-
-    Q_render_params params{};
-
-    // host copy of the scene
-    std::vector<float> verts;
-    std::vector<uint32_t> indices;
-    std::vector<float> albedo, emission;
-
-    // device scene
-    DeviceBuffer tris, mats, nodes, nodes4, lights, gstack;
-    uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
-    uint32_t num_nodes4 = 0, bvh_depth4 = 0, gstack_entries = 0;
-    double bvh_build_ms = 0;
-
-    // framebuffer
-    uint32_t width = 0, height = 0;
-    uint32_t sample_count = 0;
-    DeviceBuffer accum[Q_AOV_COUNT];
-    DeviceBuffer ids;
-    DeviceBuffer resolved;
-    DeviceBuffer counters;
-
-    // wavefront engine
-    WaveBuffers wave;
-    TraceConfig trace_cfg{2, 256, 64};
-    uint32_t trace_blocks = 0;
-    hipEvent_t pending_start = nullptr;
-    int num_cus = 256;
-
-    // stats
-    std::vector<hipEvent_t> event_pool;
-    std::vector<TimedLaunch> timed;
-    uint64_t launches[K_COUNT] = {};
-    double kernel_ms[K_COUNT] = {};
-    double render_ms = 0;
-};
+const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve",  "fill",         "wf_raygen", "wf_extend",
+                                           "wf_shade",   "wf_shadow", "wf_accumulate", "batch_pass"};
+const bool kRenderKernel[K_COUNT] = {true, false, false, true, true, true, true, true, true};
 
 void log_msg(plugin_state* st, const char* fmt, ...) {
     if (!st || !st->context || !st->context->log) return;
@@ -114,10 +40,6 @@ void log_msg(plugin_state* st, const char* fmt, ...) {
     std::vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     st->context->log(st->context->host_data, buf);
-}
-
-void log_ctx(Q_plugin_context* ctx, const char* msg) {
-    if (ctx && ctx->log) ctx->log(ctx->host_data, msg);
 }
 
 bool hip_ok(plugin_state* st, hipError_t e, const char* what) {
@@ -155,30 +77,24 @@ hipEvent_t take_event(plugin_state* st) {
     return e;
 }
 
-// Brackets one launch with events when kernel timing is on.
-struct LaunchTimer {
-    plugin_state* st;
-    hipStream_t s;
-    uint32_t slot;
-    hipEvent_t a = nullptr, b = nullptr;
-    LaunchTimer(plugin_state* st_, hipStream_t s_, uint32_t slot_) : st(st_), s(s_), slot(slot_) {
-        st->launches[slot]++;
-        if (st->params.flags & Q_FLAG_TIME_KERNELS) {
-            a = take_event(st);
-            b = take_event(st);
-            if (a && b) (void)hipEventRecord(a, s);
-        }
+LaunchTimer::LaunchTimer(plugin_state* st_, hipStream_t s_, uint32_t slot_) : st(st_), s(s_), slot(slot_) {
+    st->launches[slot]++;
+    if (st->params.flags & Q_FLAG_TIME_KERNELS) {
+        a = take_event(st);
+        b = take_event(st);
+        if (a && b) (void)hipEventRecord(a, s);
     }
-    ~LaunchTimer() {
-        if (a && b) {
-            (void)hipEventRecord(b, s);
-            st->timed.push_back({slot, a, b});
-        } else {
-            if (a) st->event_pool.push_back(a);
-            if (b) st->event_pool.push_back(b);
-        }
+}
+
+LaunchTimer::~LaunchTimer() {
+    if (a && b) {
+        (void)hipEventRecord(b, s);
+        st->timed.push_back({slot, a, b});
+    } else {
+        if (a) st->event_pool.push_back(a);
+        if (b) st->event_pool.push_back(b);
     }
-};
+}
 
 // Sums finished event pairs into the per-kernel totals (synchronises on them).
 void harvest_events(plugin_state* st) {
@@ -191,6 +107,108 @@ void harvest_events(plugin_state* st) {
     }
     st->timed.clear();
 }
+
+bool sync_render(plugin_state* st) {
+    if (st->last_stream && !hip_ok(st, hipStreamSynchronize(st->last_stream), "render synchronise")) return false;
+    return hip_ok(st, hipStreamSynchronize(st->stream), "stream synchronise");
+}
+
+int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* albedo, const float* emission, unsigned threads, SceneHost& out) {
+    for (size_t i = 0; i < 3 * size_t(nt); ++i)
+        if (idx[i] >= nv) {
+            log_msg(st, "scene: vertex index %u out of range (%u vertices)", idx[i], nv);
+            return Q_EXT_ERR_ARGUMENT;
+        }
+    std::vector<float> p0(3 * size_t(nt)), p1(3 * size_t(nt)), p2(3 * size_t(nt));
+    for (size_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            p0[3 * t + k] = verts[3 * size_t(idx[3 * t + 0]) + k];
+            p1[3 * t + k] = verts[3 * size_t(idx[3 * t + 1]) + k];
+            p2[3 * t + k] = verts[3 * size_t(idx[3 * t + 2]) + k];
+        }
+    BvhBuildResult bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, threads);
+    if (bvh.depth > megakernel_max_depth()) {
+        log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", bvh.depth, megakernel_max_depth());
+        return Q_EXT_ERR_UNSUPPORTED;
+    }
+    std::vector<uint32_t> bvh_pos(nt);
+    out.tri4.assign(12 * size_t(nt), 0.f);
+    out.mat4.assign(8 * size_t(nt), 0.f);
+    for (uint32_t i = 0; i < nt; ++i) {
+        const uint32_t t = bvh.order[i];
+        bvh_pos[t] = i;
+        float* d = &out.tri4[12 * size_t(i)];
+        for (int k = 0; k < 3; ++k) {
+            d[k] = p0[3 * size_t(t) + k];
+            d[4 + k] = p1[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+            d[8 + k] = p2[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+        }
+        std::memcpy(&d[3], &t, 4);
+        float* m = &out.mat4[8 * size_t(i)];
+        const float* a = &albedo[3 * size_t(t)];
+        const float* e = &emission[3 * size_t(t)];
+        m[0] = a[0]; m[1] = a[1]; m[2] = a[2]; m[3] = std::max(e[0], std::max(e[1], e[2]));
+        m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
+    }
+    // Light table in input-triangle order: weight = area * (Le.r + Le.g + Le.b),
+    // prefix sums in double, cdf stored as float with the last entry set to 1.
+    out.lights.clear();
+    std::vector<double> wts;
+    for (uint32_t t = 0; t < nt; ++t) {
+        const float* e = &emission[3 * size_t(t)];
+        if (!(std::max(e[0], std::max(e[1], e[2])) > 0.1f)) continue;
+        double a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = double(p1[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
+            b[k] = double(p2[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
+        }
+        const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        out.lights.push_back({bvh_pos[t], 0.f, 0.f, float(area)});
+        wts.push_back(area * (double(e[0]) + double(e[1]) + double(e[2])));
+    }
+    double total = 0;
+    for (double w : wts) total += w;
+    double run = 0;
+    for (size_t i = 0; i < out.lights.size(); ++i) {
+        run += wts[i];
+        out.lights[i].cdf = total > 0 ? float(run / total) : 0.f;
+        out.lights[i].psel = total > 0 ? float(wts[i] / total) : 0.f;
+    }
+    if (!out.lights.empty()) out.lights.back().cdf = 1.0f;
+    out.nodes = std::move(bvh.nodes);
+    out.nodes4 = std::move(bvh.nodes4);
+    out.num_tris = nt;
+    out.num_nodes = bvh.num_nodes;
+    out.num_nodes4 = bvh.num_nodes4;
+    out.depth = bvh.depth;
+    out.depth4 = bvh.depth4;
+    return Q_EXT_OK;
+}
+
+uint32_t stack_overflow_entries(uint32_t depth, uint32_t depth4) {
+    // the BVH2 walk keeps at most one entry per level, the BVH4 walk at most three
+    const uint32_t need = std::max(depth, 3 * depth4);
+    return need > uint32_t(kShortStack) ? need - uint32_t(kShortStack) : 0u;
+}
+
+bool ensure_gstack(plugin_state* st, uint32_t entries) {
+    const size_t threads = size_t(st->num_cus) * kMaxThreadsPerCU;
+    return ensure_buffer(st, st->gstack, std::max<size_t>(4, size_t(entries) * threads * 4), "traversal stacks");
+}
+
+namespace {
+
+plugin_state* as_state(Q_plugin_handle* h) { return reinterpret_cast<plugin_state*>(h); }
+
+void log_ctx(Q_plugin_context* ctx, const char* msg) {
+    if (ctx && ctx->log) ctx->log(ctx->host_data, msg);
+}
+
+constexpr const char* kName = "HIP Path Tracer";
+constexpr const char* kDescription = "Cornell Box path tracer using HIP on AMD Instinct (gfx950)";
+constexpr const char* kAuthor = "Quasi";
 
 Q_render_params default_params() {
     Q_render_params p{};
@@ -235,102 +253,36 @@ bool resize_framebuffer(plugin_state* st, uint32_t w, uint32_t h, hipStream_t s)
     return clear_accumulation(st, s);
 }
 
-// Uploads a mesh: BVH build on the host, then triangle / material / node /
-// light arrays in the layout of common.hpp.
+// Builds and uploads the single scene held in st->verts / indices / albedo / emission.
 int32_t upload_scene(plugin_state* st) {
     const uint32_t nt = uint32_t(st->indices.size() / 3);
     const uint32_t nv = uint32_t(st->verts.size() / 3);
-    for (uint32_t i : st->indices)
-        if (i >= nv) {
-            log_msg(st, "scene: vertex index %u out of range (%u vertices)", i, nv);
-            return Q_EXT_ERR_ARGUMENT;
-        }
     auto t0 = std::chrono::steady_clock::now();
-    std::vector<float> p0(3 * size_t(nt)), p1(3 * size_t(nt)), p2(3 * size_t(nt));
-    for (size_t t = 0; t < nt; ++t)
-        for (int k = 0; k < 3; ++k) {
-            p0[3 * t + k] = st->verts[3 * size_t(st->indices[3 * t + 0]) + k];
-            p1[3 * t + k] = st->verts[3 * size_t(st->indices[3 * t + 1]) + k];
-            p2[3 * t + k] = st->verts[3 * size_t(st->indices[3 * t + 2]) + k];
-        }
-    BvhBuildResult bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, 0);
-    if (bvh.depth > megakernel_max_depth()) {
-        log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", bvh.depth, megakernel_max_depth());
-        return Q_EXT_ERR_UNSUPPORTED;
-    }
-    std::vector<uint32_t> bvh_pos(nt);
-    std::vector<float> tri4(12 * size_t(nt)), mat4(8 * size_t(nt));
-    for (uint32_t i = 0; i < nt; ++i) {
-        const uint32_t t = bvh.order[i];
-        bvh_pos[t] = i;
-        float* d = &tri4[12 * size_t(i)];
-        for (int k = 0; k < 3; ++k) {
-            d[k] = p0[3 * size_t(t) + k];
-            d[4 + k] = p1[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
-            d[8 + k] = p2[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
-        }
-        std::memcpy(&d[3], &t, 4);
-        d[7] = 0.f;
-        d[11] = 0.f;
-        float* m = &mat4[8 * size_t(i)];
-        const float* a = &st->albedo[3 * size_t(t)];
-        const float* e = &st->emission[3 * size_t(t)];
-        m[0] = a[0]; m[1] = a[1]; m[2] = a[2]; m[3] = std::max(e[0], std::max(e[1], e[2]));
-        m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
-    }
-    // Light table in input-triangle order: weight = area * (Le.r + Le.g + Le.b),
-    // prefix sums in double, cdf stored as float with the last entry set to 1.
-    std::vector<LightEntry> lights;
-    std::vector<double> wts;
-    for (uint32_t t = 0; t < nt; ++t) {
-        const float* e = &st->emission[3 * size_t(t)];
-        if (!(std::max(e[0], std::max(e[1], e[2])) > 0.1f)) continue;
-        double a[3], b[3];
-        for (int k = 0; k < 3; ++k) {
-            a[k] = double(p1[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
-            b[k] = double(p2[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
-        }
-        const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
-        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-        lights.push_back({bvh_pos[t], 0.f, 0.f, float(area)});
-        wts.push_back(area * (double(e[0]) + double(e[1]) + double(e[2])));
-    }
-    double total = 0;
-    for (double w : wts) total += w;
-    double run = 0;
-    for (size_t i = 0; i < lights.size(); ++i) {
-        run += wts[i];
-        lights[i].cdf = total > 0 ? float(run / total) : 0.f;
-        lights[i].psel = total > 0 ? float(wts[i] / total) : 0.f;
-    }
-    if (!lights.empty()) lights.back().cdf = 1.0f;
-
-    // traversal-stack overflow area of the persistent kernels: the BVH2 walk
-    // keeps at most one entry per level, the BVH4 walk at most three
-    const uint32_t need = std::max(bvh.depth, 3 * bvh.depth4);
-    const uint32_t gentries = need > uint32_t(kShortStack) ? need - uint32_t(kShortStack) : 0u;
-    const size_t gthreads = size_t(st->num_cus) * kMaxThreadsPerCU;
-    if (!ensure_buffer(st, st->tris, std::max<size_t>(16, tri4.size() * 4), "triangles") ||
-        !ensure_buffer(st, st->mats, std::max<size_t>(16, mat4.size() * 4), "materials") ||
-        !ensure_buffer(st, st->nodes, std::max<size_t>(64, bvh.nodes.size() * 4), "BVH nodes") ||
-        !ensure_buffer(st, st->nodes4, std::max<size_t>(128, bvh.nodes4.size() * 4), "BVH4 nodes") ||
-        !ensure_buffer(st, st->gstack, std::max<size_t>(4, size_t(gentries) * gthreads * 4), "traversal stacks") ||
-        !ensure_buffer(st, st->lights, std::max<size_t>(16, lights.size() * sizeof(LightEntry)), "lights"))
+    SceneHost sh;
+    int32_t rc = prepare_scene(st, st->verts.data(), nv, st->indices.data(), nt, st->albedo.data(),
+                               st->emission.data(), 0, sh);
+    if (rc != Q_EXT_OK) return rc;
+    const uint32_t gentries = std::max(stack_overflow_entries(sh.depth, sh.depth4), st->batch.gstack_entries);
+    if (!ensure_buffer(st, st->tris, std::max<size_t>(16, sh.tri4.size() * 4), "triangles") ||
+        !ensure_buffer(st, st->mats, std::max<size_t>(16, sh.mat4.size() * 4), "materials") ||
+        !ensure_buffer(st, st->nodes, std::max<size_t>(64, sh.nodes.size() * 4), "BVH nodes") ||
+        !ensure_buffer(st, st->nodes4, std::max<size_t>(128, sh.nodes4.size() * 4), "BVH4 nodes") ||
+        !ensure_gstack(st, gentries) ||
+        !ensure_buffer(st, st->lights, std::max<size_t>(16, sh.lights.size() * sizeof(LightEntry)), "lights"))
         return Q_EXT_ERR_OUT_OF_MEMORY;
-    hipStream_t s = st->stream;
-    if (!hip_ok(st, hipStreamSynchronize(s), "scene upload") ||
-        (nt && !hip_ok(st, hipMemcpy(st->tris.ptr, tri4.data(), tri4.size() * 4, hipMemcpyHostToDevice), "triangle upload")) ||
-        (nt && !hip_ok(st, hipMemcpy(st->mats.ptr, mat4.data(), mat4.size() * 4, hipMemcpyHostToDevice), "material upload")) ||
-        (!bvh.nodes.empty() && !hip_ok(st, hipMemcpy(st->nodes.ptr, bvh.nodes.data(), bvh.nodes.size() * 4, hipMemcpyHostToDevice), "BVH upload")) ||
-        (!bvh.nodes4.empty() && !hip_ok(st, hipMemcpy(st->nodes4.ptr, bvh.nodes4.data(), bvh.nodes4.size() * 4, hipMemcpyHostToDevice), "BVH4 upload")) ||
-        (!lights.empty() && !hip_ok(st, hipMemcpy(st->lights.ptr, lights.data(), lights.size() * sizeof(LightEntry), hipMemcpyHostToDevice), "light upload")))
+    if (!hip_ok(st, hipStreamSynchronize(st->stream), "scene upload") ||
+        (nt && !hip_ok(st, hipMemcpy(st->tris.ptr, sh.tri4.data(), sh.tri4.size() * 4, hipMemcpyHostToDevice), "triangle upload")) ||
+        (nt && !hip_ok(st, hipMemcpy(st->mats.ptr, sh.mat4.data(), sh.mat4.size() * 4, hipMemcpyHostToDevice), "material upload")) ||
+        (!sh.nodes.empty() && !hip_ok(st, hipMemcpy(st->nodes.ptr, sh.nodes.data(), sh.nodes.size() * 4, hipMemcpyHostToDevice), "BVH upload")) ||
+        (!sh.nodes4.empty() && !hip_ok(st, hipMemcpy(st->nodes4.ptr, sh.nodes4.data(), sh.nodes4.size() * 4, hipMemcpyHostToDevice), "BVH4 upload")) ||
+        (!sh.lights.empty() && !hip_ok(st, hipMemcpy(st->lights.ptr, sh.lights.data(), sh.lights.size() * sizeof(LightEntry), hipMemcpyHostToDevice), "light upload")))
         return Q_EXT_ERR_DEVICE;
     st->num_tris = nt;
-    st->num_nodes = bvh.num_nodes;
-    st->bvh_depth = bvh.depth;
-    st->num_lights = uint32_t(lights.size());
-    st->num_nodes4 = bvh.num_nodes4;
-    st->bvh_depth4 = bvh.depth4;
+    st->num_nodes = sh.num_nodes;
+    st->bvh_depth = sh.depth;
+    st->num_lights = uint32_t(sh.lights.size());
+    st->num_nodes4 = sh.num_nodes4;
+    st->bvh_depth4 = sh.depth4;
     st->gstack_entries = gentries;
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return Q_EXT_OK;
@@ -343,7 +295,43 @@ void free_wave(WaveBuffers& w) {
     w.capacity = 0;
 }
 
-// Path-state buffers for `paths` paths (grown, never shrunk).
+// Samples per wavefront pass: about 16M paths in flight unless the caller set it.
+uint32_t wave_samples_per_pass(const plugin_state* st) {
+    const uint32_t npix = st->width * st->height;
+    uint32_t s = st->params.samples_per_pass;
+    if (s == 0) s = std::max<uint32_t>(1u, (16u << 20) / std::max<uint32_t>(1u, npix));
+    s = std::min(s, st->params.spp_per_render);
+    while (s > 1 && uint64_t(s) * npix > 0xFFFFFFFFull / 2) s /= 2;
+    return std::max(1u, s);
+}
+
+void destroy_state(plugin_state* st) {
+    if (!st) return;
+    (void)hipSetDevice(st->device);
+    if (st->last_stream) (void)hipStreamSynchronize(st->last_stream);
+    if (st->stream) (void)hipStreamSynchronize(st->stream);
+    harvest_events(st);
+    for (hipEvent_t e : st->event_pool) (void)hipEventDestroy(e);
+    st->event_pool.clear();
+    free_batch(st);
+    free_buffer(st->tris);
+    free_buffer(st->mats);
+    free_buffer(st->nodes);
+    free_buffer(st->nodes4);
+    free_buffer(st->lights);
+    free_buffer(st->gstack);
+    for (auto& b : st->accum) free_buffer(b);
+    free_buffer(st->ids);
+    free_buffer(st->resolved);
+    free_buffer(st->counters);
+    free_wave(st->wave);
+    if (st->capture_stream) (void)hipStreamDestroy(st->capture_stream);
+    if (st->own_stream && st->stream) (void)hipStreamDestroy(st->stream);
+    delete st;
+}
+
+}  // namespace
+
 bool ensure_wave(plugin_state* st, uint32_t paths) {
     WaveBuffers& w = st->wave;
     if (w.capacity >= paths) return true;
@@ -403,16 +391,6 @@ void wave_timer_after(void* user, uint32_t kernel, hipStream_t s) {
     st->pending_start = nullptr;
 }
 
-// Samples per wavefront pass: about 16M paths in flight unless the caller set it.
-uint32_t wave_samples_per_pass(const plugin_state* st) {
-    const uint32_t npix = st->width * st->height;
-    uint32_t s = st->params.samples_per_pass;
-    if (s == 0) s = std::max<uint32_t>(1u, (16u << 20) / std::max<uint32_t>(1u, npix));
-    s = std::min(s, st->params.spp_per_render);
-    while (s > 1 && uint64_t(s) * npix > 0xFFFFFFFFull / 2) s /= 2;
-    return std::max(1u, s);
-}
-
 // Host camera packing (plugin.mm:649-674) and the basis of get_camera_ray
 // (plugin.mm:134-145), evaluated once per frame in double.
 CameraBasis make_camera(const Q_camera& c, uint32_t w, uint32_t h) {
@@ -443,46 +421,15 @@ CameraBasis make_camera(const Q_camera& c, uint32_t w, uint32_t h) {
     return cb;
 }
 
-plugin_state* as_state(Q_plugin_handle* h) { return reinterpret_cast<plugin_state*>(h); }
-
-void destroy_state(plugin_state* st) {
-    if (!st) return;
-    (void)hipSetDevice(st->device);
-    if (st->last_stream) (void)hipStreamSynchronize(st->last_stream);
-    if (st->stream) (void)hipStreamSynchronize(st->stream);
-    harvest_events(st);
-    for (hipEvent_t e : st->event_pool) (void)hipEventDestroy(e);
-    st->event_pool.clear();
-    free_buffer(st->tris);
-    free_buffer(st->mats);
-    free_buffer(st->nodes);
-    free_buffer(st->nodes4);
-    free_buffer(st->lights);
-    free_buffer(st->gstack);
-    for (auto& b : st->accum) free_buffer(b);
-    free_buffer(st->ids);
-    free_buffer(st->resolved);
-    free_buffer(st->counters);
-    free_wave(st->wave);
-    if (st->own_stream && st->stream) (void)hipStreamDestroy(st->stream);
-    delete st;
-}
-
-bool sync_render(plugin_state* st) {
-    if (st->last_stream && !hip_ok(st, hipStreamSynchronize(st->last_stream), "render synchronise")) return false;
-    return hip_ok(st, hipStreamSynchronize(st->stream), "stream synchronise");
-}
-
-// Mean of one accumulation buffer into a malloc'd RGBA32F array.
-Q_aov_buffer read_mean(plugin_state* st, uint32_t aov) {
+Q_aov_buffer read_mean_of(plugin_state* st, const void* sums, uint32_t width, uint32_t height, uint32_t n) {
     Q_aov_buffer out{};
-    const size_t npix = size_t(st->width) * st->height;
-    if (st->sample_count == 0 || npix == 0) return out;
+    const size_t npix = size_t(width) * height;
+    if (n == 0 || npix == 0) return out;
     if (!ensure_buffer(st, st->resolved, 16 * npix, "readback staging")) return out;
     {
         LaunchTimer lt(st, st->stream, K_RESOLVE);
-        if (!hip_ok(st, launch_resolve(static_cast<const float4*>(st->accum[aov].ptr),
-                                       static_cast<float4*>(st->resolved.ptr), npix, st->sample_count, st->stream),
+        if (!hip_ok(st, launch_resolve(static_cast<const float4*>(sums), static_cast<float4*>(st->resolved.ptr), npix,
+                                       n, st->stream),
                     "resolve"))
             return out;
     }
@@ -494,13 +441,101 @@ Q_aov_buffer read_mean(plugin_state* st, uint32_t aov) {
         return out;
     }
     out.data = data;
-    out.width = st->width;
-    out.height = st->height;
+    out.width = width;
+    out.height = height;
     out.channels = 4;
     return out;
 }
 
-}  // namespace
+bool clear_single(plugin_state* st, hipStream_t s) { return clear_accumulation(st, s); }
+
+bool render_frame(plugin_state* st, const Q_render_frame* frame) {
+    if (frame->width == 0 || frame->height == 0) return false;
+    (void)hipSetDevice(st->device);
+    hipStream_t s = frame->command_buffer ? static_cast<hipStream_t>(frame->command_buffer) : st->stream;
+    if (st->last_stream && st->last_stream != s) (void)hipStreamSynchronize(st->last_stream);
+    st->last_stream = s;
+    if (frame->width != st->width || frame->height != st->height) {      // plugin.mm:638-640
+        if (!resize_framebuffer(st, frame->width, frame->height, s)) return false;
+    }
+    if (frame->camera_dirty) {                                            // plugin.mm:643-646
+        if (!clear_accumulation(st, s)) return false;
+    }
+    st->last_camera = frame->camera;
+    st->have_camera = true;
+    const Q_render_params& rp = st->params;
+    PassParams p{};
+    p.tris = st->tris.ptr;
+    p.mats = st->mats.ptr;
+    p.nodes = st->nodes.ptr;
+    p.lights = static_cast<const LightEntry*>(st->lights.ptr);
+    p.accum_beauty = st->accum[Q_AOV_BEAUTY].ptr;
+    p.accum_albedo = st->accum[Q_AOV_ALBEDO].ptr;
+    p.accum_normal = st->accum[Q_AOV_NORMAL].ptr;
+    p.accum_depth = st->accum[Q_AOV_DEPTH].ptr;
+    p.ids = static_cast<uint32_t*>(st->ids.ptr);
+    p.counters = static_cast<Counters*>(st->counters.ptr);
+    p.cam = make_camera(frame->camera, st->width, st->height);
+    p.width = st->width;
+    p.height = st->height;
+    p.sample_begin = st->sample_count;
+    p.sample_count = rp.spp_per_render;
+    p.max_depth = rp.max_depth;
+    p.rr_start = rp.rr_start;
+    p.seed = rp.seed;
+    p.rng_mode = rp.rng_mode;
+    p.integrator = rp.integrator;
+    p.num_lights = st->num_lights;
+    p.num_nodes = st->num_nodes;
+    p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
+    p.id_sample = rp.id_sample;
+    p.nodes4 = st->nodes4.ptr;
+    p.num_nodes4 = st->num_nodes4;
+    p.gstack = static_cast<int*>(st->gstack.ptr);
+    p.gstack_entries = st->gstack_entries;
+    if (rp.engine == Q_ENGINE_WAVEFRONT) {
+        SceneDesc& sd = p.scene0;
+        sd.tris = p.tris;
+        sd.mats = p.mats;
+        sd.nodes = p.nodes;
+        sd.nodes4 = p.nodes4;
+        sd.lights = p.lights;
+        sd.num_nodes = p.num_nodes;
+        sd.num_lights = p.num_lights;
+        sd.cam = p.cam;
+        sd.width = p.width;
+        sd.height = p.height;
+        sd.path_begin = 0;
+        sd.pix_begin = 0;
+        sd.seed = rp.seed;
+        p.num_scenes = 0;
+        p.total_pixels = st->width * st->height;
+        const uint32_t per_pass = wave_samples_per_pass(st);
+        const uint32_t npix = st->width * st->height;
+        if (!ensure_wave(st, per_pass * npix)) return false;
+        WaveTimers tm{st, wave_timer_before, wave_timer_after};
+        for (uint32_t done = 0; done < rp.spp_per_render; done += per_pass) {
+            const uint32_t n = std::min(per_pass, rp.spp_per_render - done);
+            p.sample_begin = st->sample_count + done;
+            p.sample_count = n;
+            const WaveState ws = wave_state(st->wave, n * npix);
+            if (!hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, &tm),
+                        "wavefront launch"))
+                return false;
+        }
+    } else {
+        LaunchTimer lt(st, s, K_MEGAKERNEL);
+        if (!hip_ok(st, launch_megakernel(p, st->bvh_depth, s), "megakernel launch")) return false;
+    }
+    st->sample_count += rp.spp_per_render;                                // plugin.mm:791
+    return true;
+}
+
+}  // namespace host
+}  // namespace quasi_hip
+
+using namespace quasi_hip;
+using namespace quasi_hip::host;
 
 extern "C" {
 
@@ -542,6 +577,9 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
         if (hipDeviceGetAttribute(&st->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             st->num_cus <= 0)
             st->num_cus = 256;
+        hipDeviceProp_t prop{};
+        const bool have_prop = hipGetDeviceProperties(&prop, dev) == hipSuccess;
+        std::snprintf(st->arch, sizeof st->arch, "%s", have_prop ? prop.gcnArchName : "unknown");
         st->trace_cfg = TraceConfig{2, 256, 64};
         st->trace_blocks = persistent_blocks(dev, st->trace_cfg);
         if (ctx->gpu->queue) {
@@ -566,8 +604,7 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
             destroy_state(st);
             return nullptr;
         }
-        hipDeviceProp_t prop{};
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        if (have_prop)
             log_msg(st, "Cornell Box path tracer initialized (HIP device %d: %s, %s)", dev, prop.name, prop.gcnArchName);
         else
             log_msg(st, "Cornell Box path tracer initialized (HIP device %d)", dev);
@@ -598,66 +635,7 @@ Q_EXPORT void Q_plugin_render(Q_plugin_handle* handle, Q_render_frame* frame) {
     if (!handle || !frame) return;
     plugin_state* st = as_state(handle);
     try {
-        if (frame->width == 0 || frame->height == 0) return;
-        (void)hipSetDevice(st->device);
-        hipStream_t s = frame->command_buffer ? static_cast<hipStream_t>(frame->command_buffer) : st->stream;
-        if (st->last_stream && st->last_stream != s) (void)hipStreamSynchronize(st->last_stream);
-        st->last_stream = s;
-        if (frame->width != st->width || frame->height != st->height) {      // plugin.mm:638-640
-            if (!resize_framebuffer(st, frame->width, frame->height, s)) return;
-        }
-        if (frame->camera_dirty) {                                            // plugin.mm:643-646
-            if (!clear_accumulation(st, s)) return;
-        }
-        const Q_render_params& rp = st->params;
-        PassParams p{};
-        p.tris = st->tris.ptr;
-        p.mats = st->mats.ptr;
-        p.nodes = st->nodes.ptr;
-        p.lights = static_cast<const LightEntry*>(st->lights.ptr);
-        p.accum_beauty = st->accum[Q_AOV_BEAUTY].ptr;
-        p.accum_albedo = st->accum[Q_AOV_ALBEDO].ptr;
-        p.accum_normal = st->accum[Q_AOV_NORMAL].ptr;
-        p.accum_depth = st->accum[Q_AOV_DEPTH].ptr;
-        p.ids = static_cast<uint32_t*>(st->ids.ptr);
-        p.counters = static_cast<Counters*>(st->counters.ptr);
-        p.cam = make_camera(frame->camera, st->width, st->height);
-        p.width = st->width;
-        p.height = st->height;
-        p.sample_begin = st->sample_count;
-        p.sample_count = rp.spp_per_render;
-        p.max_depth = rp.max_depth;
-        p.rr_start = rp.rr_start;
-        p.seed = rp.seed;
-        p.rng_mode = rp.rng_mode;
-        p.integrator = rp.integrator;
-        p.num_lights = st->num_lights;
-        p.num_nodes = st->num_nodes;
-        p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
-        p.id_sample = rp.id_sample;
-        p.nodes4 = st->nodes4.ptr;
-        p.num_nodes4 = st->num_nodes4;
-        p.gstack = static_cast<int*>(st->gstack.ptr);
-        p.gstack_entries = st->gstack_entries;
-        if (rp.engine == Q_ENGINE_WAVEFRONT) {
-            const uint32_t per_pass = wave_samples_per_pass(st);
-            const uint32_t npix = st->width * st->height;
-            if (!ensure_wave(st, per_pass * npix)) return;
-            WaveTimers tm{st, wave_timer_before, wave_timer_after};
-            for (uint32_t done = 0; done < rp.spp_per_render; done += per_pass) {
-                const uint32_t n = std::min(per_pass, rp.spp_per_render - done);
-                p.sample_begin = st->sample_count + done;
-                p.sample_count = n;
-                const WaveState ws = wave_state(st->wave, n * npix);
-                if (!hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, &tm),
-                            "wavefront launch"))
-                    return;
-            }
-        } else {
-            LaunchTimer lt(st, s, K_MEGAKERNEL);
-            if (!hip_ok(st, launch_megakernel(p, st->bvh_depth, s), "megakernel launch")) return;
-        }
-        st->sample_count += rp.spp_per_render;                                // plugin.mm:791
+        render_frame(st, frame);
     } catch (...) {
         log_msg(st, "HIP backend: exception during render");
     }
@@ -670,7 +648,7 @@ Q_EXPORT Q_readback_result Q_plugin_readback(Q_plugin_handle* handle) {
     try {
         (void)hipSetDevice(st->device);
         if (!sync_render(st)) return result;
-        Q_aov_buffer b = read_mean(st, Q_AOV_BEAUTY);
+        Q_aov_buffer b = read_mean_of(st, st->accum[Q_AOV_BEAUTY].ptr, st->width, st->height, st->sample_count);
         result.data = b.data;
         result.width = b.width;
         result.height = b.height;
@@ -696,7 +674,8 @@ Q_EXPORT Q_readback_aov_result Q_plugin_readback_aov(Q_plugin_handle* handle) {
     try {
         (void)hipSetDevice(st->device);
         if (!sync_render(st)) return result;
-        for (uint32_t a = 0; a < Q_AOV_COUNT; ++a) result.buffers[a] = read_mean(st, a);
+        for (uint32_t a = 0; a < Q_AOV_COUNT; ++a)
+            result.buffers[a] = read_mean_of(st, st->accum[a].ptr, st->width, st->height, st->sample_count);
         log_msg(st, "AOV readback complete");
     } catch (...) {
         log_msg(st, "HIP backend: exception during AOV readback");
@@ -761,7 +740,7 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
     try {
         Q_render_params p = *in;
         p.struct_size = sizeof(Q_render_params);
-        if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > 16 ||
+        if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > kMaxDepth ||
             p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT) {
             log_msg(st, "set_params: unsupported parameter combination");
             return Q_EXT_ERR_UNSUPPORTED;
@@ -824,7 +803,7 @@ Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out
     s.bvh_build_ms = st->bvh_build_ms;
     s.render_ms = 0;
     for (uint32_t k = 0; k < K_COUNT; ++k)
-        if (kRenderKernel[k]) s.render_ms += st->kernel_ms[k];
+        if (kRenderKernel[k] && k != K_BATCH_PASS) s.render_ms += st->kernel_ms[k];
     s.num_kernels = K_COUNT;
     for (uint32_t k = 0; k < K_COUNT; ++k) {
         std::snprintf(s.kernels[k].name, sizeof s.kernels[k].name, "%s", kKernelNames[k]);
@@ -842,7 +821,6 @@ Q_EXPORT int32_t Q_plugin_reset_stats(Q_plugin_handle* handle) {
     if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
     harvest_events(st);
     for (uint32_t k = 0; k < K_COUNT; ++k) { st->launches[k] = 0; st->kernel_ms[k] = 0; }
-    st->render_ms = 0;
     if (!hip_ok(st, launch_fill_u32(st->counters.ptr, sizeof(Counters) / 4, 0u, st->stream), "counter clear") ||
         !hip_ok(st, hipStreamSynchronize(st->stream), "counter clear"))
         return Q_EXT_ERR_DEVICE;

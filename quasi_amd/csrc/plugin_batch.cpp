@@ -1,0 +1,541 @@
+// plugin_batch.cpp -- batched scenes, captured pass graphs and the autotuner
+// (extension exports of include/quasi/quasi_hip_ext.h; no reference
+// counterpart: the reference renders its one hard-coded scene,
+// backends/metal/plugin.mm:584).
+//
+// A batch concatenates every scene's triangles, materials, BVH nodes and
+// lights into one set of device arrays and keeps a device table of
+// SceneDesc entries pointing into them.  One sample pass covers all scenes
+// with shared ray queues (kernels_wave.hip, BATCH = true); the pass's launch
+// sequence is captured once into a hipGraph and replayed per pass, with the
+// first sample index read from device memory and advanced by the graph's last
+// node, so the host enqueues one graph launch per pass.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "plugin_state.hpp"
+
+#define Q_EXPORT __attribute__((visibility("default")))
+
+namespace quasi_hip {
+namespace host {
+
+void free_batch(plugin_state* st) {
+    BatchState& b = st->batch;
+    if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.graph_exec = nullptr;
+    b.graph = nullptr;
+    b.graph_key = 0;
+    for (auto* x : {&b.tris, &b.mats, &b.nodes, &b.nodes4, &b.lights, &b.table, &b.ids, &b.resolved, &b.sample_begin})
+        free_buffer(*x);
+    for (auto& a : b.accum) free_buffer(a);
+    b.descs.clear();
+    b.active = false;
+    b.num_scenes = b.total_pixels = b.total_tris = b.sample_count = 0;
+}
+
+namespace {
+
+plugin_state* as_state(Q_plugin_handle* h) { return reinterpret_cast<plugin_state*>(h); }
+
+bool clear_batch(plugin_state* st, hipStream_t s) {
+    BatchState& b = st->batch;
+    for (auto& a : b.accum)
+        if (!hip_ok(st, launch_fill_u32(a.ptr, 4 * size_t(b.total_pixels), 0u, s), "batch clear")) return false;
+    if (!hip_ok(st, launch_fill_u32(b.ids.ptr, b.total_pixels, kMissId, s), "batch id clear")) return false;
+    b.sample_count = 0;
+    return true;
+}
+
+uint32_t batch_samples_per_pass(const plugin_state* st) {
+    return st->params.samples_per_pass ? st->params.samples_per_pass : 1u;
+}
+
+// FNV-1a over the values a captured pass depends on.
+struct KeyHash {
+    uint64_t h = 1469598103934665603ull;
+    void add(uint64_t v) {
+        for (int i = 0; i < 8; ++i) { h ^= (v >> (8 * i)) & 0xFF; h *= 1099511628211ull; }
+    }
+};
+
+// The pass parameters of the current batch for passes of `spp_pass` samples.
+PassParams batch_params(plugin_state* st, uint32_t spp_pass) {
+    BatchState& b = st->batch;
+    const Q_render_params& rp = st->params;
+    PassParams p{};
+    p.accum_beauty = b.accum[Q_AOV_BEAUTY].ptr;
+    p.accum_albedo = b.accum[Q_AOV_ALBEDO].ptr;
+    p.accum_normal = b.accum[Q_AOV_NORMAL].ptr;
+    p.accum_depth = b.accum[Q_AOV_DEPTH].ptr;
+    p.ids = static_cast<uint32_t*>(b.ids.ptr);
+    p.counters = static_cast<Counters*>(st->counters.ptr);
+    p.sample_begin = 0;
+    p.sample_count = spp_pass;
+    p.max_depth = rp.max_depth;
+    p.rr_start = rp.rr_start;
+    p.seed = rp.seed;
+    p.rng_mode = rp.rng_mode;
+    p.integrator = rp.integrator;
+    p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
+    p.id_sample = rp.id_sample;
+    p.gstack = static_cast<int*>(st->gstack.ptr);
+    p.gstack_entries = b.gstack_entries;
+    p.scenes = static_cast<const SceneDesc*>(b.table.ptr);
+    p.num_scenes = b.num_scenes;
+    p.total_pixels = b.total_pixels;
+    p.dyn_sample_begin = static_cast<const uint32_t*>(b.sample_begin.ptr);
+    return p;
+}
+
+// Uploads the scene table with path offsets for passes of `spp_pass` samples
+// and (re)captures the pass graph when anything it depends on changed.
+bool prepare_batch_pass(plugin_state* st, uint32_t spp_pass) {
+    BatchState& b = st->batch;
+    const uint64_t paths = uint64_t(b.total_pixels) * spp_pass;
+    if (paths == 0 || paths > 0x7FFFFFFFull) {
+        log_msg(st, "render_batch: %llu paths per pass is out of range", (unsigned long long)paths);
+        return false;
+    }
+    if (!ensure_wave(st, uint32_t(paths))) return false;
+    KeyHash k;
+    k.add(spp_pass);
+    k.add(st->trace_cfg.width); k.add(st->trace_cfg.block); k.add(st->trace_cfg.chunk); k.add(st->trace_blocks);
+    const Q_render_params& rp = st->params;
+    k.add(rp.max_depth); k.add(rp.rr_start); k.add(rp.rng_mode); k.add(rp.integrator);
+    k.add(rp.flags & Q_FLAG_RECORD_IDS); k.add(rp.id_sample);
+    k.add(reinterpret_cast<uintptr_t>(st->wave.ray_o[0].ptr)); k.add(st->wave.capacity);
+    k.add(reinterpret_cast<uintptr_t>(st->gstack.ptr));
+    k.add(reinterpret_cast<uintptr_t>(b.table.ptr));
+    if (k.h == b.graph_key && b.graph_exec) return true;
+
+    // scene table for this pass size
+    for (uint32_t i = 0; i < b.num_scenes; ++i) b.descs[i].path_begin = b.descs[i].pix_begin * spp_pass;
+    if (!hip_ok(st, hipMemcpy(b.table.ptr, b.descs.data(), b.descs.size() * sizeof(SceneDesc), hipMemcpyHostToDevice),
+                "scene table upload"))
+        return false;
+    if (b.graph_exec) { (void)hipGraphExecDestroy(b.graph_exec); b.graph_exec = nullptr; }
+    if (b.graph) { (void)hipGraphDestroy(b.graph); b.graph = nullptr; }
+    if (!st->capture_stream &&
+        !hip_ok(st, hipStreamCreateWithFlags(&st->capture_stream, hipStreamNonBlocking), "capture stream"))
+        return false;
+    const PassParams p = batch_params(st, spp_pass);
+    const WaveState ws = wave_state(st->wave, uint32_t(paths));
+    if (!hip_ok(st, hipStreamBeginCapture(st->capture_stream, hipStreamCaptureModeThreadLocal), "begin capture"))
+        return false;
+    const hipError_t le = launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, st->capture_stream, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st->capture_stream, &g);
+    if (!hip_ok(st, le, "captured pass launch") || !hip_ok(st, ce, "end capture")) {
+        if (g) (void)hipGraphDestroy(g);
+        return false;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (!hip_ok(st, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "graph instantiate")) {
+        (void)hipGraphDestroy(g);
+        return false;
+    }
+    b.graph = g;
+    b.graph_exec = ex;
+    b.graph_key = k.h;
+    return true;
+}
+
+int32_t render_batch(plugin_state* st, uint32_t spp, hipStream_t s) {
+    BatchState& b = st->batch;
+    if (!b.active) {
+        log_msg(st, "render_batch: no batch set");
+        return Q_EXT_ERR_STATE;
+    }
+    const uint32_t per_pass = batch_samples_per_pass(st);
+    if (spp == 0 || spp % per_pass) {
+        log_msg(st, "render_batch: %u samples is not a multiple of %u samples per pass", spp, per_pass);
+        return Q_EXT_ERR_ARGUMENT;
+    }
+    if (st->last_stream && st->last_stream != s) (void)hipStreamSynchronize(st->last_stream);
+    st->last_stream = s;
+    if (!prepare_batch_pass(st, per_pass)) return Q_EXT_ERR_DEVICE;
+    if (!hip_ok(st, launch_fill_u32(b.sample_begin.ptr, 1, b.sample_count, s), "sample index")) return Q_EXT_ERR_DEVICE;
+    for (uint32_t done = 0; done < spp; done += per_pass) {
+        LaunchTimer lt(st, s, K_BATCH_PASS);
+        if (!hip_ok(st, hipGraphLaunch(b.graph_exec, s), "graph launch")) return Q_EXT_ERR_DEVICE;
+    }
+    b.sample_count += spp;
+    return Q_EXT_OK;
+}
+
+int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
+    const uint32_t n = desc->num_scenes;
+    if (n == 0 || !desc->scenes) return Q_EXT_ERR_ARGUMENT;
+    uint64_t total_pix = 0, total_tris = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Q_batch_scene& sc = desc->scenes[i];
+        const Q_scene_desc& m = sc.mesh;
+        if (sc.width == 0 || sc.height == 0 || m.struct_size < sizeof(Q_scene_desc) ||
+            (m.num_triangles && (!m.indices || !m.albedo || !m.emission)) || (m.num_vertices && !m.vertices)) {
+            log_msg(st, "set_batch: scene %u is malformed", i);
+            return Q_EXT_ERR_ARGUMENT;
+        }
+        total_pix += uint64_t(sc.width) * sc.height;
+        total_tris += m.num_triangles;
+    }
+    if (total_pix > 0x3FFFFFFFull || total_tris >= (1ull << 29)) {
+        log_msg(st, "set_batch: batch too large (%llu pixels, %llu triangles)", (unsigned long long)total_pix,
+                (unsigned long long)total_tris);
+        return Q_EXT_ERR_UNSUPPORTED;
+    }
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    free_batch(st);
+    auto t0 = std::chrono::steady_clock::now();
+
+    // BVH builds: one scene per host thread
+    std::vector<SceneHost> hosts(n);
+    std::vector<int32_t> rcs(n, Q_EXT_OK);
+    std::atomic<uint32_t> next{0};
+    const unsigned nth = std::max(1u, std::min({16u, std::thread::hardware_concurrency(), n}));
+    auto worker = [&] {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const Q_scene_desc& m = desc->scenes[i].mesh;
+            rcs[i] = prepare_scene(st, m.vertices, m.num_vertices, m.indices, m.num_triangles, m.albedo, m.emission,
+                                   1, hosts[i]);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nth; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    for (uint32_t i = 0; i < n; ++i)
+        if (rcs[i] != Q_EXT_OK) return rcs[i];
+
+    // concatenated arrays
+    size_t tri_f = 0, mat_f = 0, node_f = 0, node4_f = 0, light_n = 0;
+    uint32_t gentries = 0;
+    for (const SceneHost& h : hosts) {
+        tri_f += h.tri4.size();
+        mat_f += h.mat4.size();
+        node_f += h.nodes.size();
+        node4_f += h.nodes4.size();
+        light_n += h.lights.size();
+        gentries = std::max(gentries, stack_overflow_entries(h.depth, h.depth4));
+    }
+    BatchState& b = st->batch;
+    const size_t npix = size_t(total_pix);
+    if (!ensure_buffer(st, b.tris, std::max<size_t>(16, tri_f * 4), "batch triangles") ||
+        !ensure_buffer(st, b.mats, std::max<size_t>(16, mat_f * 4), "batch materials") ||
+        !ensure_buffer(st, b.nodes, std::max<size_t>(64, node_f * 4), "batch BVH nodes") ||
+        !ensure_buffer(st, b.nodes4, std::max<size_t>(128, node4_f * 4), "batch BVH4 nodes") ||
+        !ensure_buffer(st, b.lights, std::max<size_t>(16, light_n * sizeof(LightEntry)), "batch lights") ||
+        !ensure_buffer(st, b.table, n * sizeof(SceneDesc), "scene table") ||
+        !ensure_buffer(st, b.ids, 4 * npix, "batch ids") ||
+        !ensure_buffer(st, b.sample_begin, 4, "batch sample index") ||
+        !ensure_gstack(st, std::max(gentries, st->gstack_entries))) {
+        free_batch(st);
+        return Q_EXT_ERR_OUT_OF_MEMORY;
+    }
+    for (auto& a : b.accum)
+        if (!ensure_buffer(st, a, 16 * npix, "batch accumulation")) { free_batch(st); return Q_EXT_ERR_OUT_OF_MEMORY; }
+    b.descs.resize(n);
+    size_t to = 0, mo = 0, no = 0, n4o = 0, lo = 0;
+    uint32_t pix = 0;
+    char* tb = static_cast<char*>(b.tris.ptr);
+    char* mb = static_cast<char*>(b.mats.ptr);
+    char* nb = static_cast<char*>(b.nodes.ptr);
+    char* n4b = static_cast<char*>(b.nodes4.ptr);
+    LightEntry* lb = static_cast<LightEntry*>(b.lights.ptr);
+    for (uint32_t i = 0; i < n; ++i) {
+        const SceneHost& h = hosts[i];
+        const Q_batch_scene& sc = desc->scenes[i];
+        if ((!h.tri4.empty() && !hip_ok(st, hipMemcpy(tb + 4 * to, h.tri4.data(), 4 * h.tri4.size(), hipMemcpyHostToDevice), "batch upload")) ||
+            (!h.mat4.empty() && !hip_ok(st, hipMemcpy(mb + 4 * mo, h.mat4.data(), 4 * h.mat4.size(), hipMemcpyHostToDevice), "batch upload")) ||
+            (!h.nodes.empty() && !hip_ok(st, hipMemcpy(nb + 4 * no, h.nodes.data(), 4 * h.nodes.size(), hipMemcpyHostToDevice), "batch upload")) ||
+            (!h.nodes4.empty() && !hip_ok(st, hipMemcpy(n4b + 4 * n4o, h.nodes4.data(), 4 * h.nodes4.size(), hipMemcpyHostToDevice), "batch upload")) ||
+            (!h.lights.empty() && !hip_ok(st, hipMemcpy(lb + lo, h.lights.data(), sizeof(LightEntry) * h.lights.size(), hipMemcpyHostToDevice), "batch upload"))) {
+            free_batch(st);
+            return Q_EXT_ERR_DEVICE;
+        }
+        SceneDesc& d = b.descs[i];
+        d = SceneDesc{};
+        d.tris = tb + 4 * to;
+        d.mats = mb + 4 * mo;
+        d.nodes = nb + 4 * no;
+        d.nodes4 = n4b + 4 * n4o;
+        d.lights = lb + lo;
+        d.num_nodes = h.num_nodes;
+        d.num_lights = uint32_t(h.lights.size());
+        d.cam = make_camera(sc.camera, sc.width, sc.height);
+        d.width = sc.width;
+        d.height = sc.height;
+        d.pix_begin = pix;
+        d.path_begin = pix;
+        d.seed = sc.seed;
+        to += h.tri4.size();
+        mo += h.mat4.size();
+        no += h.nodes.size();
+        n4o += h.nodes4.size();
+        lo += h.lights.size();
+        pix += sc.width * sc.height;
+    }
+    b.num_scenes = n;
+    b.total_pixels = uint32_t(total_pix);
+    b.total_tris = uint32_t(total_tris);
+    b.gstack_entries = std::max(gentries, st->gstack_entries);
+    st->gstack_entries = b.gstack_entries;
+    b.active = true;
+    if (!clear_batch(st, st->stream) || !hip_ok(st, hipStreamSynchronize(st->stream), "batch clear")) {
+        free_batch(st);
+        return Q_EXT_ERR_DEVICE;
+    }
+    b.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    log_msg(st, "batch: %u scenes, %u triangles, %u pixels, prepared in %.0f ms", n, b.total_tris, b.total_pixels,
+            b.build_ms);
+    return Q_EXT_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+std::string tune_cache_path() {
+    if (const char* e = std::getenv("QUASI_HIP_TUNE_CACHE")) return e;
+    const char* home = std::getenv("HOME");
+    return std::string(home ? home : "/tmp") + "/.cache/quasi_hip/autotune.jsonl";
+}
+
+std::string tune_key(const plugin_state* st, uint32_t target) {
+    char buf[256];
+    if (target == Q_AUTOTUNE_BATCH)
+        std::snprintf(buf, sizeof buf, "%s|batch|%u|%u|%u|d%u|i%u", st->arch, st->batch.num_scenes,
+                      st->batch.total_tris, st->batch.total_pixels, st->params.max_depth, st->params.integrator);
+    else
+        std::snprintf(buf, sizeof buf, "%s|single|1|%u|%u|d%u|i%u", st->arch, st->num_tris, st->width * st->height,
+                      st->params.max_depth, st->params.integrator);
+    return buf;
+}
+
+bool tune_cache_lookup(const std::string& key, TraceConfig& cfg, double& ms) {
+    std::ifstream in(tune_cache_path());
+    std::string line;
+    bool found = false;
+    while (std::getline(in, line)) {
+        char k[256] = {0};
+        unsigned w = 0, bl = 0, ch = 0;
+        double t = 0;
+        if (std::sscanf(line.c_str(), "{\"key\": \"%255[^\"]\", \"bvh_width\": %u, \"block_size\": %u, \"queue_chunk\": %u, \"ms\": %lf}",
+                        k, &w, &bl, &ch, &t) == 5 &&
+            key == k) {
+            cfg = TraceConfig{w, bl, ch};
+            ms = t;
+            found = trace_config_supported(cfg);   // the last matching line wins
+        }
+    }
+    return found;
+}
+
+void tune_cache_store(plugin_state* st, const std::string& key, const TraceConfig& c, double ms) {
+    const std::string path = tune_cache_path();
+    std::error_code ec;
+    std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+    std::ofstream out(path, std::ios::app);
+    if (!out) {
+        log_msg(st, "autotune: cannot write %s", path.c_str());
+        return;
+    }
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "{\"key\": \"%s\", \"bvh_width\": %u, \"block_size\": %u, \"queue_chunk\": %u, \"ms\": %.4f}\n",
+                  key.c_str(), c.width, c.block, c.chunk, ms);
+    out << buf;
+}
+
+void apply_config(plugin_state* st, const TraceConfig& c) {
+    st->trace_cfg = c;
+    st->trace_blocks = persistent_blocks(st->device, c);
+    st->params.bvh_width = c.width;
+    st->params.block_size = c.block;
+    st->params.queue_chunk = c.chunk;
+}
+
+// One timed probe with the current trace configuration.
+bool time_probe(plugin_state* st, uint32_t target, uint32_t spp, float& ms) {
+    hipStream_t s = st->stream;
+    hipEvent_t a = take_event(st), b = take_event(st);
+    if (!a || !b) return false;
+    bool ok = true;
+    if (target == Q_AUTOTUNE_BATCH) {
+        ok = clear_batch(st, s) && hip_ok(st, hipEventRecord(a, s), "event") && render_batch(st, spp, s) == Q_EXT_OK;
+    } else {
+        Q_render_frame f{};
+        f.width = st->width;
+        f.height = st->height;
+        f.camera = st->last_camera;
+        f.camera_dirty = 1;
+        const uint32_t saved = st->params.spp_per_render;
+        st->params.spp_per_render = spp;
+        ok = hip_ok(st, hipEventRecord(a, s), "event") && render_frame(st, &f);
+        st->params.spp_per_render = saved;
+    }
+    ok = ok && hip_ok(st, hipEventRecord(b, s), "event") && hip_ok(st, hipEventSynchronize(b), "probe");
+    if (ok) ok = hip_ok(st, hipEventElapsedTime(&ms, a, b), "probe time");
+    st->event_pool.push_back(a);
+    st->event_pool.push_back(b);
+    return ok;
+}
+
+int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t flags, Q_autotune_result* out) {
+    if (target == Q_AUTOTUNE_BATCH && !st->batch.active) {
+        log_msg(st, "autotune: no batch set");
+        return Q_EXT_ERR_STATE;
+    }
+    if (target != Q_AUTOTUNE_BATCH && st->params.engine != Q_ENGINE_WAVEFRONT) {
+        log_msg(st, "autotune: the single-scene target needs the wavefront engine");
+        return Q_EXT_ERR_STATE;
+    }
+    if (probe_spp == 0) probe_spp = 1;
+    if (target == Q_AUTOTUNE_BATCH) {
+        const uint32_t pp = batch_samples_per_pass(st);
+        probe_spp = std::max(pp, probe_spp - probe_spp % pp);
+    }
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    Q_autotune_result res{};
+    res.struct_size = sizeof(Q_autotune_result);
+    const std::string key = tune_key(st, target);
+    TraceConfig cached{};
+    double cached_ms = 0;
+    if ((flags & Q_AUTOTUNE_USE_CACHE) && tune_cache_lookup(key, cached, cached_ms)) {
+        apply_config(st, cached);
+        res.num_entries = 1;
+        res.entries[0] = Q_autotune_entry{cached.width, cached.block, cached.chunk, 0, cached_ms};
+        res.best = 0;
+        res.from_cache = 1;
+    } else {
+        const TraceConfig saved = st->trace_cfg;
+        const uint32_t n = std::min<uint32_t>(trace_config_count(), Q_AUTOTUNE_MAX_ENTRIES);
+        double best = 1e300;
+        for (uint32_t i = 0; i < n; ++i) {
+            const TraceConfig c = trace_config_at(i);
+            apply_config(st, c);
+            float ms = 0.f, warm = 0.f;
+            // first run warms code and caches, the second is kept
+            if (!time_probe(st, target, probe_spp, warm) || !time_probe(st, target, probe_spp, ms)) {
+                apply_config(st, saved);
+                return Q_EXT_ERR_DEVICE;
+            }
+            res.entries[i] = Q_autotune_entry{c.width, c.block, c.chunk, 0, double(ms)};
+            if (ms < best) { best = ms; res.best = i; }
+        }
+        res.num_entries = n;
+        const Q_autotune_entry& e = res.entries[res.best];
+        apply_config(st, TraceConfig{e.bvh_width, e.block_size, e.queue_chunk});
+        if (!(flags & Q_AUTOTUNE_NO_STORE))
+            tune_cache_store(st, key, TraceConfig{e.bvh_width, e.block_size, e.queue_chunk}, e.ms);
+        log_msg(st, "autotune (%s): best width %u, block %u, chunk %u: %.3f ms",
+                target == Q_AUTOTUNE_BATCH ? "batch" : "single", e.bvh_width, e.block_size, e.queue_chunk, e.ms);
+    }
+    // leave the pass graph of the chosen configuration captured, the
+    // accumulations empty and the counters as they were before tuning
+    if (target == Q_AUTOTUNE_BATCH && !prepare_batch_pass(st, batch_samples_per_pass(st))) return Q_EXT_ERR_DEVICE;
+    if (st->batch.active && !clear_batch(st, st->stream)) return Q_EXT_ERR_DEVICE;
+    if (!clear_single(st, st->stream)) return Q_EXT_ERR_DEVICE;
+    if (!hip_ok(st, hipStreamSynchronize(st->stream), "autotune")) return Q_EXT_ERR_DEVICE;
+    if (out) *out = res;
+    return Q_EXT_OK;
+}
+
+}  // namespace
+}  // namespace host
+}  // namespace quasi_hip
+
+using namespace quasi_hip;
+using namespace quasi_hip::host;
+
+extern "C" {
+
+Q_EXPORT int32_t Q_plugin_set_batch(Q_plugin_handle* handle, const Q_batch_desc* batch) {
+    if (!handle || !batch || batch->struct_size < sizeof(Q_batch_desc)) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    try {
+        return set_batch(st, batch);
+    } catch (const std::bad_alloc&) {
+        log_msg(st, "set_batch: host allocation failed");
+        free_batch(st);
+        return Q_EXT_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        log_msg(st, "set_batch: exception");
+        free_batch(st);
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+Q_EXPORT int32_t Q_plugin_render_batch(Q_plugin_handle* handle, uint32_t spp, void* stream) {
+    if (!handle) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    try {
+        (void)hipSetDevice(st->device);
+        return render_batch(st, spp, stream ? static_cast<hipStream_t>(stream) : st->stream);
+    } catch (...) {
+        log_msg(st, "render_batch: exception");
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+Q_EXPORT int32_t Q_plugin_readback_batch(Q_plugin_handle* handle, uint32_t scene, Q_readback_aov_result* out) {
+    if (!handle || !out) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    *out = Q_readback_aov_result{};
+    try {
+        BatchState& b = st->batch;
+        if (!b.active || scene >= b.num_scenes) return Q_EXT_ERR_ARGUMENT;
+        if (b.sample_count == 0) return Q_EXT_ERR_STATE;
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        const SceneDesc& d = b.descs[scene];
+        for (uint32_t a = 0; a < Q_AOV_COUNT; ++a) {
+            const char* base = static_cast<const char*>(b.accum[a].ptr) + 16 * size_t(d.pix_begin);
+            out->buffers[a] = read_mean_of(st, base, d.width, d.height, b.sample_count);
+            if (!out->buffers[a].data) return Q_EXT_ERR_DEVICE;
+        }
+        return Q_EXT_OK;
+    } catch (...) {
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+Q_EXPORT int32_t Q_plugin_readback_batch_ids(Q_plugin_handle* handle, uint32_t scene, uint32_t* out, uint64_t count) {
+    if (!handle || !out) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    BatchState& b = st->batch;
+    if (!b.active || scene >= b.num_scenes) return Q_EXT_ERR_ARGUMENT;
+    const SceneDesc& d = b.descs[scene];
+    const uint64_t npix = uint64_t(d.width) * d.height;
+    if (count < npix) return Q_EXT_ERR_ARGUMENT;
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    const char* base = static_cast<const char*>(b.ids.ptr) + 4 * size_t(d.pix_begin);
+    return hip_ok(st, hipMemcpy(out, base, 4 * npix, hipMemcpyDeviceToHost), "batch id readback") ? Q_EXT_OK
+                                                                                                 : Q_EXT_ERR_DEVICE;
+}
+
+Q_EXPORT int32_t Q_plugin_autotune(Q_plugin_handle* handle, uint32_t target, uint32_t probe_spp, uint32_t flags,
+                                   Q_autotune_result* out) {
+    if (!handle || target > Q_AUTOTUNE_BATCH) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    try {
+        return autotune(st, target, probe_spp, flags, out);
+    } catch (...) {
+        log_msg(st, "autotune: exception");
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+}  // extern "C"

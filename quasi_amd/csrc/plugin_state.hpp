@@ -1,0 +1,166 @@
+// plugin_state.hpp -- the plugin's internal state and host helpers, shared by
+// plugin.cpp (the reference ABI and single-scene extensions) and
+// plugin_batch.cpp (batched scenes, pass graphs, autotuner).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/quasi/plugin_interface.h"
+#include "../../include/quasi/quasi_hip_ext.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace quasi_hip {
+namespace host {
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum KernelSlot : uint32_t {
+    K_MEGAKERNEL = 0, K_RESOLVE, K_FILL,
+    K_RAYGEN, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUMULATE,  // wavefront, in WaveKernel order
+    K_BATCH_PASS,                                          // one replay of a captured batch pass
+    K_COUNT
+};
+extern const char* const kKernelNames[K_COUNT];
+extern const bool kRenderKernel[K_COUNT];
+
+// Wavefront path-state buffers (see WaveState in common.hpp).
+struct WaveBuffers {
+    DeviceBuffer ray_o[2], ray_d[2], hit, thr, radiance, rng, aov0, aov1, sh_o, sh_d, sh_c;
+    DeviceBuffer counts, shadow_counts, fetch;
+    uint32_t capacity = 0;
+};
+
+struct TimedLaunch {
+    uint32_t slot;
+    hipEvent_t start, stop;
+};
+
+// One mesh prepared for upload in the layouts of common.hpp.
+struct SceneHost {
+    std::vector<float> tri4, mat4, nodes, nodes4;
+    std::vector<LightEntry> lights;
+    uint32_t num_tris = 0, num_nodes = 0, num_nodes4 = 0, depth = 0, depth4 = 0;
+};
+
+// Several scenes traced together (Q_plugin_set_batch).
+struct BatchState {
+    bool active = false;
+    std::vector<SceneDesc> descs;          // host copy with device pointers
+    DeviceBuffer tris, mats, nodes, nodes4, lights, table;
+    DeviceBuffer accum[Q_AOV_COUNT], ids, resolved, sample_begin;
+    uint32_t num_scenes = 0, total_pixels = 0, total_tris = 0;
+    uint32_t sample_count = 0;
+    uint32_t gstack_entries = 0;
+    double build_ms = 0;
+    // captured pass
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
+};
+
+struct plugin_state {
+    Q_plugin_context* context = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipStream_t last_stream = nullptr;
+    hipStream_t capture_stream = nullptr;
+    char arch[64] = {0};
+
+    Q_render_params params{};
+
+    // host copy of the scene
+    std::vector<float> verts;
+    std::vector<uint32_t> indices;
+    std::vector<float> albedo, emission;
+
+    // device scene
+    DeviceBuffer tris, mats, nodes, nodes4, lights, gstack;
+    uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
+    uint32_t num_nodes4 = 0, bvh_depth4 = 0, gstack_entries = 0;
+    double bvh_build_ms = 0;
+
+    // framebuffer
+    uint32_t width = 0, height = 0;
+    uint32_t sample_count = 0;
+    DeviceBuffer accum[Q_AOV_COUNT];
+    DeviceBuffer ids;
+    DeviceBuffer resolved;
+    DeviceBuffer counters;
+
+    // wavefront engine
+    WaveBuffers wave;
+    TraceConfig trace_cfg{2, 256, 64};
+    uint32_t trace_blocks = 0;
+    hipEvent_t pending_start = nullptr;
+    int num_cus = 256;
+
+    BatchState batch;
+    Q_camera last_camera{};
+    bool have_camera = false;
+
+    // stats
+    std::vector<hipEvent_t> event_pool;
+    std::vector<TimedLaunch> timed;
+    uint64_t launches[K_COUNT] = {};
+    double kernel_ms[K_COUNT] = {};
+};
+
+void log_msg(plugin_state* st, const char* fmt, ...);
+bool hip_ok(plugin_state* st, hipError_t e, const char* what);
+void free_buffer(DeviceBuffer& b);
+bool ensure_buffer(plugin_state* st, DeviceBuffer& b, size_t bytes, const char* what);
+hipEvent_t take_event(plugin_state* st);
+void harvest_events(plugin_state* st);
+bool sync_render(plugin_state* st);
+
+// Brackets one launch with events when kernel timing is on.
+struct LaunchTimer {
+    plugin_state* st;
+    hipStream_t s;
+    uint32_t slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(plugin_state* st_, hipStream_t s_, uint32_t slot_);
+    ~LaunchTimer();
+};
+
+// Prepares one mesh (BVH build, triangle / material / node / light arrays);
+// threads = 0 uses up to 16 host threads for the BVH build.
+int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* albedo, const float* emission, unsigned threads, SceneHost& out);
+
+// Overflow entries per thread that the persistent traversal stacks need for
+// a BVH2 of `depth` and a BVH4 of `depth4` levels.
+uint32_t stack_overflow_entries(uint32_t depth, uint32_t depth4);
+
+// The persistent kernels' overflow stacks for `entries` per thread.
+bool ensure_gstack(plugin_state* st, uint32_t entries);
+
+bool ensure_wave(plugin_state* st, uint32_t paths);
+WaveState wave_state(const WaveBuffers& w, uint32_t paths);
+void wave_timer_before(void* user, uint32_t kernel, hipStream_t s);
+void wave_timer_after(void* user, uint32_t kernel, hipStream_t s);
+
+// Host camera packing (plugin.mm:649-674) and the basis of get_camera_ray.
+CameraBasis make_camera(const Q_camera& c, uint32_t w, uint32_t h);
+
+// Mean of npix RGBA sums (divided by n samples) into a malloc'd array.
+Q_aov_buffer read_mean_of(plugin_state* st, const void* sums, uint32_t width, uint32_t height, uint32_t n);
+
+void free_batch(plugin_state* st);
+
+// The body of Q_plugin_render (false when a step failed; the failure is logged).
+bool render_frame(plugin_state* st, const Q_render_frame* frame);
+
+// Zeroes the single-scene accumulation and its sample count.
+bool clear_single(plugin_state* st, hipStream_t s);
+
+}  // namespace host
+}  // namespace quasi_hip

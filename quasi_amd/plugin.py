@@ -205,6 +205,19 @@ class Plugin:
             self.handle = None
 
 
+def scene_desc(mesh):
+    """Q_scene_desc over a mesh's arrays, plus the arrays it points into (keep
+    them alive for the call)."""
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float32)
+    i = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
+    a = np.ascontiguousarray(mesh.albedo, dtype=np.float32)
+    e = np.ascontiguousarray(mesh.emission, dtype=np.float32)
+    d = abi.Q_scene_desc(C.sizeof(abi.Q_scene_desc), v.size // 3, i.size // 3, 0,
+                         v.ctypes.data_as(C.POINTER(C.c_float)), i.ctypes.data_as(C.POINTER(C.c_uint32)),
+                         a.ctypes.data_as(C.POINTER(C.c_float)), e.ctypes.data_as(C.POINTER(C.c_float)))
+    return d, (v, i, a, e)
+
+
 def _array(buf, w, h, ch=4) -> np.ndarray:
     return np.ctypeslib.as_array(buf, shape=(h, w, ch)).copy()
 
@@ -255,6 +268,13 @@ class HipRenderer:
         self._get_stats = g("Q_plugin_get_stats", C.c_int32, [_H, P(abi.Q_render_stats)])
         self._reset_stats = g("Q_plugin_reset_stats", C.c_int32, [_H])
         self._sync = g("Q_plugin_synchronize", C.c_int32, [_H])
+        self._set_batch = g("Q_plugin_set_batch", C.c_int32, [_H, P(abi.Q_batch_desc)])
+        self._render_batch = g("Q_plugin_render_batch", C.c_int32, [_H, C.c_uint32, C.c_void_p])
+        self._readback_batch = g("Q_plugin_readback_batch", C.c_int32, [_H, C.c_uint32, P(abi.Q_readback_aov_result)])
+        self._readback_batch_ids = g("Q_plugin_readback_batch_ids", C.c_int32,
+                                     [_H, C.c_uint32, P(C.c_uint32), C.c_uint64])
+        self._autotune = g("Q_plugin_autotune", C.c_int32, [_H, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                            P(abi.Q_autotune_result)])
 
     def _check(self, rc: int, what: str) -> None:
         if rc != abi.Q_EXT_OK:
@@ -263,14 +283,53 @@ class HipRenderer:
 
     # -- extension calls -------------------------------------------------
     def set_scene(self, mesh) -> None:
-        v = np.ascontiguousarray(mesh.vertices, dtype=np.float32)
-        i = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
-        a = np.ascontiguousarray(mesh.albedo, dtype=np.float32)
-        e = np.ascontiguousarray(mesh.emission, dtype=np.float32)
-        d = abi.Q_scene_desc(C.sizeof(abi.Q_scene_desc), v.size // 3, i.size // 3, 0,
-                             v.ctypes.data_as(C.POINTER(C.c_float)), i.ctypes.data_as(C.POINTER(C.c_uint32)),
-                             a.ctypes.data_as(C.POINTER(C.c_float)), e.ctypes.data_as(C.POINTER(C.c_float)))
+        d, keep = scene_desc(mesh)
         self._check(self._set_scene(self.plugin.handle, C.byref(d)), "set_scene")
+        del keep
+
+    def set_batch(self, scenes) -> None:
+        """scenes: iterable of (mesh, width, height, seed)."""
+        scenes = list(scenes)
+        arr = (abi.Q_batch_scene * len(scenes))()
+        keep = []
+        for k, (mesh, w, h, seed) in enumerate(scenes):
+            d, kp = scene_desc(mesh)
+            keep.append(kp)
+            c = mesh.camera
+            arr[k] = abi.Q_batch_scene(w, h, seed, 0, make_camera(c["position"], c["target"], c["up"], c["fov"]), d)
+        desc = abi.Q_batch_desc(C.sizeof(abi.Q_batch_desc), len(scenes), arr)
+        self._check(self._set_batch(self.plugin.handle, C.byref(desc)), "set_batch")
+        self.batch_sizes = [(w, h) for (_, w, h, _) in scenes]
+
+    def render_batch(self, spp: int, stream: Optional[int] = None) -> None:
+        self._check(self._render_batch(self.plugin.handle, spp, C.c_void_p(stream) if stream else None),
+                    "render_batch")
+
+    def readback_batch(self, scene: int) -> dict:
+        r = abi.Q_readback_aov_result()
+        rc = self._readback_batch(self.plugin.handle, scene, C.byref(r))
+        out = {}
+        try:
+            self._check(rc, "readback_batch")
+            for k, name in enumerate(("beauty", "albedo", "normal", "depth")):
+                b = r.buffers[k]
+                out[name] = _array(b.data, b.width, b.height, b.channels) if b.data else None
+        finally:
+            self.plugin.readback_aov_free(r)
+        return out
+
+    def batch_ids(self, scene: int) -> np.ndarray:
+        w, h = self.batch_sizes[scene]
+        out = np.empty(w * h, dtype=np.uint32)
+        self._check(self._readback_batch_ids(self.plugin.handle, scene, out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                             out.size), "readback_batch_ids")
+        return out.reshape(h, w)
+
+    def autotune(self, target: int = abi.Q_AUTOTUNE_BATCH, probe_spp: int = 1, flags: int = 0) -> abi.Q_autotune_result:
+        r = abi.Q_autotune_result()
+        r.struct_size = C.sizeof(abi.Q_autotune_result)
+        self._check(self._autotune(self.plugin.handle, target, probe_spp, flags, C.byref(r)), "autotune")
+        return r
 
     def get_params(self) -> abi.Q_render_params:
         p = abi.Q_render_params()

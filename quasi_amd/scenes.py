@@ -181,5 +181,43 @@ def soup(num_triangles: int = 1_000_000, seed: int = 1) -> Mesh:
     return b.mesh(SOUP_CAMERA, f"soup{n}")
 
 
+BATCH_RESOLUTIONS = (128, 256, 512, 1024)
+
+
+def room_with_soup(num_triangles: int, seed: int) -> Mesh:
+    """The 36-triangle Cornell-style room of ``cornell36`` (albedos from this
+    seed) filled with num_triangles - 36 soup triangles (centres uniform in
+    the room's interior [-0.9, 0.9] x [0.1, 1.9] x [-0.9, 0.9], sizes
+    log-uniform in [1e-3, 5e-2], albedo U[0, 1])."""
+    room = cornell36(seed)
+    extra = max(0, int(num_triangles) - room.num_triangles)
+    rng = np.random.Generator(np.random.PCG64(seed + 0x5EED))
+    c = rng.uniform((-0.9, 0.1, -0.9), (0.9, 1.9, 0.9), size=(extra, 1, 3))
+    s = np.exp(rng.uniform(np.log(1e-3), np.log(5e-2), size=(extra, 1, 1)))
+    d = rng.normal(size=(extra, 3, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    tri = (c + s * d).astype(F32)
+    b = _Builder()
+    b.v, b.i, b.a, b.e = [room.vertices], [room.indices], [room.albedo], [room.emission]
+    b.triangles(tri, rng.uniform(0.0, 1.0, size=(extra, 3)).astype(F32), np.zeros((extra, 3), dtype=F32))
+    return b.mesh(REFERENCE_CAMERA, f"room{num_triangles}")
+
+
+def batch_scenes(num_scenes: int = 64, base_seed: int = 1000, min_tris: float = 1e3, max_tris: float = 5e5,
+                 resolutions=BATCH_RESOLUTIONS):
+    """Scenes of uneven size for the batched launch: scene i uses seed
+    base_seed + i for a triangle count round(exp(U[ln min, ln max])), a
+    resolution drawn from ``resolutions`` (square images) and its mesh
+    (``room_with_soup``).  Returns [(mesh, width, height, seed)]."""
+    out = []
+    for i in range(num_scenes):
+        seed = base_seed + i
+        rng = np.random.Generator(np.random.PCG64(seed))
+        t = int(round(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris)))))
+        res = int(rng.choice(resolutions))
+        out.append((room_with_soup(t, seed), res, res, seed))
+    return out
+
+
 def camera_args(mesh: Mesh) -> dict:
     return dict(mesh.camera)

@@ -1,0 +1,151 @@
+"""Batched scenes (Q_plugin_set_batch / render_batch): one pass covers every
+scene with shared queues and is replayed from a captured hipGraph.
+
+Bars: every scene of a batch equals, bit for bit, a single-scene wavefront
+render of it with the same seed and samples per pass (same kernels, same
+per-path arithmetic, same accumulation order); graph replays interleaved with
+eager work each match the eager result; a batch scene matches the FP64 oracle
+to the image bar (relative L1 <= 1e-4).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quasi_amd import abi, scenes
+from quasi_amd.plugin import HipRenderer, make_camera
+from tests.parity import check_ids, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_batch():
+    out = scenes.batch_scenes(num_scenes=6, base_seed=77, min_tris=40, max_tris=20000,
+                              resolutions=(24, 40, 64))
+    # one non-square scene and one without lights (all soup) for coverage
+    m, w, h, s = out[1]
+    out[1] = (m, 48, 32, s)
+    soup = scenes.soup(3000, seed=9)
+    soup.emission[:] = 0
+    out.append((soup, 32, 24, 4242))
+    return out
+
+
+PARAMS = dict(max_depth=6, rr_start=3, rng_mode=1, integrator=1, engine=abi.Q_ENGINE_WAVEFRONT, samples_per_pass=1,
+              bvh_width=4, block_size=128, queue_chunk=64, flags=abi.Q_FLAG_RECORD_IDS, id_sample=1)
+
+
+def plain_render(r, mesh, w, h, seed, spp):
+    r.set_scene(mesh)
+    r.resize(w, h)
+    c = mesh.camera
+    r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+    r.set_params(spp_per_render=spp, seed=seed, **PARAMS)
+    r.render(1)
+    return r.readback_aov(), r.primary_ids()
+
+
+def test_batch_equals_plain_per_scene():
+    batch = small_batch()
+    spp = 4
+    say(f"[batch==plain] {len(batch)} scenes")
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        r.set_batch(batch)
+        r.render_batch(spp)
+        got = [(r.readback_batch(i), r.batch_ids(i)) for i in range(len(batch))]
+        for i, (mesh, w, h, seed) in enumerate(batch):
+            aov, ids = plain_render(r, mesh, w, h, seed, spp)
+            for k in ("beauty", "albedo", "normal", "depth"):
+                assert np.array_equal(got[i][0][k], aov[k]), (i, k)
+            assert np.array_equal(got[i][1], ids), i
+
+
+def test_graph_replays_match_eager_with_work_between():
+    batch = small_batch()[:4]
+    say("[graph-replay]")
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        r.set_batch(batch)
+        for k in range(1, 4):
+            r.render_batch(1)           # one graph replay
+            r.synchronize()
+            # eager work on the same plugin between replays
+            r.render(1)
+            r.synchronize()
+            imgs = [r.readback_batch(i)["beauty"] for i in range(len(batch))]
+            with HipRenderer(16, 16) as e:
+                for i, (mesh, w, h, seed) in enumerate(batch):
+                    aov, _ = plain_render(e, mesh, w, h, seed, k)
+                    assert np.array_equal(imgs[i], aov["beauty"]), (k, i)
+
+
+def test_batch_scene_matches_oracle():
+    batch = small_batch()
+    spp = 8
+    i = 3
+    mesh, w, h, seed = batch[i]
+    say(f"[batch-oracle] scene {i} {w}x{h} ntri={mesh.num_triangles}")
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        r.set_batch(batch)
+        r.render_batch(spp)
+        got = r.readback_batch(i)
+        ids = r.batch_ids(i)
+    oscene = O.Scene(mesh)
+    ref = oscene.render(w, h, spp, max_depth=6, rr_start=3, rng_mode=1, integrator=1, seed=seed, id_sample=1)
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert rel_l1(got[k], ref.mean(k)) <= 1e-4, k
+    frac, nbad, not_ties = check_ids(ids, ref.ids, mesh, oscene,
+                                     dict(width=w, height=h, rng_mode=1, seed=seed, camera=mesh.camera), sample=1)
+    assert frac >= 0.9999 and not not_ties
+
+
+def test_batch_stats_and_errors():
+    batch = small_batch()[:3]
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        with pytest.raises(RuntimeError):
+            r.render_batch(1)                       # no batch yet
+        r.set_batch(batch)
+        r.reset_stats()
+        r.render_batch(2)
+        st = r.stats()
+        total_pix = sum(w * h for _, w, h, _ in batch)
+        assert st.rays_camera == 2 * total_pix
+        kern = {st.kernels[k].name.decode(): st.kernels[k] for k in range(st.num_kernels)}
+        assert kern["batch_pass"].launches == 2
+        with pytest.raises(RuntimeError):
+            r.readback_batch(len(batch))            # out of range
+        r.set_params(spp_per_render=1, seed=0, **dict(PARAMS, samples_per_pass=2))
+        with pytest.raises(RuntimeError):
+            r.render_batch(3)                       # not a multiple of 2 samples per pass
+        r.render_batch(4)
+
+
+def test_autotune_batch(tmp_path, monkeypatch):
+    monkeypatch.setenv("QUASI_HIP_TUNE_CACHE", str(tmp_path / "tune.jsonl"))
+    batch = small_batch()[:4]
+    say("[autotune]")
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        r.set_batch(batch)
+        res = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=1)
+        assert res.num_entries == 18 and res.best < res.num_entries and not res.from_cache
+        best = res.entries[res.best]
+        assert all(best.ms <= res.entries[k].ms for k in range(res.num_entries))
+        p = r.get_params()
+        assert (p.bvh_width, p.block_size, p.queue_chunk) == (best.bvh_width, best.block_size, best.queue_chunk)
+        assert (tmp_path / "tune.jsonl").exists()
+        again = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=1, flags=abi.Q_AUTOTUNE_USE_CACHE)
+        assert again.from_cache == 1 and again.entries[0].bvh_width == best.bvh_width
+        # the tuned configuration still renders the same image as any other
+        r.render_batch(2)
+        img = r.readback_batch(0)["beauty"]
+    with HipRenderer(16, 16) as e:
+        mesh, w, h, seed = batch[0]
+        aov, _ = plain_render(e, mesh, w, h, seed, 2)
+        assert np.array_equal(img, aov["beauty"])
