@@ -91,6 +91,8 @@ def bench_batch(args):
     plain_s = 0.0
     plain_rays = 0
     plain_imgs = []
+    prog_s = 0.0
+    prog_rays = 0
     with HipRenderer(64, 64, device=0) as r:
         for i, (mesh, w, h, seed) in enumerate(batch):
             r.set_scene(mesh)
@@ -110,6 +112,16 @@ def bench_batch(args):
             plain_s += time.perf_counter() - t0
             plain_rays += rays_of(r.stats())
             plain_imgs.append(r.readback_aov()["beauty"])
+            if args.batch_progressive:
+                # the reference host's usage: one sample per Q_plugin_render call
+                r.set_params(spp_per_render=1, seed=seed, samples_per_pass=0, flags=0, **common)
+                r.reset_stats()
+                r.synchronize()
+                t0 = time.perf_counter()
+                r.render(spp)
+                r.synchronize()
+                prog_s += time.perf_counter() - t0
+                prog_rays += rays_of(r.stats())
         log(f"batch: plain sequential {plain_s:.3f} s, {plain_rays / plain_s / 1e6:.1f} Mrays/s")
 
     with HipRenderer(64, 64, device=0) as r:
@@ -147,7 +159,68 @@ def bench_batch(args):
             "samples_per_pass": args.batch_samples_per_pass,
             "autotuned": {"bvh_width": best.bvh_width, "block_size": best.block_size,
                           "queue_chunk": best.queue_chunk},
-            "max_rel_l1_vs_plain": err}
+            "max_rel_l1_vs_plain": err,
+            "plain_progressive_mrays": round(prog_rays / prog_s / 1e6, 3) if prog_s else None,
+            "speedup_vs_progressive": round(batched / (prog_rays / prog_s / 1e6), 4) if prog_s else None}
+
+
+def bench_adjoint(args):
+    """Differentiable-rendering workload: icosphere of 9,680 triangles on a
+    ground quad with two quad lights, 512 x 512, 32 spp, depth 4, NEE.  The
+    forward pass renders the initial scene (32 spp from a cleared
+    accumulation); the adjoint pass replays the same 32 samples and returns
+    dL/d(albedo) and dL/d(vertex positions) of the L2 loss against a target
+    render.  Medians of --adjoint-runs runs, wall clock (host calls included)
+    and device time (hipEvents around the kernels)."""
+    import numpy as np
+    from quasi_amd import abi, scenes
+    from quasi_amd.plugin import HipRenderer
+
+    w = h = 512
+    spp = 32
+    init = scenes.diff_sphere(22, "init")
+    target = scenes.diff_sphere(22, "target")
+    common = dict(spp_per_render=spp, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=1,
+                  engine=abi.Q_ENGINE_WAVEFRONT, bvh_width=args.bvh_width, block_size=args.block,
+                  queue_chunk=args.chunk, samples_per_pass=0)
+    with HipRenderer(w, h, device=0) as r:
+        for mesh in (target, init):
+            r.set_scene(mesh)
+            c = mesh.camera
+            r.set_camera(c["position"], c["target"], c["up"], c["fov"])
+            r.set_params(flags=abi.Q_FLAG_TIME_KERNELS, **common)
+            r.render(1)
+            if mesh is target:
+                T = r.readback_aov()["beauty"]
+        r.adjoint(T, spp, mode=abi.Q_ADJOINT_L2_TARGET)        # warm-up
+        fwd_wall, fwd_dev, adj_wall, adj_dev = [], [], [], []
+        for _ in range(args.adjoint_runs):
+            r.reset_stats()
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.plugin.render(r.frame(camera_dirty=True))
+            r.synchronize()
+            fwd_wall.append(time.perf_counter() - t0)
+            fwd_dev.append(r.stats().render_ms * 1e-3)
+            t0 = time.perf_counter()
+            ga, gv, res = r.adjoint(T, spp, mode=abi.Q_ADJOINT_L2_TARGET)
+            adj_wall.append(time.perf_counter() - t0)
+            adj_dev.append(res.kernel_ms * 1e-3)
+        st = r.stats()
+    med = lambda v: float(np.median(v))
+    out = {"workload": f"icosphere {init.num_triangles} triangles ({init.vertices.shape[0]} vertices), "
+                       f"{w}x{h}, {spp} spp, depth 4, NEE; gradients of an L2 image loss w.r.t. albedo and "
+                       "vertex positions",
+           "forward_ms": round(med(fwd_wall) * 1e3, 3), "adjoint_ms": round(med(adj_wall) * 1e3, 3),
+           "forward_device_ms": round(med(fwd_dev) * 1e3, 3), "adjoint_device_ms": round(med(adj_dev) * 1e3, 3),
+           "ratio": round(med(adj_wall) / med(fwd_wall), 4),
+           "ratio_device": round(med(adj_dev) / med(fwd_dev), 4),
+           "loss": res.loss, "paths": int(res.paths), "forward_rays": int(st.rays_camera + st.rays_extension
+                                                                         + st.rays_shadow),
+           "grad_albedo_l1": float(np.abs(ga).sum()), "grad_vertices_l1": float(np.abs(gv).sum())}
+    log(f"adjoint: forward {out['forward_ms']:.2f} ms, adjoint {out['adjoint_ms']:.2f} ms "
+        f"(device {out['forward_device_ms']:.2f} / {out['adjoint_device_ms']:.2f} ms), ratio {out['ratio']:.3f}")
+    return out
 
 
 def main():
@@ -172,6 +245,10 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=32)
     ap.add_argument("--batch-samples-per-pass", type=int, default=1)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--adjoint-runs", type=int, default=5)
+    ap.add_argument("--batch-progressive", type=int, default=1,
+                    help="also time the plain engine at one sample per render call (the reference host's usage)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -277,6 +354,7 @@ def main():
         "kernel_ms": {n: round(v.total_ms, 3) for n, v in kern.items() if v.launches},
     }
     out["batch"] = None if args.no_batch else bench_batch(args)
+    out["adjoint"] = None if args.no_adjoint else bench_adjoint(args)
     if not args.no_cpu:
         log("CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(mesh, args, args.cpu_seconds)

@@ -209,6 +209,55 @@ typedef struct Q_autotune_result {
 int32_t Q_plugin_autotune(Q_plugin_handle* handle, uint32_t target, uint32_t probe_spp, uint32_t flags,
                           Q_autotune_result* out);
 
+/* ---------------------------------------------------------------- adjoint
+ * Gradients of a scalar image loss L with respect to the current scene's
+ * per-triangle albedo and vertex positions (no reference counterpart; the
+ * reference renders a fixed scene and has no derivatives).
+ *
+ * The image is I = (1/spp) sum_s C_s over samples sample_begin ..
+ * sample_begin + spp - 1 of the current camera (the last rendered frame's),
+ * resolution, seed (or desc->seed with Q_ADJOINT_SEED), depth, Russian
+ * roulette start and integrator.  Each path is re-traced from its RNG seed
+ * and differentiated with its discrete decisions held fixed: the triangles
+ * hit, facing sides, light choices, shadow-ray visibility, Russian-roulette
+ * outcomes and probabilities.  Emitter geometry (light sampling) is held
+ * fixed; the gradients of vertices are those of the non-emitting triangles
+ * that use them.
+ *
+ * mode Q_ADJOINT_IMAGE_GRADIENT: image = dL/dI, RGBA32F top-to-bottom, the
+ *   alpha channel ignored (the pixels with dL/dI = 0 are skipped).
+ * mode Q_ADJOINT_L2_TARGET: image = target T (RGBA32F); L = sum over pixels
+ *   and RGB of (M - T)^2 with M the current accumulated mean image, so
+ *   dL/dI = 2 (M - T); L is returned in result->loss.
+ * Gradients are written in FP64, in input order: grad_albedo 3 per
+ * triangle, grad_vertices 3 per vertex (either may be NULL).
+ */
+#define Q_ADJOINT_IMAGE_GRADIENT 0u
+#define Q_ADJOINT_L2_TARGET      1u
+#define Q_ADJOINT_SEED           1u   /* flags: use desc->seed instead of the render seed */
+
+typedef struct Q_adjoint_desc {
+    uint32_t struct_size;
+    uint32_t mode;
+    uint32_t spp;              /* samples replayed per pixel (>= 1) */
+    uint32_t sample_begin;     /* first sample index */
+    uint32_t seed;             /* with Q_ADJOINT_SEED */
+    uint32_t flags;
+    const float* image;        /* width * height * 4 floats (host memory) */
+    double* grad_albedo;       /* 3 * num_triangles, or NULL */
+    double* grad_vertices;     /* 3 * num_vertices, or NULL */
+} Q_adjoint_desc;
+
+typedef struct Q_adjoint_result {
+    uint32_t struct_size;
+    uint32_t reserved;
+    uint64_t paths;            /* paths replayed */
+    double loss;               /* Q_ADJOINT_L2_TARGET: L; otherwise 0 */
+    double kernel_ms;          /* hipEvent time of the adjoint and reduction kernels */
+} Q_adjoint_result;
+
+int32_t Q_plugin_adjoint(Q_plugin_handle* handle, const Q_adjoint_desc* desc, Q_adjoint_result* result);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

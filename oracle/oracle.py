@@ -61,6 +61,17 @@ def lib():
         L.qo_render.restype = C.c_int
         L.qo_render.argtypes = [C.c_void_p, P(QoParams), P(D), P(D), P(D), P(D), P(U), P(D), P(QoCounts)]
         L.qo_hardware_concurrency.restype = U
+        L.qo_adjoint.restype = C.c_int
+        L.qo_adjoint.argtypes = [C.c_void_p, P(QoParams), P(F), P(D), P(D), P(D)]
+        L.qo_tape_create.restype = C.c_void_p
+        L.qo_tape_create.argtypes = [C.c_void_p, P(QoParams), P(F)]
+        L.qo_tape_destroy.argtypes = [C.c_void_p]
+        L.qo_tape_paths.restype = C.c_uint64
+        L.qo_tape_paths.argtypes = [C.c_void_p]
+        L.qo_tape_vertices.restype = C.c_uint64
+        L.qo_tape_vertices.argtypes = [C.c_void_p]
+        L.qo_tape_eval.restype = D
+        L.qo_tape_eval.argtypes = [C.c_void_p, P(D), P(D), P(U), U, U]
         _lib = L
     return _lib
 
@@ -186,6 +197,42 @@ class Scene:
                         _p(d, C.c_double), _p(ids, C.c_uint32), _p(tp, C.c_double), C.byref(counts))
         return RenderResult(b, a, n, d, ids, tp, counts, spp)
 
+    def _params(self, width, height, spp, camera, sample_offset, max_depth, rr_start, seed, rng_mode, integrator,
+                use_bvh, threads, pixel_stride, id_sample, precision) -> "QoParams":
+        cam = camera or self.mesh.camera
+        p = QoParams()
+        p.width, p.height, p.spp, p.sample_offset = width, height, spp, sample_offset
+        p.max_depth, p.rr_start, p.seed, p.rng_mode, p.integrator = max_depth, rr_start, seed, rng_mode, integrator
+        p.use_bvh, p.threads, p.pixel_stride, p.id_sample, p.precision = (
+            1 if use_bvh else 0, threads, pixel_stride, id_sample, precision)
+        p.cam_position[:] = cam["position"]
+        p.cam_target[:] = cam["target"]
+        p.cam_up[:] = cam["up"]
+        p.cam_fov = cam["fov"]
+        return p
+
+    def adjoint(self, width, height, spp, dimg, camera=None, sample_offset=0, max_depth=5, rr_start=3, seed=0,
+                rng_mode=1, integrator=1, threads=0, pixel_stride=1, precision=64):
+        """Reverse-mode gradients of l = sum_pixels dimg . (1/spp) sum_s C_s
+        (dimg: (H, W, 4) float32) -> (d l/d albedo (T, 3), d l/d vertices (V, 3))."""
+        p = self._params(width, height, spp, camera, sample_offset, max_depth, rr_start, seed, rng_mode, integrator,
+                         True, threads, pixel_stride, 0, precision)
+        dimg = np.ascontiguousarray(dimg, dtype=np.float32)
+        assert dimg.shape == (height, width, 4)
+        ga = np.zeros((self.mesh.num_triangles, 3), dtype=np.float64)
+        gv = np.zeros((self.mesh.vertices.size // 3, 3), dtype=np.float64)
+        sec = C.c_double(0)
+        lib().qo_adjoint(self.handle, C.byref(p), _p(dimg, C.c_float), _p(ga, C.c_double), _p(gv, C.c_double),
+                         C.byref(sec))
+        self.last_adjoint_seconds = sec.value
+        return ga, gv
+
+    def tape(self, width, height, spp, dimg, camera=None, sample_offset=0, max_depth=5, rr_start=3, seed=0,
+             rng_mode=1, integrator=1, pixel_stride=1) -> "Tape":
+        p = self._params(width, height, spp, camera, sample_offset, max_depth, rr_start, seed, rng_mode, integrator,
+                         True, 0, pixel_stride, 0, 64)
+        return Tape(self, p, dimg)
+
     def primary_ray(self, params: dict, x, y, s):
         p = QoParams()
         cam = params.get("camera") or self.mesh.camera
@@ -203,6 +250,65 @@ class Scene:
     def close(self):
         if self.handle:
             lib().qo_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tape:
+    """The discrete decisions of every path of an image (FP64), for evaluating
+    l = sum_pixels dimg . (1/spp) sum_s C_s at other parameters with the
+    decisions frozen (central finite differences of the adjoint's estimator)."""
+
+    def __init__(self, scene: Scene, params, dimg):
+        self.scene = scene
+        dimg = np.ascontiguousarray(dimg, dtype=np.float32)
+        self.handle = lib().qo_tape_create(scene.handle, C.byref(params), _p(dimg, C.c_float))
+        mesh = scene.mesh
+        self.verts = np.ascontiguousarray(mesh.vertices, dtype=np.float64).reshape(-1, 3)
+        self.albedo = np.ascontiguousarray(mesh.albedo, dtype=np.float64).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(mesh.indices, dtype=np.uint32).reshape(-1, 3)
+
+    @property
+    def paths(self) -> int:
+        return int(lib().qo_tape_paths(self.handle))
+
+    def eval(self, verts=None, albedo=None, tris=None, threads=0) -> float:
+        v = np.ascontiguousarray(self.verts if verts is None else verts, dtype=np.float64)
+        a = np.ascontiguousarray(self.albedo if albedo is None else albedo, dtype=np.float64)
+        t = np.ascontiguousarray([] if tris is None else tris, dtype=np.uint32)
+        return float(lib().qo_tape_eval(self.handle, _p(v, C.c_double), _p(a, C.c_double),
+                                        _p(t, C.c_uint32) if t.size else None, int(t.size), threads))
+
+    def fd(self, kind: str, index: int, comp: int, eps: float = 1e-3, richardson: bool = False) -> float:
+        """Central difference of l in one parameter: kind 'albedo' (triangle
+        index, channel) or 'vertex' (vertex index, axis).  richardson: the
+        extrapolation (4 D(eps/2) - D(eps)) / 3, which cancels the O(eps^2)
+        truncation term of the central difference D."""
+        if richardson:
+            return (4.0 * self.fd(kind, index, comp, eps / 2) - self.fd(kind, index, comp, eps)) / 3.0
+        if kind == "albedo":
+            tris = [index]
+        else:
+            tris = np.nonzero((self.indices == index).any(axis=1))[0]
+        out = []
+        for sgn in (1.0, -1.0):
+            v = self.verts.copy()
+            a = self.albedo.copy()
+            if kind == "albedo":
+                a[index, comp] += sgn * eps
+            else:
+                v[index, comp] += sgn * eps
+            out.append(self.eval(v, a, tris))
+        return (out[0] - out[1]) / (2.0 * eps)
+
+    def close(self):
+        if self.handle:
+            lib().qo_tape_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

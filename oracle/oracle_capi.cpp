@@ -3,6 +3,7 @@
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the CPU
 // baseline leg of bench.py through ctypes, never by the product.
 #include "oracle.hpp"
+#include "oracle_adjoint.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -11,6 +12,8 @@ using namespace qo;
 
 struct qo_scene {
     Scene scene;
+    std::vector<uint32_t> idx;   // 3 vertex indices per triangle
+    uint32_t nv = 0;
     Bvh bvh;
     bool has_bvh = false;
     double bvh_build_ms = 0;
@@ -99,6 +102,8 @@ qo_scene* qo_scene_create(uint32_t nv, uint32_t nt, const float* verts, const ui
                           const float* albedo, const float* emission, int build_bvh) {
     auto* s = new qo_scene;
     s->scene.set(nv, nt, verts, idx, albedo, emission);
+    s->idx.assign(idx, idx + 3 * size_t(nt));
+    s->nv = nv;
     if (build_bvh) {
         auto t0 = std::chrono::steady_clock::now();
         s->bvh.build(s->scene);
@@ -174,5 +179,72 @@ int qo_render(const qo_scene* s, const qo_params* q, double* beauty, double* alb
 }
 
 uint32_t qo_hardware_concurrency(void) { return std::thread::hardware_concurrency(); }
+
+// ---------------------------------------------------------------- adjoint
+// Reverse-mode gradients of l = sum_pixels dimg . (1/spp) sum_s C_s (dimg:
+// RGBA per pixel) with respect to albedo (3 per triangle) and vertex
+// positions (3 per vertex), in q->precision (32 or 64).
+int qo_adjoint(const qo_scene* s, const qo_params* q, const float* dimg, double* grad_albedo,
+               double* grad_vertices, double* seconds) {
+    RenderParams p = to_rp(q);
+    const Bvh* bvh = (p.use_bvh && s->has_bvh) ? &s->bvh : nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> ga, gc;
+    if (q->precision == 64)
+        adjoint_render<double>(s->scene, bvh, p, dimg, ga, gc);
+    else
+        adjoint_render<float>(s->scene, bvh, p, dimg, ga, gc);
+    if (grad_albedo) std::copy(ga.begin(), ga.end(), grad_albedo);
+    if (grad_vertices) {
+        std::fill(grad_vertices, grad_vertices + 3 * size_t(s->nv), 0.0);
+        for (size_t t = 0; t < s->scene.ntri; ++t)
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) grad_vertices[3 * size_t(s->idx[3 * t + k]) + c] += gc[9 * t + 3 * k + c];
+    }
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+struct qo_tape {
+    const qo_scene* scene;
+    Tape tape;
+};
+
+// Records (in FP64) the decisions of every path of the pixels with non-zero
+// dimg, for qo_tape_eval.
+qo_tape* qo_tape_create(const qo_scene* s, const qo_params* q, const float* dimg) {
+    auto* t = new qo_tape;
+    t->scene = s;
+    RenderParams p = to_rp(q);
+    t->tape.record(s->scene, (p.use_bvh && s->has_bvh) ? &s->bvh : nullptr, p, dimg);
+    return t;
+}
+
+void qo_tape_destroy(qo_tape* t) { delete t; }
+
+uint64_t qo_tape_paths(const qo_tape* t) { return t->tape.path_pixel.size(); }
+uint64_t qo_tape_vertices(const qo_tape* t) { return t->tape.vx.size(); }
+
+// l over the recorded paths for vertex positions `verts` (3 per vertex) and
+// albedos `albedo` (3 per triangle), FP64, decisions frozen.  Only the paths
+// that visit one of the `ntris` triangles listed are evaluated (all when
+// ntris is 0): the others do not depend on those triangles' parameters.
+double qo_tape_eval(const qo_tape* t, const double* verts, const double* albedo, const uint32_t* tris,
+                    uint32_t ntris, uint32_t threads) {
+    const qo_scene* s = t->scene;
+    const size_t nt = s->scene.ntri;
+    std::vector<double> corners(9 * nt);
+    for (size_t i = 0; i < nt; ++i)
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) corners[9 * i + 3 * k + c] = verts[3 * size_t(s->idx[3 * i + k]) + c];
+    ParamView<double> pv{&s->scene, corners.data(), albedo};
+    std::vector<char> rel;
+    if (ntris) {
+        rel.assign(nt, 0);
+        for (uint32_t i = 0; i < ntris; ++i) rel[tris[i]] = 1;
+    }
+    unsigned nth = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    return t->tape.eval(pv, ntris ? &rel : nullptr, nth);
+}
 
 }  // extern "C"

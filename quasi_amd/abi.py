@@ -126,6 +126,22 @@ class Q_autotune_result(C.Structure):
                 ("from_cache", C.c_uint32), ("entries", Q_autotune_entry * Q_AUTOTUNE_MAX_ENTRIES)]
 
 
+Q_ADJOINT_IMAGE_GRADIENT, Q_ADJOINT_L2_TARGET = 0, 1
+Q_ADJOINT_SEED = 1
+
+
+class Q_adjoint_desc(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("mode", C.c_uint32), ("spp", C.c_uint32),
+                ("sample_begin", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32),
+                ("image", C.POINTER(C.c_float)), ("grad_albedo", C.POINTER(C.c_double)),
+                ("grad_vertices", C.POINTER(C.c_double))]
+
+
+class Q_adjoint_result(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("reserved", C.c_uint32), ("paths", C.c_uint64),
+                ("loss", C.c_double), ("kernel_ms", C.c_double)]
+
+
 # Symbol names, plugin_interface.hpp:170-179
 K_SYMBOL_ABI_VERSION = "Q_plugin_abi_version"
 K_SYMBOL_GET_INFO = "Q_plugin_get_info"
@@ -145,4 +161,5 @@ OPTIONAL_SYMBOLS = (K_SYMBOL_READBACK, K_SYMBOL_READBACK_FREE, K_SYMBOL_READBACK
 EXTENSION_SYMBOLS = ("Q_plugin_set_scene", "Q_plugin_get_params", "Q_plugin_set_params",
                      "Q_plugin_readback_ids", "Q_plugin_get_stats", "Q_plugin_reset_stats",
                      "Q_plugin_synchronize", "Q_plugin_set_batch", "Q_plugin_render_batch",
-                     "Q_plugin_readback_batch", "Q_plugin_readback_batch_ids", "Q_plugin_autotune")
+                     "Q_plugin_readback_batch", "Q_plugin_readback_batch_ids", "Q_plugin_autotune",
+                     "Q_plugin_adjoint")

@@ -36,8 +36,10 @@ PLUGIN_SOURCES = [
     ("kernels_mega.hip", DEVICE),
     ("kernels_wave.hip", DEVICE),
     ("kernels_util.hip", DEVICE),
+    ("kernels_adjoint.hip", DEVICE),
     ("plugin.cpp", HOSTONLY),
     ("plugin_batch.cpp", HOSTONLY),
+    ("plugin_adjoint.cpp", HOSTONLY),
     ("bvh_build.cpp", HOSTONLY),
     ("scene_cornell.cpp", HOSTONLY),
 ]

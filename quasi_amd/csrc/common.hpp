@@ -135,6 +135,18 @@ struct PassParams {
     const uint32_t* dyn_sample_begin;  // when set, the pass's first sample index is read here
 };
 
+// One adjoint launch (path replay of pass.sample_begin .. + pass.sample_count - 1).
+struct AdjointParams {
+    PassParams pass;              // scene, camera, integrator settings, replayed samples
+    const void* dimg;             // float4 per pixel: dL/dI (rgb; w unused)
+    float scale;                  // 1 / samples averaged into the image
+    uint32_t replicas;            // FP32 gradient copies (blockIdx % replicas)
+    const uint32_t* indices;      // 3 vertex indices per input triangle
+    float* grad_albedo;           // replicas x 3 x num_tris
+    float* grad_vertices;         // replicas x 3 x num_vertices
+    uint32_t num_tris, num_vertices;
+};
+
 // Launch shape of the persistent traversal kernels (autotuned).
 struct TraceConfig {
     uint32_t width;   // BVH width: 2 or 4

@@ -427,6 +427,17 @@ struct FirstHit {
     uint32_t id;
 };
 
+// Light choice: the first entry with rs < cdf (the last when none).
+__device__ __forceinline__ uint32_t select_light(const LightEntry* __restrict__ lights, uint32_t num_lights,
+                                                 float rs) {
+    uint32_t lo = 0, hi = num_lights;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (rs < lights[mid].cdf) hi = mid; else lo = mid + 1;
+    }
+    return lo >= num_lights ? num_lights - 1 : lo;
+}
+
 // One vertex of either integrator at bounce b for a ray that hit BVH triangle
 // `tri` at distance t.  Updates throughput, radiance and the random stream.
 __device__ __forceinline__ void shade_hit(const PassParams& P, const float4* __restrict__ tris,
@@ -472,15 +483,7 @@ __device__ __forceinline__ void shade_hit(const PassParams& P, const float4* __r
     if (b + 1 >= P.max_depth) return;
     const f3 org = pt + nf * kOffset;
     if (num_lights) {
-        // light choice: first entry with r < cdf (the last when none)
-        const float rs = random_float(rng);
-        uint32_t lo = 0, hi = num_lights;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (rs < lights[mid].cdf) hi = mid; else lo = mid + 1;
-        }
-        if (lo >= num_lights) lo = num_lights - 1;
-        const LightEntry le = lights[lo];
+        const LightEntry le = lights[select_light(lights, num_lights, random_float(rng))];
         const float r1 = random_float(rng);
         const float r2 = random_float(rng);
         const float4 l0 = tris[3 * le.tri + 0];

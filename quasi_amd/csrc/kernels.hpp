@@ -54,6 +54,24 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
 // out[i] = sum[i] / n for npix RGBA pixels (the running mean of plugin.mm:326-327).
 hipError_t launch_resolve(const float4* sum, float4* out, size_t npix, uint32_t n, hipStream_t stream);
 
+// Path-replay adjoint over the whole framebuffer with `blocks` blocks of
+// adjoint_block_size() threads (blocks * adjoint_block_size() must not exceed
+// the threads p.pass.gstack was sized for).  Adds into the FP32 replicas,
+// which the caller clears first.
+uint32_t adjoint_block_size();
+uint32_t adjoint_max_depth();
+hipError_t launch_adjoint(const AdjointParams& p, uint32_t blocks, hipStream_t stream);
+
+// out[i] = sum over r < replicas of (double) rep[r * n + i].
+hipError_t launch_replica_reduce(const float* rep, size_t n, uint32_t replicas, double* out, hipStream_t stream);
+
+// L2 image loss against a target: dimg = 2 (sum / n - target) per pixel (rgb),
+// and one FP32 partial sum of squared differences per block in partials
+// (l2_image_blocks(npix) of them).
+uint32_t l2_image_blocks(size_t npix);
+hipError_t launch_l2_image(const float4* sum, uint32_t n, const float4* target, float4* dimg, size_t npix,
+                           float* partials, hipStream_t stream);
+
 // Fills n 32-bit words with value (used instead of hipMemsetAsync so that the
 // clears stay valid inside captured graphs).
 hipError_t launch_fill_u32(void* ptr, size_t n, uint32_t value, hipStream_t stream);

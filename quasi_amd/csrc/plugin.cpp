@@ -284,6 +284,7 @@ int32_t upload_scene(plugin_state* st) {
     st->num_nodes4 = sh.num_nodes4;
     st->bvh_depth4 = sh.depth4;
     st->gstack_entries = gentries;
+    ++st->scene_generation;
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return Q_EXT_OK;
 }
@@ -314,6 +315,7 @@ void destroy_state(plugin_state* st) {
     for (hipEvent_t e : st->event_pool) (void)hipEventDestroy(e);
     st->event_pool.clear();
     free_batch(st);
+    free_adjoint(st);
     free_buffer(st->tris);
     free_buffer(st->mats);
     free_buffer(st->nodes);

@@ -1,0 +1,172 @@
+// plugin_adjoint.cpp -- Q_plugin_adjoint: gradients of an image loss with
+// respect to the current scene's per-triangle albedo and vertex positions
+// (extension export of include/quasi/quasi_hip_ext.h; the reference renders a
+// fixed scene and has no derivatives, backends/metal/plugin.mm:584).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "plugin_state.hpp"
+
+#define Q_EXPORT __attribute__((visibility("default")))
+
+namespace quasi_hip {
+namespace host {
+
+constexpr uint32_t kAdjointReplicas = 8;   // FP32 gradient copies, one per XCD of an MI355X
+
+void free_adjoint(plugin_state* st) {
+    AdjointState& a = st->adjoint;
+    for (auto* b : {&a.indices, &a.image, &a.dimg, &a.grads, &a.out, &a.partials}) free_buffer(*b);
+    a.indices_generation = 0;
+}
+
+namespace {
+
+plugin_state* as_state(Q_plugin_handle* h) { return reinterpret_cast<plugin_state*>(h); }
+
+int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res) {
+    if (d->mode > Q_ADJOINT_L2_TARGET || d->spp == 0 || !d->image) {
+        log_msg(st, "adjoint: bad descriptor (mode %u, spp %u)", d->mode, d->spp);
+        return Q_EXT_ERR_ARGUMENT;
+    }
+    const Q_render_params& rp = st->params;
+    if (st->num_tris == 0 || !st->have_camera || st->width == 0 || st->height == 0) {
+        log_msg(st, "adjoint: needs a scene and a rendered frame (camera and resolution)");
+        return Q_EXT_ERR_STATE;
+    }
+    if (rp.max_depth > adjoint_max_depth()) {
+        log_msg(st, "adjoint: depth %u is not supported (at most %u)", rp.max_depth, adjoint_max_depth());
+        return Q_EXT_ERR_UNSUPPORTED;
+    }
+    if (d->mode == Q_ADJOINT_L2_TARGET && st->sample_count == 0) {
+        log_msg(st, "adjoint: the L2 target mode needs an accumulated image");
+        return Q_EXT_ERR_STATE;
+    }
+    if (uint64_t(d->sample_begin) + d->spp > 0xFFFFFFFFull) return Q_EXT_ERR_ARGUMENT;
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    AdjointState& a = st->adjoint;
+    const size_t npix = size_t(st->width) * st->height;
+    const size_t nt = st->num_tris;
+    const size_t nv = st->verts.size() / 3;
+    const uint32_t nblk = l2_image_blocks(npix);
+    if (!ensure_buffer(st, a.indices, 4 * 3 * nt, "adjoint indices") ||
+        !ensure_buffer(st, a.image, 16 * npix, "adjoint image") ||
+        !ensure_buffer(st, a.dimg, 16 * npix, "adjoint image gradient") ||
+        !ensure_buffer(st, a.grads, 4 * size_t(kAdjointReplicas) * 3 * (nt + nv), "adjoint gradients") ||
+        !ensure_buffer(st, a.out, 8 * 3 * (nt + nv), "adjoint FP64 gradients") ||
+        !ensure_buffer(st, a.partials, 4 * size_t(nblk), "adjoint loss partials"))
+        return Q_EXT_ERR_OUT_OF_MEMORY;
+    hipStream_t s = st->stream;
+    if (a.indices_generation != st->scene_generation) {
+        if (!hip_ok(st, hipMemcpy(a.indices.ptr, st->indices.data(), 4 * 3 * nt, hipMemcpyHostToDevice),
+                    "adjoint index upload"))
+            return Q_EXT_ERR_DEVICE;
+        a.indices_generation = st->scene_generation;
+    }
+    if (!hip_ok(st, hipMemcpy(a.image.ptr, d->image, 16 * npix, hipMemcpyHostToDevice), "adjoint image upload") ||
+        !hip_ok(st, launch_fill_u32(a.grads.ptr, size_t(kAdjointReplicas) * 3 * (nt + nv), 0u, s), "gradient clear"))
+        return Q_EXT_ERR_DEVICE;
+
+    hipEvent_t e0 = take_event(st), e1 = take_event(st);
+    if (!e0 || !e1) return Q_EXT_ERR_DEVICE;
+    bool ok = hip_ok(st, hipEventRecord(e0, s), "event");
+    const void* dimg = a.image.ptr;
+    if (ok && d->mode == Q_ADJOINT_L2_TARGET) {
+        ok = hip_ok(st, launch_l2_image(static_cast<const float4*>(st->accum[Q_AOV_BEAUTY].ptr), st->sample_count,
+                                        static_cast<const float4*>(a.image.ptr), static_cast<float4*>(a.dimg.ptr),
+                                        npix, static_cast<float*>(a.partials.ptr), s),
+                    "L2 image gradient");
+        dimg = a.dimg.ptr;
+    }
+    AdjointParams A{};
+    PassParams& p = A.pass;
+    p.tris = st->tris.ptr;
+    p.mats = st->mats.ptr;
+    p.nodes = st->nodes.ptr;
+    p.lights = static_cast<const LightEntry*>(st->lights.ptr);
+    p.cam = make_camera(st->last_camera, st->width, st->height);
+    p.width = st->width;
+    p.height = st->height;
+    p.sample_begin = d->sample_begin;
+    p.sample_count = d->spp;
+    p.max_depth = rp.max_depth;
+    p.rr_start = rp.rr_start;
+    p.seed = (d->flags & Q_ADJOINT_SEED) ? d->seed : rp.seed;
+    p.rng_mode = rp.rng_mode;
+    p.integrator = rp.integrator;
+    p.num_lights = st->num_lights;
+    p.num_nodes = st->num_nodes;
+    p.nodes4 = st->nodes4.ptr;
+    p.num_nodes4 = st->num_nodes4;
+    p.gstack = static_cast<int*>(st->gstack.ptr);
+    p.gstack_entries = st->gstack_entries;
+    A.dimg = dimg;
+    A.scale = 1.0f / float(d->spp);
+    A.replicas = kAdjointReplicas;
+    A.indices = static_cast<const uint32_t*>(a.indices.ptr);
+    A.grad_albedo = static_cast<float*>(a.grads.ptr);
+    A.grad_vertices = static_cast<float*>(a.grads.ptr) + size_t(kAdjointReplicas) * 3 * nt;
+    A.num_tris = uint32_t(nt);
+    A.num_vertices = uint32_t(nv);
+    // the persistent traversal stacks hold kMaxThreadsPerCU threads per CU
+    const uint32_t tiles = ((st->width + 15) / 16) * ((st->height + 7) / 8);
+    const uint32_t cap = uint32_t(st->num_cus) * (kMaxThreadsPerCU / adjoint_block_size());
+    const uint32_t blocks = std::max(1u, std::min(tiles, cap));
+    double* out = static_cast<double*>(a.out.ptr);
+    ok = ok && hip_ok(st, launch_adjoint(A, blocks, s), "adjoint") &&
+         hip_ok(st, launch_replica_reduce(A.grad_albedo, 3 * nt, kAdjointReplicas, out, s), "albedo reduction") &&
+         hip_ok(st, launch_replica_reduce(A.grad_vertices, 3 * nv, kAdjointReplicas, out + 3 * nt, s),
+                "vertex reduction") &&
+         hip_ok(st, hipEventRecord(e1, s), "event") && hip_ok(st, hipEventSynchronize(e1), "adjoint");
+    float ms = 0.f;
+    if (ok) ok = hip_ok(st, hipEventElapsedTime(&ms, e0, e1), "adjoint time");
+    st->event_pool.push_back(e0);
+    st->event_pool.push_back(e1);
+    if (!ok) return Q_EXT_ERR_DEVICE;
+    if (d->grad_albedo &&
+        !hip_ok(st, hipMemcpy(d->grad_albedo, out, 8 * 3 * nt, hipMemcpyDeviceToHost), "albedo gradient readback"))
+        return Q_EXT_ERR_DEVICE;
+    if (d->grad_vertices &&
+        !hip_ok(st, hipMemcpy(d->grad_vertices, out + 3 * nt, 8 * 3 * nv, hipMemcpyDeviceToHost),
+                "vertex gradient readback"))
+        return Q_EXT_ERR_DEVICE;
+    double loss = 0.0;
+    if (d->mode == Q_ADJOINT_L2_TARGET) {
+        std::vector<float> part(nblk);
+        if (!hip_ok(st, hipMemcpy(part.data(), a.partials.ptr, 4 * size_t(nblk), hipMemcpyDeviceToHost),
+                    "loss readback"))
+            return Q_EXT_ERR_DEVICE;
+        for (float v : part) loss += double(v);
+    }
+    if (res) {
+        res->paths = uint64_t(npix) * d->spp;
+        res->loss = loss;
+        res->kernel_ms = ms;
+    }
+    return Q_EXT_OK;
+}
+
+}  // namespace
+}  // namespace host
+}  // namespace quasi_hip
+
+using namespace quasi_hip::host;
+
+extern "C" {
+
+Q_EXPORT int32_t Q_plugin_adjoint(Q_plugin_handle* handle, const Q_adjoint_desc* desc, Q_adjoint_result* result) {
+    if (!handle || !desc || desc->struct_size < sizeof(Q_adjoint_desc) ||
+        (result && result->struct_size < sizeof(Q_adjoint_result)))
+        return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    try {
+        return adjoint(st, desc, result);
+    } catch (...) {
+        log_msg(st, "adjoint: exception");
+        return Q_EXT_ERR_STATE;
+    }
+}
+
+}  // extern "C"

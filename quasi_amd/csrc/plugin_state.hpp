@@ -65,6 +65,17 @@ struct BatchState {
     uint64_t graph_key = 0;
 };
 
+// Buffers of Q_plugin_adjoint (kept between calls).
+struct AdjointState {
+    DeviceBuffer indices;        // 3 vertex indices per input triangle
+    DeviceBuffer image;          // caller's image: dL/dI or the L2 target
+    DeviceBuffer dimg;           // dL/dI formed on the device (L2 target mode)
+    DeviceBuffer grads;          // FP32 replicas: albedo then vertices
+    DeviceBuffer out;            // FP64 reduced gradients: albedo then vertices
+    DeviceBuffer partials;       // FP32 per-block loss sums
+    uint64_t indices_generation = 0;
+};
+
 struct plugin_state {
     Q_plugin_context* context = nullptr;
     int device = 0;
@@ -103,6 +114,8 @@ struct plugin_state {
     int num_cus = 256;
 
     BatchState batch;
+    AdjointState adjoint;
+    uint64_t scene_generation = 0;      // incremented by every scene upload
     Q_camera last_camera{};
     bool have_camera = false;
 
@@ -155,6 +168,7 @@ CameraBasis make_camera(const Q_camera& c, uint32_t w, uint32_t h);
 Q_aov_buffer read_mean_of(plugin_state* st, const void* sums, uint32_t width, uint32_t height, uint32_t n);
 
 void free_batch(plugin_state* st);
+void free_adjoint(plugin_state* st);
 
 // The body of Q_plugin_render (false when a step failed; the failure is logged).
 bool render_frame(plugin_state* st, const Q_render_frame* frame);

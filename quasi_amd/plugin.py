@@ -256,6 +256,7 @@ class HipRenderer:
         self._bind_ext()
         self.width, self.height = width, height
         self.camera = default_camera()
+        self._num_vertices = 64          # the plugin's default scene: 16 quads of 4 vertices
         self._camera_dirty = True
 
     def _bind_ext(self):
@@ -275,6 +276,7 @@ class HipRenderer:
                                      [_H, C.c_uint32, P(C.c_uint32), C.c_uint64])
         self._autotune = g("Q_plugin_autotune", C.c_int32, [_H, C.c_uint32, C.c_uint32, C.c_uint32,
                                                             P(abi.Q_autotune_result)])
+        self._adjoint = g("Q_plugin_adjoint", C.c_int32, [_H, P(abi.Q_adjoint_desc), P(abi.Q_adjoint_result)])
 
     def _check(self, rc: int, what: str) -> None:
         if rc != abi.Q_EXT_OK:
@@ -285,6 +287,7 @@ class HipRenderer:
     def set_scene(self, mesh) -> None:
         d, keep = scene_desc(mesh)
         self._check(self._set_scene(self.plugin.handle, C.byref(d)), "set_scene")
+        self._num_vertices = int(d.num_vertices)
         del keep
 
     def set_batch(self, scenes) -> None:
@@ -330,6 +333,25 @@ class HipRenderer:
         r.struct_size = C.sizeof(abi.Q_autotune_result)
         self._check(self._autotune(self.plugin.handle, target, probe_spp, flags, C.byref(r)), "autotune")
         return r
+
+    def adjoint(self, image, spp: int, sample_begin: int = 0, mode: int = abi.Q_ADJOINT_IMAGE_GRADIENT,
+                seed: Optional[int] = None, num_vertices: Optional[int] = None):
+        """Q_plugin_adjoint for the current scene, camera and resolution.
+        image: (H, W, 4) float32 dL/dI (or the L2 target).  Returns
+        (grad_albedo (T, 3) float64, grad_vertices (V, 3) float64, result)."""
+        img = np.ascontiguousarray(image, dtype=np.float32)
+        nt = self.stats().num_triangles
+        nv = self._num_vertices if num_vertices is None else num_vertices
+        ga = np.zeros((nt, 3), dtype=np.float64)
+        gv = np.zeros((nv, 3), dtype=np.float64)
+        d = abi.Q_adjoint_desc(C.sizeof(abi.Q_adjoint_desc), mode, spp, sample_begin, 0 if seed is None else seed,
+                               0 if seed is None else abi.Q_ADJOINT_SEED,
+                               img.ctypes.data_as(C.POINTER(C.c_float)),
+                               ga.ctypes.data_as(C.POINTER(C.c_double)), gv.ctypes.data_as(C.POINTER(C.c_double)))
+        res = abi.Q_adjoint_result()
+        res.struct_size = C.sizeof(abi.Q_adjoint_result)
+        self._check(self._adjoint(self.plugin.handle, C.byref(d), C.byref(res)), "adjoint")
+        return ga, gv, res
 
     def get_params(self) -> abi.Q_render_params:
         p = abi.Q_render_params()

@@ -221,3 +221,79 @@ def batch_scenes(num_scenes: int = 64, base_seed: int = 1000, min_tris: float = 
 
 def camera_args(mesh: Mesh) -> dict:
     return dict(mesh.camera)
+
+
+# ---------------------------------------------------------------- differentiable-rendering scene
+DIFF_CAMERA = dict(position=(0.0, 1.1, 2.4), target=(0.0, 0.45, 0.0), up=(0.0, 1.0, 0.0), fov=40.0)
+
+
+def icosphere(frequency: int, radius: float = 0.5, center=(0.0, 0.5, 0.0)):
+    """Geodesic icosphere: every face of the icosahedron split into
+    frequency^2 triangles whose grid points are projected onto the sphere.
+    Returns (vertices (10 f^2 + 2, 3) float32, indices (20 f^2, 3) uint32),
+    triangles wound outwards."""
+    f = int(frequency)
+    t = (1.0 + np.sqrt(5.0)) / 2.0
+    ico_v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                      [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    ico_v /= np.linalg.norm(ico_v, axis=1, keepdims=True)
+    ico_f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    key_of = {}
+    verts = []
+
+    def vid(p):
+        q = p / np.linalg.norm(p)
+        k = tuple(np.round(q, 9))
+        if k not in key_of:
+            key_of[k] = len(verts)
+            verts.append(q)
+        return key_of[k]
+
+    tris = []
+    for a, b, c in ico_f:
+        A, B, Cc = ico_v[a], ico_v[b], ico_v[c]
+        grid = {}
+        for i in range(f + 1):
+            for j in range(f + 1 - i):
+                grid[(i, j)] = vid(A + (B - A) * (i / f) + (Cc - A) * (j / f))
+        for i in range(f):
+            for j in range(f - i):
+                tris.append((grid[(i, j)], grid[(i + 1, j)], grid[(i, j + 1)]))
+                if i + j + 1 < f:
+                    tris.append((grid[(i + 1, j)], grid[(i + 1, j + 1)], grid[(i, j + 1)]))
+    v = np.asarray(verts) * radius + np.asarray(center)
+    idx = np.asarray(tris, dtype=np.uint32)
+    # wind outwards
+    p0, p1, p2 = v[idx[:, 0]], v[idx[:, 1]], v[idx[:, 2]]
+    n = np.cross(p1 - p0, p2 - p0)
+    inward = (n * (p0 - np.asarray(center))).sum(axis=1) < 0
+    idx[inward] = idx[inward][:, [0, 2, 1]]
+    return v.astype(F32), idx
+
+
+def diff_sphere(frequency: int = 22, stage: str = "target", seed_target: int = 4, seed_init: int = 5) -> Mesh:
+    """Differentiable-rendering scene: an icosphere (radius 0.5, resting on the
+    ground at the origin; frequency 22 gives 9,680 triangles and 4,842
+    vertices) on a 4 x 4 ground quad (albedo 0.6), lit by two quad lights
+    facing -y (0.6 x 0.6 at y = 2.0, radiance 12; 0.4 x 0.4 at y = 1.6,
+    radiance (10, 8, 6)).
+    stage "target": sphere albedos U[0.2, 0.9]^3 per triangle (seed_target);
+    stage "init": sphere albedos 0.5 and sphere vertices moved by N(0, 0.01)
+    per coordinate (seed_init)."""
+    v, idx = icosphere(frequency)
+    nt = idx.shape[0]
+    if stage == "target":
+        alb = np.random.Generator(np.random.PCG64(seed_target)).uniform(0.2, 0.9, size=(nt, 3)).astype(F32)
+    elif stage == "init":
+        alb = np.full((nt, 3), 0.5, dtype=F32)
+        v = (v + np.random.Generator(np.random.PCG64(seed_init)).normal(0.0, 0.01, size=v.shape)).astype(F32)
+    else:
+        raise ValueError(stage)
+    b = _Builder()
+    b.v, b.i, b.a, b.e = [v], [idx], [alb], [np.zeros((nt, 3), dtype=F32)]
+    b.quad((-2.0, 0.0, 2.0), (4.0, 0.0, 0.0), (0.0, 0.0, -4.0), (0.6, 0.6, 0.6))
+    b.quad((-0.9, 2.0, -0.1), (0.6, 0.0, 0.0), (0.0, 0.0, 0.6), (0.0, 0.0, 0.0), (12.0, 12.0, 12.0))
+    b.quad((0.6, 1.6, -0.6), (0.4, 0.0, 0.0), (0.0, 0.0, 0.4), (0.0, 0.0, 0.0), (10.0, 8.0, 6.0))
+    return b.mesh(DIFF_CAMERA, f"diff_sphere{frequency}_{stage}")

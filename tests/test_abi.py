@@ -55,7 +55,7 @@ def test_symbol_name_constants():
 LAYOUT_STRUCTS = ["Q_gpu_context", "Q_camera", "Q_render_frame", "Q_plugin_version", "Q_plugin_info",
                   "Q_plugin_context", "Q_readback_result", "Q_aov_buffer", "Q_readback_aov_result",
                   "Q_scene_desc", "Q_render_params", "Q_kernel_stat", "Q_render_stats", "Q_batch_scene",
-                  "Q_batch_desc", "Q_autotune_entry", "Q_autotune_result"]
+                  "Q_batch_desc", "Q_autotune_entry", "Q_autotune_result", "Q_adjoint_desc", "Q_adjoint_result"]
 
 
 def test_struct_layouts_match_c(tmp_path):
