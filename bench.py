@@ -129,7 +129,9 @@ def bench_batch(args):
         t0 = time.perf_counter()
         r.set_batch(batch)
         prep_s = time.perf_counter() - t0
-        tune = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=args.batch_samples_per_pass, flags=abi.Q_AUTOTUNE_NO_STORE)
+        # probe with one pass of the size the timed render uses
+        probe = args.batch_samples_per_pass or 4
+        tune = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=probe, flags=abi.Q_AUTOTUNE_NO_STORE)
         best = tune.entries[tune.best]
         log(f"batch: prepared in {prep_s:.1f} s; autotuned width {best.bvh_width} block {best.block_size} "
             f"chunk {best.queue_chunk} ({best.ms:.2f} ms per probe)")
@@ -233,7 +235,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--ntri", type=int, default=1_000_000)
-    ap.add_argument("--spp-per-launch", type=int, default=8)
+    ap.add_argument("--spp-per-launch", type=int, default=64)
     ap.add_argument("--engine", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--samples-per-pass", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=4)
@@ -244,7 +246,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=32)
-    ap.add_argument("--batch-samples-per-pass", type=int, default=1)
+    ap.add_argument("--batch-samples-per-pass", type=int, default=0, help="0: the plugin's automatic pass size")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--adjoint-runs", type=int, default=5)
     ap.add_argument("--batch-progressive", type=int, default=1,

@@ -83,7 +83,9 @@ typedef struct Q_render_params {
     uint32_t integrator;       /* Q_INTEGRATOR_* */
     uint32_t engine;           /* Q_ENGINE_* */
     uint32_t bvh_width;        /* wavefront traversal: 2 or 4 (0 = 2); the megakernel walks BVH2 */
-    uint32_t samples_per_pass; /* wavefront: samples per pixel in flight at once (0 = auto) */
+    uint32_t samples_per_pass; /* wavefront: samples per pixel in flight at once (0 = auto: about
+                                  2^27 paths per pass; for a batch, the largest power of two that
+                                  divides the samples requested and stays within that) */
     uint32_t block_size;       /* wavefront traversal threads per block: 64, 128, 256 (0 = 256) */
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
@@ -142,7 +144,7 @@ int32_t Q_plugin_synchronize(Q_plugin_handle* handle);
  * pass covers all scenes with shared, compacted ray queues, and the pass is
  * captured once in a hipGraph and replayed per pass.  The render parameters
  * (depth, integrator, seeding mode, traversal shape, samples_per_pass: samples
- * of every scene per replayed pass, 0 = 1) are those of Q_plugin_set_params;
+ * of every scene per replayed pass) are those of Q_plugin_set_params;
  * each scene brings its own camera, resolution and seed.  With the same
  * parameters, scene i of a batch renders exactly what a single-scene render
  * of it with seed = scenes[i].seed and the same samples_per_pass gives. */
@@ -166,8 +168,8 @@ typedef struct Q_batch_desc {
 int32_t Q_plugin_set_batch(Q_plugin_handle* handle, const Q_batch_desc* batch);
 
 /* Adds spp samples to every scene of the batch, asynchronously on `stream`
- * (a hipStream_t; NULL: the plugin's stream).  spp must be a multiple of the
- * samples per pass. */
+ * (a hipStream_t; NULL: the plugin's stream).  spp must be a multiple of an
+ * explicit samples_per_pass. */
 int32_t Q_plugin_render_batch(Q_plugin_handle* handle, uint32_t spp, void* stream);
 
 /* The mean AOVs of one scene (malloc'd, free with Q_plugin_readback_aov_free). */

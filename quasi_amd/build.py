@@ -70,16 +70,28 @@ def build_plugin(verbose: bool = False, jobs: int = 8) -> Path:
     return PLUGIN_SO
 
 
+HOST_BIN_SOURCES = ["main.cpp", "hip_context.cpp", "exr_writer.cpp", "scene_file.cpp"]
+IO_LIB = PKG / "libquasi_io.so"
+IO_LIB_SOURCES = ["exr_writer.cpp", "scene_file.cpp", "io_capi.cpp"]
+
+
 def build_host(verbose: bool = False) -> Path | None:
-    main = HOST / "main.cpp"
-    if not main.exists():
+    """The headless host (quasi_amd/bin/quasi_host: HIP context, plugin loader,
+    EXR writer) and libquasi_io.so (EXR writer and scene-file reader for the
+    tests)."""
+    if not (HOST / "main.cpp").exists():
         return None
     HOST_BIN.parent.mkdir(exist_ok=True)
-    srcs = sorted(str(p) for p in HOST.glob("*.cpp"))
     tmp = HOST_BIN.with_suffix(".tmp")
     _run([HIPCC, "-O2", "-std=c++20", "-Wall", "-Wextra", "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-          "-I", ROOT / "include", "-I", HOST, *srcs, "-o", tmp, "-ldl", "-lz", "-pthread"], verbose)
+          "-I", ROOT / "include", "-I", HOST, *[str(HOST / s) for s in HOST_BIN_SOURCES], "-o", tmp,
+          "-ldl", "-lz", "-pthread"], verbose)
     os.replace(tmp, HOST_BIN)
+    tmp = IO_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-fPIC", "-shared", "-fvisibility=hidden",
+          "-I", ROOT / "include", "-I", HOST, *[str(HOST / s) for s in IO_LIB_SOURCES], "-o", tmp, "-lz",
+          "-pthread"], verbose)
+    os.replace(tmp, IO_LIB)
     return HOST_BIN
 
 

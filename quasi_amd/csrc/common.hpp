@@ -155,6 +155,11 @@ struct TraceConfig {
 };
 
 constexpr int kShortStack = 16;       // LDS entries per lane in the persistent kernels
+
+// Paths in flight per automatically sized wavefront pass: about 25 GB of path
+// state (188 B per path), large enough that the short queues of the last
+// bounces are a small part of a pass.
+constexpr uint32_t kAutoPassPaths = 1u << 27;
 constexpr uint32_t kMaxThreadsPerCU = 2048;
 
 }  // namespace quasi_hip

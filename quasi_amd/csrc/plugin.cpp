@@ -300,7 +300,7 @@ void free_wave(WaveBuffers& w) {
 uint32_t wave_samples_per_pass(const plugin_state* st) {
     const uint32_t npix = st->width * st->height;
     uint32_t s = st->params.samples_per_pass;
-    if (s == 0) s = std::max<uint32_t>(1u, (16u << 20) / std::max<uint32_t>(1u, npix));
+    if (s == 0) s = std::max<uint32_t>(1u, kAutoPassPaths / std::max<uint32_t>(1u, npix));
     s = std::min(s, st->params.spp_per_render);
     while (s > 1 && uint64_t(s) * npix > 0xFFFFFFFFull / 2) s /= 2;
     return std::max(1u, s);

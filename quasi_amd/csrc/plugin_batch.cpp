@@ -57,8 +57,14 @@ bool clear_batch(plugin_state* st, hipStream_t s) {
     return true;
 }
 
-uint32_t batch_samples_per_pass(const plugin_state* st) {
-    return st->params.samples_per_pass ? st->params.samples_per_pass : 1u;
+// Samples per batched pass: the explicit setting, or (0) the largest power of
+// two that divides spp and keeps a pass within kAutoPassPaths paths.
+uint32_t batch_samples_per_pass(const plugin_state* st, uint32_t spp) {
+    if (st->params.samples_per_pass) return st->params.samples_per_pass;
+    const uint64_t cap = std::max<uint64_t>(1, kAutoPassPaths / std::max<uint32_t>(1u, st->batch.total_pixels));
+    uint32_t s = 1;
+    while (uint64_t(s) * 2 <= cap && spp % (s * 2) == 0) s *= 2;
+    return s;
 }
 
 // FNV-1a over the values a captured pass depends on.
@@ -157,8 +163,12 @@ int32_t render_batch(plugin_state* st, uint32_t spp, hipStream_t s) {
         log_msg(st, "render_batch: no batch set");
         return Q_EXT_ERR_STATE;
     }
-    const uint32_t per_pass = batch_samples_per_pass(st);
-    if (spp == 0 || spp % per_pass) {
+    if (spp == 0) {
+        log_msg(st, "render_batch: no samples");
+        return Q_EXT_ERR_ARGUMENT;
+    }
+    const uint32_t per_pass = batch_samples_per_pass(st, spp);
+    if (spp % per_pass) {
         log_msg(st, "render_batch: %u samples is not a multiple of %u samples per pass", spp, per_pass);
         return Q_EXT_ERR_ARGUMENT;
     }
@@ -402,7 +412,7 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
     }
     if (probe_spp == 0) probe_spp = 1;
     if (target == Q_AUTOTUNE_BATCH) {
-        const uint32_t pp = batch_samples_per_pass(st);
+        const uint32_t pp = batch_samples_per_pass(st, probe_spp);
         probe_spp = std::max(pp, probe_spp - probe_spp % pp);
     }
     (void)hipSetDevice(st->device);
@@ -444,7 +454,8 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
     }
     // leave the pass graph of the chosen configuration captured, the
     // accumulations empty and the counters as they were before tuning
-    if (target == Q_AUTOTUNE_BATCH && !prepare_batch_pass(st, batch_samples_per_pass(st))) return Q_EXT_ERR_DEVICE;
+    if (target == Q_AUTOTUNE_BATCH && !prepare_batch_pass(st, batch_samples_per_pass(st, probe_spp)))
+        return Q_EXT_ERR_DEVICE;
     if (st->batch.active && !clear_batch(st, st->stream)) return Q_EXT_ERR_DEVICE;
     if (!clear_single(st, st->stream)) return Q_EXT_ERR_DEVICE;
     if (!hip_ok(st, hipStreamSynchronize(st->stream), "autotune")) return Q_EXT_ERR_DEVICE;

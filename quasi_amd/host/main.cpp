@@ -1,0 +1,245 @@
+// main.cpp -- headless host for libquasi_hip.so.
+//
+// The reference host (src/quasi/host/main.cpp:120-330) opens a window, loads
+// the Metal plugin through dynamic_library + loader, and in batch mode
+// (--render N) renders N frames, reads the AOVs back and writes a
+// timestamped EXR.  This host does the same without a window, on a HIP
+// context:
+//
+//   quasi_host [plugin.so] --render N [--out file.exr | --out-dir DIR]
+//              [--width W] [--height H] [--device K] [--scene file.qscene]
+//              [--spp-per-render S] [--depth D] [--rr-start R] [--seed X]
+//              [--integrator reference|nee] [--rng reference|hashed]
+//              [--engine megakernel|wavefront] [--compression zip|none]
+//              [--single-layer]
+//
+// Without a plugin path the library next to this executable's directory
+// (../libquasi_hip.so) is used.  The camera is the reference's orbit camera at
+// rest (main.cpp:38-45, 104-115), or the scene file's; it is marked dirty on
+// the first frame only, so the N frames accumulate.  The render parameters and
+// the scene go through the plugin's extension exports when given.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <string_view>
+
+#include "dynamic_library.hpp"
+#include "exr_writer.hpp"
+#include "hip_context.hpp"
+#include "loader.hpp"
+#include "quasi/quasi_hip_ext.h"
+#include "scene_file.hpp"
+
+namespace {
+
+void plugin_log(void* /*host*/, const char* msg) { std::printf("[Plugin] %s\n", msg); }
+void plugin_request_shutdown(void* /*host*/) {}
+
+struct Options {
+    std::filesystem::path plugin, out, out_dir = ".", scene;
+    int frames = 16;
+    uint32_t width = 720, height = 720;   // the reference window (main.cpp:155)
+    int device = 0;
+    long spp = -1, depth = -1, rr_start = -1, seed = -1, integrator = -1, rng = -1, engine = -1;
+    Q::io::exr_compression compression = Q::io::exr_compression::zip;
+    bool single_layer = false;
+};
+
+int usage(const char* argv0) {
+    std::fprintf(stderr,
+                 "usage: %s [plugin.so] --render N [--out file.exr | --out-dir DIR] [--width W] [--height H]\n"
+                 "       [--device K] [--scene file.qscene] [--spp-per-render S] [--depth D] [--rr-start R]\n"
+                 "       [--seed X] [--integrator reference|nee] [--rng reference|hashed]\n"
+                 "       [--engine megakernel|wavefront] [--compression zip|none] [--single-layer]\n",
+                 argv0);
+    return EXIT_FAILURE;
+}
+
+bool parse(int argc, char** argv, Options& o) {
+    for (int i = 1; i < argc; ++i) {
+        const std::string_view a = argv[i];
+        auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
+        const char* v = nullptr;
+        if (a == "--single-layer") { o.single_layer = true; continue; }
+        if (a.size() > 2 && a.substr(0, 2) == "--") {
+            v = next();
+            if (!v) return false;
+        }
+        if (a == "--render") o.frames = std::atoi(v);
+        else if (a == "--out") o.out = v;
+        else if (a == "--out-dir") o.out_dir = v;
+        else if (a == "--width") o.width = uint32_t(std::atol(v));
+        else if (a == "--height") o.height = uint32_t(std::atol(v));
+        else if (a == "--device") o.device = std::atoi(v);
+        else if (a == "--scene") o.scene = v;
+        else if (a == "--spp-per-render") o.spp = std::atol(v);
+        else if (a == "--depth") o.depth = std::atol(v);
+        else if (a == "--rr-start") o.rr_start = std::atol(v);
+        else if (a == "--seed") o.seed = std::atol(v);
+        else if (a == "--integrator") o.integrator = std::string_view(v) == "nee" ? Q_INTEGRATOR_NEE : Q_INTEGRATOR_REFERENCE;
+        else if (a == "--rng") o.rng = std::string_view(v) == "hashed" ? Q_RNG_HASHED : Q_RNG_REFERENCE;
+        else if (a == "--engine") o.engine = std::string_view(v) == "wavefront" ? Q_ENGINE_WAVEFRONT : Q_ENGINE_MEGAKERNEL;
+        else if (a == "--compression") o.compression = std::string_view(v) == "none" ? Q::io::exr_compression::none : Q::io::exr_compression::zip;
+        else if (a[0] != '-') o.plugin = std::string(a);
+        else return false;
+    }
+    return o.frames > 0 && o.width > 0 && o.height > 0;
+}
+
+std::filesystem::path default_plugin(const char* argv0) {
+    std::error_code ec;
+    auto self = std::filesystem::canonical("/proc/self/exe", ec);
+    if (ec) self = std::filesystem::absolute(argv0);
+    return self.parent_path().parent_path() / "libquasi_hip.so";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Options o;
+    if (!parse(argc, argv, o)) return usage(argv[0]);
+    if (o.plugin.empty()) o.plugin = default_plugin(argv[0]);
+
+    auto ctx_result = Q::gpu::hip::context::create(o.device, o.width, o.height);
+    if (!ctx_result) {
+        std::fprintf(stderr, "Failed to create HIP context: %s\n", Q::gpu::hip::to_string(ctx_result.error()));
+        return EXIT_FAILURE;
+    }
+    auto& hip = *ctx_result;
+
+    auto lib_result = Q::plugin::dynamic_library::open(o.plugin);
+    if (!lib_result) {
+        std::fprintf(stderr, "Failed to load plugin library: %s (%s)\n",
+                     std::string(Q::plugin::to_string(lib_result.error())).c_str(), o.plugin.c_str());
+        return EXIT_FAILURE;
+    }
+    auto& lib = *lib_result;
+
+    Q_plugin_context ctx{};
+    ctx.viewport_width = o.width;
+    ctx.viewport_height = o.height;
+    ctx.host_data = nullptr;
+    ctx.gpu = hip.gpu();
+    ctx.log = plugin_log;
+    ctx.request_shutdown = plugin_request_shutdown;
+    auto plugin_result = Q::plugin::loader::load(lib, &ctx);
+    if (!plugin_result) {
+        std::fprintf(stderr, "Failed to load plugin: %s\n", Q::plugin::to_string(plugin_result.error()));
+        return EXIT_FAILURE;
+    }
+    auto& plugin = *plugin_result;
+    const Q_plugin_info info = plugin.info();
+    std::printf("Loaded plugin: %s v%u.%u.%u\n", info.name, info.version.major, info.version.minor,
+                info.version.patch);
+    std::printf("Description: %s\n", info.description);
+
+    // camera: the reference orbit camera at rest, or the scene's
+    Q_camera cam{{0.0f, 1.0f, 3.5f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, 40.0f};
+    if (!o.scene.empty()) {
+        auto set_scene = lib.get_symbol<int32_t (*)(Q_plugin_handle*, const Q_scene_desc*)>("Q_plugin_set_scene");
+        if (!set_scene) {
+            std::fprintf(stderr, "The plugin has no Q_plugin_set_scene: cannot load %s\n", o.scene.c_str());
+            return EXIT_FAILURE;
+        }
+        auto sc = Q::io::read_scene(o.scene);
+        if (!sc) {
+            std::fprintf(stderr, "Scene %s: %s\n", o.scene.c_str(), Q::io::to_string(sc.error()));
+            return EXIT_FAILURE;
+        }
+        Q_scene_desc d{};
+        d.struct_size = sizeof d;
+        d.num_vertices = uint32_t(sc->vertices.size() / 3);
+        d.num_triangles = uint32_t(sc->indices.size() / 3);
+        d.vertices = sc->vertices.data();
+        d.indices = sc->indices.data();
+        d.albedo = sc->albedo.data();
+        d.emission = sc->emission.data();
+        if ((*set_scene)(plugin.handle(), &d) != Q_EXT_OK) {
+            std::fprintf(stderr, "Q_plugin_set_scene failed\n");
+            return EXIT_FAILURE;
+        }
+        std::memcpy(cam.position, sc->camera, 3 * sizeof(float));
+        std::memcpy(cam.target, sc->camera + 3, 3 * sizeof(float));
+        std::memcpy(cam.up, sc->camera + 6, 3 * sizeof(float));
+        cam.fov = sc->camera[9];
+        std::printf("[Host] Scene %s: %u triangles\n", o.scene.c_str(), d.num_triangles);
+    }
+    if (o.spp > 0 || o.depth > 0 || o.rr_start >= 0 || o.seed >= 0 || o.integrator >= 0 || o.rng >= 0 || o.engine >= 0) {
+        auto get = lib.get_symbol<int32_t (*)(Q_plugin_handle*, Q_render_params*)>("Q_plugin_get_params");
+        auto set = lib.get_symbol<int32_t (*)(Q_plugin_handle*, const Q_render_params*)>("Q_plugin_set_params");
+        if (!get || !set) {
+            std::fprintf(stderr, "The plugin has no render-parameter exports\n");
+            return EXIT_FAILURE;
+        }
+        Q_render_params p{};
+        p.struct_size = sizeof p;
+        if ((*get)(plugin.handle(), &p) != Q_EXT_OK) return EXIT_FAILURE;
+        if (o.spp > 0) p.spp_per_render = uint32_t(o.spp);
+        if (o.depth > 0) p.max_depth = uint32_t(o.depth);
+        if (o.rr_start >= 0) p.rr_start = uint32_t(o.rr_start);
+        if (o.seed >= 0) p.seed = uint32_t(o.seed);
+        if (o.integrator >= 0) p.integrator = uint32_t(o.integrator);
+        if (o.rng >= 0) p.rng_mode = uint32_t(o.rng);
+        if (o.engine >= 0) p.engine = uint32_t(o.engine);
+        if ((*set)(plugin.handle(), &p) != Q_EXT_OK) {
+            std::fprintf(stderr, "Q_plugin_set_params rejected the parameters\n");
+            return EXIT_FAILURE;
+        }
+    }
+
+    std::printf("[Host] Batch mode: rendering %d frames then saving EXR\n", o.frames);
+    auto last = std::chrono::steady_clock::now();
+    const auto t0 = last;
+    for (int i = 0; i < o.frames; ++i) {
+        const auto now = std::chrono::steady_clock::now();
+        plugin.update(std::chrono::duration<float>(now - last).count());
+        last = now;
+        auto fr = hip.begin_frame();
+        if (!fr) {
+            std::fprintf(stderr, "begin_frame failed: %s\n", Q::gpu::hip::to_string(fr.error()));
+            return EXIT_FAILURE;
+        }
+        Q_render_frame& frame = *fr;
+        frame.camera = cam;
+        frame.camera_dirty = i == 0 ? 1u : 0u;
+        plugin.render(&frame);
+        hip.end_frame(frame);
+    }
+    hip.finish();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("[Host] %d frames in %.3f s\n", o.frames, secs);
+
+    const auto path = o.out.empty() ? Q::io::make_timestamped_path(o.out_dir) : o.out;
+    int rc = EXIT_SUCCESS;
+    if (plugin.supports_readback_aov() && !o.single_layer) {
+        std::printf("[Host] Saving EXR with AOVs (%d samples)...\n", o.frames);
+        Q_readback_aov_result rb = plugin.readback_aov();
+        if (rb.buffers[Q_AOV_BEAUTY].data) {
+            auto w = Q::io::write_exr(path, rb, o.compression);
+            if (w) std::printf("[Host] Saved: %s\n", path.c_str());
+            else { std::fprintf(stderr, "[Host] EXR write failed: %s\n", Q::io::to_string(w.error())); rc = EXIT_FAILURE; }
+            plugin.readback_aov_free(&rb);
+        } else {
+            std::fprintf(stderr, "[Host] AOV readback returned no data\n");
+            rc = EXIT_FAILURE;
+        }
+    } else if (plugin.supports_readback()) {
+        std::printf("[Host] Saving EXR (%d samples)...\n", o.frames);
+        Q_readback_result rb = plugin.readback();
+        if (rb.data) {
+            auto w = Q::io::write_exr(path, rb, o.compression);
+            if (w) std::printf("[Host] Saved: %s\n", path.c_str());
+            else { std::fprintf(stderr, "[Host] EXR write failed: %s\n", Q::io::to_string(w.error())); rc = EXIT_FAILURE; }
+            plugin.readback_free(&rb);
+        } else {
+            std::fprintf(stderr, "[Host] Readback returned no data\n");
+            rc = EXIT_FAILURE;
+        }
+    } else {
+        std::printf("[Host] Plugin does not support HDR readback\n");
+    }
+    plugin.destroy();
+    return rc;
+}

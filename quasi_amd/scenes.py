@@ -297,3 +297,20 @@ def diff_sphere(frequency: int = 22, stage: str = "target", seed_target: int = 4
     b.quad((-0.9, 2.0, -0.1), (0.6, 0.0, 0.0), (0.0, 0.0, 0.6), (0.0, 0.0, 0.0), (12.0, 12.0, 12.0))
     b.quad((0.6, 1.6, -0.6), (0.4, 0.0, 0.0), (0.0, 0.0, 0.4), (0.0, 0.0, 0.0), (10.0, 8.0, 6.0))
     return b.mesh(DIFF_CAMERA, f"diff_sphere{frequency}_{stage}")
+
+
+def save_qscene(mesh: Mesh, path) -> None:
+    """Writes the host's scene file (little-endian): "QSCN", version 1,
+    vertex and triangle counts, camera (position, target, up, fov), then
+    vertices, indices, albedo and emission."""
+    import struct
+    c = mesh.camera
+    cam = list(c["position"]) + list(c["target"]) + list(c["up"]) + [c["fov"]]
+    with open(path, "wb") as f:
+        f.write(b"QSCN")
+        f.write(struct.pack("<3I", 1, mesh.vertices.shape[0], mesh.num_triangles))
+        f.write(np.asarray(cam, dtype="<f4").tobytes())
+        f.write(np.ascontiguousarray(mesh.vertices, dtype="<f4").tobytes())
+        f.write(np.ascontiguousarray(mesh.indices, dtype="<u4").tobytes())
+        f.write(np.ascontiguousarray(mesh.albedo, dtype="<f4").tobytes())
+        f.write(np.ascontiguousarray(mesh.emission, dtype="<f4").tobytes())

@@ -1,0 +1,66 @@
+"""The headless C++ host (quasi_amd/bin/quasi_host: HIP context, C++ plugin
+loader, EXR writer) against the Python harness on the same plugin: the EXR it
+writes holds the binary16 rounding of the plugin's float readback, bit for
+bit, and depth.Z the FP32 values themselves."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from quasi_amd import abi, scenes
+from quasi_amd.plugin import HipRenderer, make_camera
+from tests.conftest import ROOT
+from tests.exr_reader import read_exr
+
+pytestmark = pytest.mark.gpu
+
+HOST = ROOT / "quasi_amd" / "bin" / "quasi_host"
+
+
+def run_host(args):
+    print("[host]", " ".join(args), flush=True)
+    r = subprocess.run([str(HOST), *args], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    return r.stdout
+
+
+def check_exr(path, aov):
+    _, ch = read_exr(path)
+    for name, key, k in (("R", "beauty", 0), ("G", "beauty", 1), ("B", "beauty", 2), ("A", "beauty", 3),
+                         ("albedo.R", "albedo", 0), ("albedo.G", "albedo", 1), ("albedo.B", "albedo", 2),
+                         ("normal.X", "normal", 0), ("normal.Y", "normal", 1), ("normal.Z", "normal", 2)):
+        assert np.array_equal(ch[name].view(np.uint16), aov[key][..., k].astype(np.float16).view(np.uint16)), name
+    assert np.array_equal(ch["depth.Z"], aov["depth"][..., 0])
+
+
+def test_host_default_scene_matches_plugin(tmp_path):
+    w, h, frames = 64, 48, 8
+    out = tmp_path / "ref.exr"
+    log = run_host(["--render", str(frames), "--width", str(w), "--height", str(h), "--out", str(out)])
+    assert "Saved:" in log and "HIP Path Tracer" in log
+    with HipRenderer(w, h) as r:
+        r.render(frames)
+        aov = r.readback_aov()
+    check_exr(out, aov)
+
+
+@pytest.mark.parametrize("engine", ["megakernel", "wavefront"])
+def test_host_scene_file_nee(tmp_path, engine):
+    w = h = 96
+    mesh = scenes.cornell36(0)
+    sf = tmp_path / "c36.qscene"
+    scenes.save_qscene(mesh, sf)
+    out = tmp_path / "c36.exr"
+    run_host([str(ROOT / "quasi_amd" / "libquasi_hip.so"), "--render", "2", "--width", str(w), "--height", str(h),
+              "--scene", str(sf), "--spp-per-render", "4", "--depth", "4", "--integrator", "nee", "--rng", "hashed",
+              "--seed", "3", "--engine", engine, "--compression", "none", "--out", str(out)])
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        c = mesh.camera
+        r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+        r.set_params(spp_per_render=4, max_depth=4, rr_start=3, integrator=1, rng_mode=1, seed=3,
+                     engine=abi.Q_ENGINE_WAVEFRONT if engine == "wavefront" else abi.Q_ENGINE_MEGAKERNEL)
+        r.render(2)
+        aov = r.readback_aov()
+    check_exr(out, aov)
