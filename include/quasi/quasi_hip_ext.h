@@ -91,7 +91,16 @@ typedef struct Q_render_params {
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
     uint32_t queue_chunk;      /* wavefront: rays a wave takes per queue access, 64 or 256 (0 = 64);
                                   1 = each lane fetches its next ray as soon as it is idle */
+    uint32_t bvh_builder;      /* Q_BVH_BUILD_*: where Q_plugin_set_scene builds the BVH */
+    uint32_t reserved;
 } Q_render_params;
+
+/* Q_render_params.bvh_builder (the same binned-SAH algorithm either way; any
+ * valid tree renders the same image, because the closest hit does not
+ * depend on the tree) */
+#define Q_BVH_BUILD_AUTO 0u   /* on the GPU from 2^20 triangles, else on the host */
+#define Q_BVH_BUILD_HOST 1u   /* host threads */
+#define Q_BVH_BUILD_GPU  2u   /* GPU kernels (falls back to the host when the GPU build fails) */
 
 #define Q_STATS_MAX_KERNELS 16
 
@@ -118,6 +127,10 @@ typedef struct Q_render_stats {
     uint32_t num_kernels;
     uint32_t reserved;
     Q_kernel_stat kernels[Q_STATS_MAX_KERNELS];
+    uint32_t bvh_builder;      /* Q_BVH_BUILD_HOST or Q_BVH_BUILD_GPU: where the current BVH was built */
+    uint32_t reserved2;
+    double bvh_sah_cost;       /* SAH cost of the current BVH2 (traversal 1, triangle 1, root area 1) */
+    double bvh_device_ms;      /* GPU build: hipEvent time of the build kernels */
 } Q_render_stats;
 
 /* Replaces the hard-coded scene of plugin.mm:584 (cornell_box.hpp:95-134) with a

@@ -86,7 +86,11 @@ class Q_scene_desc(C.Structure):
 class Q_render_params(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "struct_size", "spp_per_render", "max_depth", "rr_start", "seed", "rng_mode", "integrator", "engine",
-        "bvh_width", "samples_per_pass", "block_size", "flags", "id_sample", "queue_chunk")]
+        "bvh_width", "samples_per_pass", "block_size", "flags", "id_sample", "queue_chunk", "bvh_builder",
+        "reserved")]
+
+
+Q_BVH_BUILD_AUTO, Q_BVH_BUILD_HOST, Q_BVH_BUILD_GPU = 0, 1, 2
 
 
 class Q_kernel_stat(C.Structure):
@@ -99,7 +103,8 @@ class Q_render_stats(C.Structure):
                 ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32), ("rays_camera", C.c_uint64),
                 ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64), ("bvh_build_ms", C.c_double),
                 ("render_ms", C.c_double), ("num_kernels", C.c_uint32), ("reserved", C.c_uint32),
-                ("kernels", Q_kernel_stat * Q_STATS_MAX_KERNELS)]
+                ("kernels", Q_kernel_stat * Q_STATS_MAX_KERNELS), ("bvh_builder", C.c_uint32),
+                ("reserved2", C.c_uint32), ("bvh_sah_cost", C.c_double), ("bvh_device_ms", C.c_double)]
 
 
 class Q_batch_scene(C.Structure):

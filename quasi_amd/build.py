@@ -37,6 +37,7 @@ PLUGIN_SOURCES = [
     ("kernels_wave.hip", DEVICE),
     ("kernels_util.hip", DEVICE),
     ("kernels_adjoint.hip", DEVICE),
+    ("bvh_gpu.hip", DEVICE),
     ("plugin.cpp", HOSTONLY),
     ("plugin_batch.cpp", HOSTONLY),
     ("plugin_adjoint.cpp", HOSTONLY),

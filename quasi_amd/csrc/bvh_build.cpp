@@ -242,22 +242,92 @@ struct Builder {
 
 BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint32_t ntri,
                          unsigned threads) {
-    BvhBuildResult res;
-    if (ntri == 0) return res;
+    if (ntri == 0) return BvhBuildResult{};
     if (threads == 0) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     Builder b;
     b.p0 = p0; b.p1 = p1; b.p2 = p2; b.ntri = ntri;
     b.prepare(threads);
     b.build(threads);
+    const uint32_t used = b.next_node.load();
+    std::vector<BinaryNode> tree(used);
+    for (uint32_t i = 0; i < used; ++i) {
+        const TmpNode& t = b.nodes[i];
+        BinaryNode& o = tree[i];
+        for (int k = 0; k < 3; ++k) { o.lo[k] = t.box.lo[k]; o.hi[k] = t.box.hi[k]; }
+        o.left = t.left;
+        o.right = t.right;
+        o.first = t.first;
+        o.count = t.count;
+    }
+    return convert_binary_tree(tree.data(), used, b.prims.data(), ntri);
+}
+
+bool validate_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, const uint32_t* prims, uint32_t ntri) {
+    if (num_nodes == 0 || ntri == 0) return false;
+    std::vector<uint8_t> seen_node(num_nodes, 0), seen_pos(ntri, 0), seen_prim(ntri, 0);
+    std::vector<uint32_t> st{0};
+    uint32_t visited = 0;
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        if (i >= num_nodes || seen_node[i]) return false;
+        seen_node[i] = 1;
+        ++visited;
+        const BinaryNode& n = nodes[i];
+        if (n.count) {
+            if (n.count > uint32_t(kLeafMax) || n.first >= ntri || n.count > ntri - n.first) return false;
+            for (uint32_t k = n.first; k < n.first + n.count; ++k) {
+                if (seen_pos[k]) return false;
+                seen_pos[k] = 1;
+            }
+        } else {
+            st.push_back(n.right);
+            st.push_back(n.left);
+        }
+    }
+    if (visited != num_nodes) return false;
+    for (uint32_t k = 0; k < ntri; ++k) {
+        if (!seen_pos[k] || prims[k] >= ntri || seen_prim[prims[k]]) return false;
+        seen_prim[prims[k]] = 1;
+    }
+    return true;
+}
+
+namespace {
+
+Box box_of(const BinaryNode& n) {
+    Box b;
+    for (int k = 0; k < 3; ++k) { b.lo[k] = n.lo[k]; b.hi[k] = n.hi[k]; }
+    return b;
+}
+
+}  // namespace
+
+BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes, const uint32_t* prims,
+                                   uint32_t ntri) {
+    BvhBuildResult res;
+    if (ntri == 0 || num_nodes == 0) return res;
+    struct Tmp {
+        Box box;
+        uint32_t left, right, first, count;
+    };
+    std::vector<Tmp> nodes(num_nodes);
+    for (uint32_t i = 0; i < num_nodes; ++i)
+        nodes[i] = Tmp{box_of(tnodes[i]), tnodes[i].left, tnodes[i].right, tnodes[i].first, tnodes[i].count};
+    struct {
+        const std::vector<Tmp>& nodes;
+        const uint32_t* prims;
+        uint32_t next_count;
+    } b{nodes, prims, num_nodes};
 
     // Depth-first conversion to pair nodes.
-    const TmpNode& root = b.nodes[0];
+    const Tmp& root = b.nodes[0];
     res.order.reserve(ntri);
     std::vector<float>& out = res.nodes;
     // leaf reference of every temporary leaf node, shared by both layouts
-    std::vector<int32_t> leaf_ref(b.next_node.load(), 0);
+    std::vector<int32_t> leaf_ref(b.next_count, 0);
     auto encode_leaf = [&](uint32_t tmp) -> int32_t {
-        const TmpNode& lf = b.nodes[tmp];
+        const Tmp& lf = b.nodes[tmp];
         uint32_t first = uint32_t(res.order.size());
         for (uint32_t i = 0; i < lf.count; ++i) res.order.push_back(b.prims[lf.first + i]);
         res.num_leaves++;
@@ -328,11 +398,11 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
             int32_t c = int32_t(idx);
             std::memcpy(&out[16 * size_t(it.parent) + 12 + it.side], &c, 4);
         }
-        const TmpNode& nd = b.nodes[it.tmp];
+        const Tmp& nd = b.nodes[it.tmp];
         res.depth = std::max(res.depth, it.depth);
         cost += nd.box.area() / root_area;
-        const TmpNode& L = b.nodes[nd.left];
-        const TmpNode& R = b.nodes[nd.right];
+        const Tmp& L = b.nodes[nd.left];
+        const Tmp& R = b.nodes[nd.right];
         int32_t cl = 0, cr = 0;
         if (L.count) { cl = encode_leaf(nd.left); cost += L.box.area() / root_area * L.count; }
         if (R.count) { cr = encode_leaf(nd.right); cost += R.box.area() / root_area * R.count; }
@@ -367,11 +437,11 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
             int pick = -1;
             double best_area = -1.0;
             for (uint32_t k = 0; k < nk; ++k) {
-                const TmpNode& c = b.nodes[kids[k]];
+                const Tmp& c = b.nodes[kids[k]];
                 if (c.count == 0 && c.box.area() > best_area) { best_area = c.box.area(); pick = int(k); }
             }
             if (pick < 0) break;
-            const TmpNode& c = b.nodes[kids[pick]];
+            const Tmp& c = b.nodes[kids[pick]];
             for (uint32_t k = nk; k > uint32_t(pick) + 1; --k) kids[k] = kids[k - 1];
             kids[pick + 1] = c.right;
             kids[pick] = c.left;
@@ -381,7 +451,7 @@ BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint
         for (uint32_t k = 0; k < 4; ++k) {
             int32_t ref = kEmptyChild;
             if (k < nk) {
-                const TmpNode& c = b.nodes[kids[k]];
+                const Tmp& c = b.nodes[kids[k]];
                 for (int a = 0; a < 3; ++a) { n[8 * a + k] = c.box.lo[a]; n[8 * a + 4 + k] = c.box.hi[a]; }
                 ref = c.count ? leaf_ref[kids[k]] : 0;
             }

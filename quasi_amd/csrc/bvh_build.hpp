@@ -29,4 +29,26 @@ struct BvhBuildResult {
 BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint32_t ntri,
                          unsigned threads);
 
+// A node of a binary build tree: internal (count == 0: children left, right)
+// or leaf (count 1..kLeafMax triangles prims[first .. first + count)).  Node 0
+// is the root.
+struct BinaryNode {
+    float lo[3], hi[3];
+    uint32_t left, right;
+    uint32_t first, count;
+};
+
+// The pair-node and 4-wide layouts of a binary build tree (shared by the host
+// build and the GPU build); prims maps the tree's triangle positions to input
+// triangles.  The output is independent of how the tree's node slots were
+// numbered.
+BvhBuildResult convert_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, const uint32_t* prims,
+                                   uint32_t ntri);
+
+// True when the tree rooted at node 0 reaches every one of its num_nodes
+// nodes exactly once, internal children are in range, leaves hold 1..kLeafMax
+// triangles, the leaves cover tree positions 0..ntri-1 exactly once and prims
+// is a permutation of 0..ntri-1.
+bool validate_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, const uint32_t* prims, uint32_t ntri);
+
 }  // namespace quasi_hip

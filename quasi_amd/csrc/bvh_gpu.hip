@@ -1,0 +1,738 @@
+// bvh_gpu.hip -- binned-SAH BVH build on the GPU.
+//
+// Same algorithm as the host build (bvh_build.cpp): per node, 32 centroid
+// bins on each axis, the cheapest of the 31 x 3 candidate planes by the SAH
+// (traversal cost 1, triangle cost 1), at most kLeafMax triangles per leaf, a
+// median split when every centroid coincides.  The output is a binary tree of
+// BinaryNode plus the triangle order, converted to the traversal layouts by
+// convert_binary_tree on the host.
+//
+// Two phases:
+//  - large nodes (more than kLargeNode triangles) are split one tree level at
+//    a time with many blocks per node: bounds (block reductions + one atomic
+//    per block), binning (LDS bins merged with global atomics), split choice
+//    (one thread per node), partition (block-level compaction, one atomic per
+//    block and side into the other id buffer);
+//  - each remaining subtree is built by one wave: bounds by wave reductions,
+//    bins in LDS, stable partition with ballots, the smaller child first so
+//    the LDS task stack stays within log2 of the subtree size.
+// Every index a kernel derives is checked against its buffer; a violation
+// sets an error flag and the host falls back to the host build.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bvh_build.hpp"
+#include "common.hpp"
+
+namespace quasi_hip {
+
+namespace {
+
+constexpr int kBins = kSahBins;             // 32
+constexpr uint32_t kLargeNode = 4096;       // larger nodes take the level-synchronous path
+constexpr uint32_t kChunk = 8192;           // triangles per block in the large-node kernels
+constexpr int kBlock = 256;
+constexpr int kStack = 48;                  // subtree task stack (smaller child first: <= log2(4096) + 1 used)
+constexpr uint32_t kIntMax = 0x7FFFFFFF;
+
+// Error flags
+constexpr uint32_t kErrRange = 1, kErrNodes = 2, kErrStack = 4;
+
+__device__ __forceinline__ int fkey(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float funkey(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7FFFFFFF); }
+
+__device__ __forceinline__ float bin_scale(float lo, float hi) {
+    const float ext = hi - lo;
+    const float s = ext > 0.f ? float(kBins) / ext : 0.f;
+    return s < INFINITY ? s : 0.f;   // a subnormal extent: no split on this axis
+}
+__device__ __forceinline__ int bin_of(float c, float lo, float scale) {
+    int b = int((c - lo) * scale);
+    return min(max(b, 0), kBins - 1);
+}
+
+__device__ __forceinline__ float area_of(float lx, float ly, float lz, float hx, float hy, float hz) {
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    if (!(dx >= 0.f) || !(dy >= 0.f) || !(dz >= 0.f)) return 0.f;
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+
+// One large node of the current level.
+struct LargeTask {
+    uint32_t node, begin, count, buf;   // buf: which id buffer holds the range
+    int box_lo[3], box_hi[3];           // fkey bounds of the triangles
+    int cen_lo[3], cen_hi[3];           // fkey bounds of the centroids
+    int axis, split;                    // decision (axis -1: median split)
+    uint32_t nl;                        // triangles going left
+    uint32_t lcur, rcur;                // partition cursors
+    uint32_t pad[3];
+};
+
+// A node handed to the subtree kernel (or a child of a large node).
+struct NodeTask {
+    uint32_t node, begin, count, buf;
+};
+
+struct Bins {
+    uint32_t cnt[3][kBins];
+    int lo[3][kBins][3];
+    int hi[3][kBins][3];
+};
+
+struct WorkItem {
+    uint32_t task, start, end;   // prim positions [start, end) of task
+};
+
+__global__ void k_prim_bounds(const float* __restrict__ p0, const float* __restrict__ p1,
+                              const float* __restrict__ p2, uint32_t n, float4* __restrict__ plo,
+                              float4* __restrict__ phi, uint32_t* __restrict__ ids) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            const float a = p0[3 * size_t(t) + k], b = p1[3 * size_t(t) + k], c = p2[3 * size_t(t) + k];
+            lo[k] = fminf(a, fminf(b, c));
+            hi[k] = fmaxf(a, fmaxf(b, c));
+        }
+        plo[t] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        phi[t] = make_float4(hi[0], hi[1], hi[2], 0.f);
+        ids[t] = t;
+    }
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------- large nodes
+__global__ __launch_bounds__(kBlock) void k_large_bounds(LargeTask* tasks, const WorkItem* items, uint32_t nitems,
+                                                         const float4* __restrict__ plo,
+                                                         const float4* __restrict__ phi,
+                                                         const uint32_t* __restrict__ ids_a,
+                                                         const uint32_t* __restrict__ ids_b, uint32_t n,
+                                                         uint32_t* err) {
+    __shared__ int red[12][kBlock / 64];
+    if (blockIdx.x >= nitems) return;
+    const WorkItem it = items[blockIdx.x];
+    LargeTask& t = tasks[it.task];
+    const uint32_t* ids = t.buf ? ids_b : ids_a;
+    int v[12];
+    for (int k = 0; k < 3; ++k) {
+        v[k] = kIntMax; v[3 + k] = -int(kIntMax) - 1; v[6 + k] = kIntMax; v[9 + k] = -int(kIntMax) - 1;
+    }
+    if (it.end > n || it.start > it.end) {
+        if (threadIdx.x == 0) atomicOr(err, kErrRange);
+        return;
+    }
+    for (uint32_t i = it.start + threadIdx.x; i < it.end; i += kBlock) {
+        const uint32_t p = ids[i];
+        if (p >= n) { atomicOr(err, kErrRange); continue; }
+        const float4 l = plo[p], h = phi[p];
+        const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+        for (int k = 0; k < 3; ++k) {
+            const float c = 0.5f * (lo[k] + hi[k]);
+            v[k] = min(v[k], fkey(lo[k]));
+            v[3 + k] = max(v[3 + k], fkey(hi[k]));
+            v[6 + k] = min(v[6 + k], fkey(c));
+            v[9 + k] = max(v[9 + k], fkey(c));
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int k = 0; k < 12; ++k) {
+        const int r = (k < 3 || (k >= 6 && k < 9)) ? wave_min_i(v[k]) : wave_max_i(v[k]);
+        if (lane == 0) red[k][wave] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        const bool is_min = k < 3 || (k >= 6 && k < 9);
+        int r = red[k][0];
+        for (int w = 1; w < kBlock / 64; ++w) r = is_min ? min(r, red[k][w]) : max(r, red[k][w]);
+        int* dst = k < 3 ? &t.box_lo[k] : k < 6 ? &t.box_hi[k - 3] : k < 9 ? &t.cen_lo[k - 6] : &t.cen_hi[k - 9];
+        if (is_min) atomicMin(dst, r); else atomicMax(dst, r);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_large_bin(const LargeTask* tasks, const WorkItem* items, uint32_t nitems,
+                                                      Bins* bins, const float4* __restrict__ plo,
+                                                      const float4* __restrict__ phi,
+                                                      const uint32_t* __restrict__ ids_a,
+                                                      const uint32_t* __restrict__ ids_b, uint32_t n) {
+    __shared__ Bins lb;
+    if (blockIdx.x >= nitems) return;
+    const WorkItem it = items[blockIdx.x];
+    const LargeTask& t = tasks[it.task];
+    const uint32_t* ids = t.buf ? ids_b : ids_a;
+    uint32_t* lw = reinterpret_cast<uint32_t*>(&lb);
+    for (int i = threadIdx.x; i < 3 * kBins; i += kBlock) lw[i] = 0u;
+    int* llo = &lb.lo[0][0][0];
+    int* lhi = &lb.hi[0][0][0];
+    for (int i = threadIdx.x; i < 3 * kBins * 3; i += kBlock) { llo[i] = kIntMax; lhi[i] = -int(kIntMax) - 1; }
+    __syncthreads();
+    float clo[3], sc[3];
+    for (int k = 0; k < 3; ++k) {
+        clo[k] = funkey(t.cen_lo[k]);
+        sc[k] = bin_scale(clo[k], funkey(t.cen_hi[k]));
+    }
+    if (it.end <= n && it.start <= it.end) {
+        for (uint32_t i = it.start + threadIdx.x; i < it.end; i += kBlock) {
+            const uint32_t p = ids[i];
+            if (p >= n) continue;
+            const float4 l = plo[p], h = phi[p];
+            const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+            for (int k = 0; k < 3; ++k) {
+                if (sc[k] == 0.f) continue;
+                const int b = bin_of(0.5f * (lo[k] + hi[k]), clo[k], sc[k]);
+                atomicAdd(&lb.cnt[k][b], 1u);
+                for (int a = 0; a < 3; ++a) {
+                    atomicMin(&lb.lo[k][b][a], fkey(lo[a]));
+                    atomicMax(&lb.hi[k][b][a], fkey(hi[a]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    Bins& g = bins[it.task];
+    for (int i = threadIdx.x; i < 3 * kBins; i += kBlock) {
+        const int k = i / kBins, b = i % kBins;
+        const uint32_t c = lb.cnt[k][b];
+        if (c == 0) continue;
+        atomicAdd(&g.cnt[k][b], c);
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&g.lo[k][b][a], lb.lo[k][b][a]);
+            atomicMax(&g.hi[k][b][a], lb.hi[k][b][a]);
+        }
+    }
+}
+
+// One thread per large node: the SAH split, the node's box and its two children.
+__global__ void k_large_split(LargeTask* tasks, uint32_t ntasks, const Bins* bins, BinaryNode* nodes,
+                              uint32_t node_cap, uint32_t* node_counter, NodeTask* children, uint32_t* err) {
+    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= ntasks) return;
+    LargeTask& t = tasks[ti];
+    float blo[3], bhi[3], clo[3], chi[3], sc[3];
+    for (int k = 0; k < 3; ++k) {
+        blo[k] = funkey(t.box_lo[k]);
+        bhi[k] = funkey(t.box_hi[k]);
+        clo[k] = funkey(t.cen_lo[k]);
+        chi[k] = funkey(t.cen_hi[k]);
+        sc[k] = bin_scale(clo[k], chi[k]);
+    }
+    const Bins& g = bins[ti];
+    const float parent = fmaxf(area_of(blo[0], blo[1], blo[2], bhi[0], bhi[1], bhi[2]), 1e-30f);
+    int best_axis = -1, best_split = -1;
+    float best_cost = INFINITY;
+    uint32_t best_nl = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (sc[k] == 0.f) continue;
+        float rarea[kBins];
+        uint32_t rcnt[kBins];
+        float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t run = 0;
+        for (int b = kBins - 1; b >= 1; --b) {
+            if (g.cnt[k][b]) {
+                for (int a = 0; a < 3; ++a) {
+                    rl[a] = fminf(rl[a], funkey(g.lo[k][b][a]));
+                    rh[a] = fmaxf(rh[a], funkey(g.hi[k][b][a]));
+                }
+            }
+            run += g.cnt[k][b];
+            rarea[b] = area_of(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]);
+            rcnt[b] = run;
+        }
+        float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t lrun = 0;
+        for (int s = 1; s < kBins; ++s) {
+            if (g.cnt[k][s - 1]) {
+                for (int a = 0; a < 3; ++a) {
+                    ll[a] = fminf(ll[a], funkey(g.lo[k][s - 1][a]));
+                    lh[a] = fmaxf(lh[a], funkey(g.hi[k][s - 1][a]));
+                }
+            }
+            lrun += g.cnt[k][s - 1];
+            if (lrun == 0 || rcnt[s] == 0) continue;
+            const float cost =
+                1.f + (area_of(ll[0], ll[1], ll[2], lh[0], lh[1], lh[2]) * float(lrun) + rarea[s] * float(rcnt[s])) /
+                          parent;
+            if (cost < best_cost) { best_cost = cost; best_axis = k; best_split = s; best_nl = lrun; }
+        }
+    }
+    const uint32_t nl = best_axis >= 0 ? best_nl : t.count / 2;
+    t.axis = best_axis;
+    t.split = best_split;
+    t.nl = nl;
+    t.lcur = 0;
+    t.rcur = 0;
+    const uint32_t l = atomicAdd(node_counter, 2u);
+    if (t.node >= node_cap || l + 1 >= node_cap) {
+        atomicOr(err, kErrNodes);
+        children[2 * ti] = NodeTask{0, 0, 0, 0};
+        children[2 * ti + 1] = NodeTask{0, 0, 0, 0};
+        return;
+    }
+    BinaryNode& nd = nodes[t.node];
+    for (int k = 0; k < 3; ++k) { nd.lo[k] = blo[k]; nd.hi[k] = bhi[k]; }
+    nd.left = l;
+    nd.right = l + 1;
+    nd.first = 0;
+    nd.count = 0;
+    children[2 * ti] = NodeTask{l, t.begin, nl, t.buf ^ 1u};
+    children[2 * ti + 1] = NodeTask{l + 1, t.begin + nl, t.count - nl, t.buf ^ 1u};
+}
+
+__global__ __launch_bounds__(kBlock) void k_large_partition(LargeTask* tasks, const WorkItem* items, uint32_t nitems,
+                                                            const float4* __restrict__ plo,
+                                                            const float4* __restrict__ phi, uint32_t* ids_a,
+                                                            uint32_t* ids_b, uint32_t n, uint32_t* err) {
+    __shared__ uint32_t wl[kBlock / 64], wr[kBlock / 64];
+    __shared__ uint32_t base_l, base_r;
+    if (blockIdx.x >= nitems) return;
+    const WorkItem it = items[blockIdx.x];
+    LargeTask& t = tasks[it.task];
+    const uint32_t* in = t.buf ? ids_b : ids_a;
+    uint32_t* out = t.buf ? ids_a : ids_b;
+    if (it.end > n || it.start > it.end || t.begin + t.count > n) {
+        if (threadIdx.x == 0) atomicOr(err, kErrRange);
+        return;
+    }
+    const int axis = t.axis;
+    float clo = 0.f, sc = 0.f;
+    if (axis >= 0) {
+        clo = funkey(t.cen_lo[axis]);
+        sc = bin_scale(clo, funkey(t.cen_hi[axis]));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t r0 = it.start; r0 < it.end; r0 += kBlock) {
+        const uint32_t i = r0 + threadIdx.x;
+        const bool valid = i < it.end;
+        uint32_t p = 0;
+        bool left = false;
+        if (valid) {
+            p = in[i];
+            if (axis < 0) {
+                left = (i - t.begin) < t.nl;
+            } else if (p < n) {
+                const float4 l = plo[p], h = phi[p];
+                const float c = axis == 0 ? 0.5f * (l.x + h.x) : axis == 1 ? 0.5f * (l.y + h.y) : 0.5f * (l.z + h.z);
+                left = bin_of(c, clo, sc) < t.split;
+            }
+        }
+        const unsigned long long bl = __ballot(valid && left);
+        const unsigned long long br = __ballot(valid && !left);
+        if (lane == 0) { wl[wave] = __popcll(bl); wr[wave] = __popcll(br); }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t sl = 0, sr = 0;
+            for (int w = 0; w < kBlock / 64; ++w) { sl += wl[w]; sr += wr[w]; }
+            base_l = atomicAdd(&t.lcur, sl);
+            base_r = atomicAdd(&t.rcur, sr);
+        }
+        __syncthreads();
+        uint32_t off_l = base_l, off_r = base_r;
+        for (int w = 0; w < wave; ++w) { off_l += wl[w]; off_r += wr[w]; }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (valid) {
+            const uint32_t dst = left ? t.begin + off_l + __popcll(bl & below)
+                                      : t.begin + t.nl + off_r + __popcll(br & below);
+            if (dst < t.begin + t.count) out[dst] = p;
+            else atomicOr(err, kErrRange);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- subtrees (one wave each)
+struct SubTask {
+    uint32_t node, begin, count, buf;
+};
+
+__global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t ntasks,
+                                                const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                                uint32_t* ids_a, uint32_t* ids_b, uint32_t* order, uint32_t n,
+                                                BinaryNode* nodes, uint32_t node_cap, uint32_t* node_counter,
+                                                uint32_t* err) {
+    __shared__ Bins lb;
+    __shared__ SubTask stack[kStack];
+    __shared__ float cost_s[3];
+    __shared__ int split_s[3];
+    __shared__ uint32_t nl_s[3];
+    const int lane = threadIdx.x;
+    for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+        const NodeTask root = tasks[ti];
+        if (root.count == 0) continue;
+        int sp = 0;
+        SubTask cur{root.node, root.begin, root.count, root.buf};
+        bool have = true;
+        while (have) {
+            // guard every range before touching memory (uniform across the wave)
+            if (cur.begin + cur.count > n || cur.begin + cur.count < cur.begin || cur.node >= node_cap) {
+                if (lane == 0) atomicOr(err, kErrRange);
+                break;
+            }
+            const uint32_t* in = cur.buf ? ids_b : ids_a;
+            uint32_t* out = cur.buf ? ids_a : ids_b;
+            // bounds
+            int v[12];
+            for (int k = 0; k < 3; ++k) {
+                v[k] = kIntMax; v[3 + k] = -int(kIntMax) - 1; v[6 + k] = kIntMax; v[9 + k] = -int(kIntMax) - 1;
+            }
+            for (uint32_t i = cur.begin + lane; i < cur.begin + cur.count; i += 64) {
+                const uint32_t p = in[i];
+                if (p >= n) continue;
+                const float4 l = plo[p], h = phi[p];
+                const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+                for (int k = 0; k < 3; ++k) {
+                    const float c = 0.5f * (lo[k] + hi[k]);
+                    v[k] = min(v[k], fkey(lo[k]));
+                    v[3 + k] = max(v[3 + k], fkey(hi[k]));
+                    v[6 + k] = min(v[6 + k], fkey(c));
+                    v[9 + k] = max(v[9 + k], fkey(c));
+                }
+            }
+            for (int k = 0; k < 12; ++k)
+                v[k] = (k < 3 || (k >= 6 && k < 9)) ? wave_min_i(v[k]) : wave_max_i(v[k]);
+            float blo[3], bhi[3], clo[3], sc[3];
+            for (int k = 0; k < 3; ++k) {
+                blo[k] = funkey(v[k]);
+                bhi[k] = funkey(v[3 + k]);
+                clo[k] = funkey(v[6 + k]);
+                sc[k] = bin_scale(clo[k], funkey(v[9 + k]));
+            }
+            const uint32_t cnt = cur.count;
+            int best_axis = -1, best_split = -1;
+            float best_cost = INFINITY;
+            uint32_t best_nl = 0;
+            if (cnt > 1) {
+                uint32_t* lw = &lb.cnt[0][0];
+                for (int i = lane; i < 3 * kBins; i += 64) lw[i] = 0u;
+                int* llo = &lb.lo[0][0][0];
+                int* lhi = &lb.hi[0][0][0];
+                for (int i = lane; i < 3 * kBins * 3; i += 64) { llo[i] = kIntMax; lhi[i] = -int(kIntMax) - 1; }
+                __syncthreads();
+                for (uint32_t i = cur.begin + lane; i < cur.begin + cnt; i += 64) {
+                    const uint32_t p = in[i];
+                    if (p >= n) continue;
+                    const float4 l = plo[p], h = phi[p];
+                    const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+                    for (int k = 0; k < 3; ++k) {
+                        if (sc[k] == 0.f) continue;
+                        const int b = bin_of(0.5f * (lo[k] + hi[k]), clo[k], sc[k]);
+                        atomicAdd(&lb.cnt[k][b], 1u);
+                        for (int a = 0; a < 3; ++a) {
+                            atomicMin(&lb.lo[k][b][a], fkey(lo[a]));
+                            atomicMax(&lb.hi[k][b][a], fkey(hi[a]));
+                        }
+                    }
+                }
+                __syncthreads();
+                // lanes 0..2: the sweep over one axis each
+                if (lane < 3) {
+                    const int k = lane;
+                    float bc = INFINITY;
+                    int bs = -1;
+                    uint32_t bnl = 0;
+                    if (sc[k] != 0.f) {
+                        const float parent = fmaxf(area_of(blo[0], blo[1], blo[2], bhi[0], bhi[1], bhi[2]), 1e-30f);
+                        float rarea[kBins];
+                        uint32_t rcnt[kBins];
+                        float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                        uint32_t run = 0;
+                        for (int b = kBins - 1; b >= 1; --b) {
+                            if (lb.cnt[k][b]) {
+                                for (int a = 0; a < 3; ++a) {
+                                    rl[a] = fminf(rl[a], funkey(lb.lo[k][b][a]));
+                                    rh[a] = fmaxf(rh[a], funkey(lb.hi[k][b][a]));
+                                }
+                            }
+                            run += lb.cnt[k][b];
+                            rarea[b] = area_of(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]);
+                            rcnt[b] = run;
+                        }
+                        float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                        uint32_t lrun = 0;
+                        for (int s = 1; s < kBins; ++s) {
+                            if (lb.cnt[k][s - 1]) {
+                                for (int a = 0; a < 3; ++a) {
+                                    ll[a] = fminf(ll[a], funkey(lb.lo[k][s - 1][a]));
+                                    lh[a] = fmaxf(lh[a], funkey(lb.hi[k][s - 1][a]));
+                                }
+                            }
+                            lrun += lb.cnt[k][s - 1];
+                            if (lrun == 0 || rcnt[s] == 0) continue;
+                            const float c = 1.f + (area_of(ll[0], ll[1], ll[2], lh[0], lh[1], lh[2]) * float(lrun) +
+                                                   rarea[s] * float(rcnt[s])) / parent;
+                            if (c < bc) { bc = c; bs = s; bnl = lrun; }
+                        }
+                    }
+                    cost_s[k] = bc;
+                    split_s[k] = bs;
+                    nl_s[k] = bnl;
+                }
+                __syncthreads();
+                for (int k = 0; k < 3; ++k)
+                    if (split_s[k] >= 0 && cost_s[k] < best_cost) {
+                        best_cost = cost_s[k]; best_axis = k; best_split = split_s[k]; best_nl = nl_s[k];
+                    }
+                __syncthreads();
+            }
+            const bool leaf = cnt <= 1u || (cnt <= uint32_t(kLeafMax) && (best_axis < 0 || float(cnt) <= best_cost));
+            BinaryNode& nd = nodes[cur.node];
+            if (leaf) {
+                if (lane == 0) {
+                    for (int k = 0; k < 3; ++k) { nd.lo[k] = blo[k]; nd.hi[k] = bhi[k]; }
+                    nd.left = 0;
+                    nd.right = 0;
+                    nd.first = cur.begin;
+                    nd.count = cnt;
+                }
+                for (uint32_t i = cur.begin + lane; i < cur.begin + cnt; i += 64) order[i] = in[i];
+            } else {
+                const uint32_t nl = best_axis >= 0 ? best_nl : cnt / 2;
+                // stable partition into the other buffer
+                uint32_t lc = 0, rc = 0;
+                const float pclo = best_axis >= 0 ? clo[best_axis] : 0.f;
+                const float psc = best_axis >= 0 ? sc[best_axis] : 0.f;
+                const unsigned long long below = (1ull << lane) - 1ull;
+                for (uint32_t r0 = cur.begin; r0 < cur.begin + cnt; r0 += 64) {
+                    const uint32_t i = r0 + lane;
+                    const bool valid = i < cur.begin + cnt;
+                    uint32_t p = 0;
+                    bool left = false;
+                    if (valid) {
+                        p = in[i];
+                        if (best_axis < 0) {
+                            left = (i - cur.begin) < nl;
+                        } else if (p < n) {
+                            const float4 l = plo[p], h = phi[p];
+                            const float c = best_axis == 0 ? 0.5f * (l.x + h.x)
+                                          : best_axis == 1 ? 0.5f * (l.y + h.y) : 0.5f * (l.z + h.z);
+                            left = bin_of(c, pclo, psc) < best_split;
+                        }
+                    }
+                    const unsigned long long bl = __ballot(valid && left);
+                    const unsigned long long br = __ballot(valid && !left);
+                    if (valid) {
+                        const uint32_t dst = left ? cur.begin + lc + __popcll(bl & below)
+                                                  : cur.begin + nl + rc + __popcll(br & below);
+                        if (dst < cur.begin + cnt) out[dst] = p;
+                        else atomicOr(err, kErrRange);
+                    }
+                    lc += __popcll(bl);
+                    rc += __popcll(br);
+                }
+                uint32_t l = 0;
+                if (lane == 0) l = atomicAdd(node_counter, 2u);
+                l = __shfl(l, 0, 64);
+                if (l + 1 >= node_cap) {
+                    if (lane == 0) atomicOr(err, kErrNodes);
+                    break;
+                }
+                if (lane == 0) {
+                    for (int k = 0; k < 3; ++k) { nd.lo[k] = blo[k]; nd.hi[k] = bhi[k]; }
+                    nd.left = l;
+                    nd.right = l + 1;
+                    nd.first = 0;
+                    nd.count = 0;
+                }
+                const SubTask a{l, cur.begin, nl, cur.buf ^ 1u};
+                const SubTask b{l + 1, cur.begin + nl, cnt - nl, cur.buf ^ 1u};
+                // continue with the smaller child, keep the larger
+                const bool a_small = a.count <= b.count;
+                if (sp >= kStack) {
+                    if (lane == 0) atomicOr(err, kErrStack);
+                    break;
+                }
+                if (lane == 0) stack[sp] = a_small ? b : a;
+                ++sp;
+                cur = a_small ? a : b;
+                __syncthreads();
+                continue;
+            }
+            if (sp == 0) {
+                have = false;
+            } else {
+                --sp;
+                __syncthreads();
+                cur = stack[sp];
+                __syncthreads();
+            }
+        }
+    }
+}
+
+__global__ void k_fill_u32_bvh(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_init_bins(Bins* bins, uint32_t ntasks) {
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        Bins& g = bins[t];
+        uint32_t* c = &g.cnt[0][0];
+        int* lo = &g.lo[0][0][0];
+        int* hi = &g.hi[0][0][0];
+        for (int i = threadIdx.x; i < 3 * kBins; i += blockDim.x) c[i] = 0u;
+        for (int i = threadIdx.x; i < 3 * kBins * 3; i += blockDim.x) { lo[i] = kIntMax; hi[i] = -int(kIntMax) - 1; }
+    }
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) { p = nullptr; return false; }
+        n = count;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+#define BVH_CHECK(x)                     \
+    do {                                 \
+        if ((x) != hipSuccess) return false; \
+    } while (0)
+
+}  // namespace
+
+// Builds on the current device and stream; false on any failure (the caller
+// then uses the host build).  timing_ms receives the device build time.
+bool build_bvh_gpu(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
+                   BvhBuildResult& out, double* build_ms, const char** why) {
+    *why = "";
+    if (ntri < 8 || ntri > (1u << 29)) { *why = "size"; return false; }
+    const uint32_t n = ntri;
+    const uint32_t cap = 2 * n;
+    DevBuf<float> dp[3];
+    DevBuf<float4> plo, phi;
+    DevBuf<uint32_t> ids_a, ids_b, order, counter, err;
+    DevBuf<BinaryNode> nodes;
+    if (!dp[0].alloc(3 * size_t(n)) || !dp[1].alloc(3 * size_t(n)) || !dp[2].alloc(3 * size_t(n)) ||
+        !plo.alloc(n) || !phi.alloc(n) || !ids_a.alloc(n) || !ids_b.alloc(n) || !order.alloc(n) ||
+        !counter.alloc(1) || !err.alloc(1) || !nodes.alloc(cap)) {
+        *why = "allocation";
+        return false;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BVH_CHECK(hipEventCreate(&e0));
+    BVH_CHECK(hipEventCreate(&e1));
+    struct EvGuard { hipEvent_t a, b; ~EvGuard() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); } } eg{e0, e1};
+    const float* src[3] = {p0, p1, p2};
+    for (int k = 0; k < 3; ++k)
+        BVH_CHECK(hipMemcpyAsync(dp[k].p, src[k], 12 * size_t(n), hipMemcpyHostToDevice, s));
+    BVH_CHECK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_prim_bounds, dim3(std::min<uint32_t>((n + 255) / 256, 65536)), dim3(256), 0, s, dp[0].p,
+                       dp[1].p, dp[2].p, n, plo.p, phi.p, ids_a.p);
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, counter.p, size_t(1), 1u);
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, err.p, size_t(1), 0u);
+    BVH_CHECK(hipGetLastError());
+
+    // large-node levels
+    std::vector<NodeTask> medium;
+    std::vector<NodeTask> level{NodeTask{0, 0, n, 0}};
+    DevBuf<LargeTask> dtasks;
+    DevBuf<WorkItem> ditems;
+    DevBuf<Bins> dbins;
+    DevBuf<NodeTask> dchildren;
+    for (int levels = 0; !level.empty(); ++levels) {
+        if (levels >= 64) { *why = "level count"; return false; }
+        std::vector<NodeTask> large;
+        for (const NodeTask& t : level) {
+            if (t.count > kLargeNode) large.push_back(t);
+            else if (t.count > 0) medium.push_back(t);
+        }
+        if (large.empty()) break;
+        const uint32_t nt = uint32_t(large.size());
+        std::vector<LargeTask> ht(nt);
+        std::vector<WorkItem> hi;
+        for (uint32_t i = 0; i < nt; ++i) {
+            LargeTask& t = ht[i];
+            std::memset(&t, 0, sizeof t);
+            t.node = large[i].node;
+            t.begin = large[i].begin;
+            t.count = large[i].count;
+            t.buf = large[i].buf;
+            for (int k = 0; k < 3; ++k) {
+                t.box_lo[k] = int(kIntMax); t.box_hi[k] = -int(kIntMax) - 1;
+                t.cen_lo[k] = int(kIntMax); t.cen_hi[k] = -int(kIntMax) - 1;
+            }
+            for (uint32_t c = t.begin; c < t.begin + t.count; c += kChunk)
+                hi.push_back(WorkItem{i, c, std::min(t.begin + t.count, c + kChunk)});
+        }
+        const uint32_t ni = uint32_t(hi.size());
+        if (dtasks.n < nt && !dtasks.alloc(nt)) { *why = "allocation"; return false; }
+        if (ditems.n < ni && !ditems.alloc(ni)) { *why = "allocation"; return false; }
+        if (dbins.n < nt && !dbins.alloc(nt)) { *why = "allocation"; return false; }
+        if (dchildren.n < 2 * nt && !dchildren.alloc(2 * nt)) { *why = "allocation"; return false; }
+        BVH_CHECK(hipMemcpyAsync(dtasks.p, ht.data(), nt * sizeof(LargeTask), hipMemcpyHostToDevice, s));
+        BVH_CHECK(hipMemcpyAsync(ditems.p, hi.data(), ni * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_init_bins, dim3(std::min<uint32_t>(nt, 4096)), dim3(256), 0, s, dbins.p, nt);
+        hipLaunchKernelGGL(k_large_bounds, dim3(ni), dim3(kBlock), 0, s, dtasks.p, ditems.p, ni, plo.p, phi.p,
+                           ids_a.p, ids_b.p, n, err.p);
+        hipLaunchKernelGGL(k_large_bin, dim3(ni), dim3(kBlock), 0, s, dtasks.p, ditems.p, ni, dbins.p, plo.p, phi.p,
+                           ids_a.p, ids_b.p, n);
+        hipLaunchKernelGGL(k_large_split, dim3((nt + 63) / 64), dim3(64), 0, s, dtasks.p, nt, dbins.p, nodes.p, cap,
+                           counter.p, dchildren.p, err.p);
+        hipLaunchKernelGGL(k_large_partition, dim3(ni), dim3(kBlock), 0, s, dtasks.p, ditems.p, ni, plo.p, phi.p,
+                           ids_a.p, ids_b.p, n, err.p);
+        BVH_CHECK(hipGetLastError());
+        level.resize(2 * nt);
+        BVH_CHECK(hipMemcpyAsync(level.data(), dchildren.p, 2 * nt * sizeof(NodeTask), hipMemcpyDeviceToHost, s));
+        uint32_t herr = 0;
+        BVH_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+        BVH_CHECK(hipStreamSynchronize(s));
+        if (herr) { *why = "large-node pass"; return false; }
+        for (const NodeTask& c : level)
+            if (c.begin + c.count > n || c.node >= cap) { *why = "large-node children"; return false; }
+    }
+    // subtrees
+    if (!medium.empty()) {
+        DevBuf<NodeTask> dmed;
+        if (!dmed.alloc(medium.size())) { *why = "allocation"; return false; }
+        BVH_CHECK(hipMemcpyAsync(dmed.p, medium.data(), medium.size() * sizeof(NodeTask), hipMemcpyHostToDevice, s));
+        const uint32_t grid = uint32_t(std::min<size_t>(medium.size(), 65536));
+        hipLaunchKernelGGL(k_subtree, dim3(grid), dim3(64), 0, s, dmed.p, uint32_t(medium.size()), plo.p, phi.p,
+                           ids_a.p, ids_b.p, order.p, n, nodes.p, cap, counter.p, err.p);
+        BVH_CHECK(hipGetLastError());
+        BVH_CHECK(hipStreamSynchronize(s));
+    }
+    BVH_CHECK(hipEventRecord(e1, s));
+    uint32_t used = 0, herr = 0;
+    BVH_CHECK(hipMemcpyAsync(&used, counter.p, 4, hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (build_ms) *build_ms = ms;
+    if (herr || used == 0 || used > cap) { *why = "subtree pass"; return false; }
+    std::vector<BinaryNode> tree(used);
+    std::vector<uint32_t> prims(n);
+    BVH_CHECK(hipMemcpyAsync(tree.data(), nodes.p, used * sizeof(BinaryNode), hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipMemcpyAsync(prims.data(), order.p, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipStreamSynchronize(s));
+    if (!validate_binary_tree(tree.data(), used, prims.data(), n)) { *why = "validation"; return false; }
+    out = convert_binary_tree(tree.data(), used, prims.data(), n);
+    return true;
+}
+
+}  // namespace quasi_hip

@@ -72,6 +72,22 @@ uint32_t l2_image_blocks(size_t npix);
 hipError_t launch_l2_image(const float4* sum, uint32_t n, const float4* target, float4* dimg, size_t npix,
                            float* partials, hipStream_t stream);
 
+}  // namespace quasi_hip
+
+#include "bvh_build.hpp"
+
+namespace quasi_hip {
+
+// Binned-SAH build on the current device (stream s) of the triangles p0/p1/p2
+// (host arrays, 3 floats per triangle): true with the result in out and the
+// device build time in build_ms, or false with a reason in why (the caller
+// then builds on the host).
+bool build_bvh_gpu(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
+                   BvhBuildResult& out, double* build_ms, const char** why);
+
+// Triangle count from which Q_BVH_BUILD_AUTO builds on the GPU.
+constexpr uint32_t kGpuBuildMinTriangles = 1u << 20;
+
 // Fills n 32-bit words with value (used instead of hipMemsetAsync so that the
 // clears stay valid inside captured graphs).
 hipError_t launch_fill_u32(void* ptr, size_t n, uint32_t value, hipStream_t stream);

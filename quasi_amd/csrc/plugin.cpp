@@ -114,7 +114,8 @@ bool sync_render(plugin_state* st) {
 }
 
 int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
-                      const float* albedo, const float* emission, unsigned threads, SceneHost& out) {
+                      const float* albedo, const float* emission, unsigned threads, SceneHost& out,
+                      uint32_t builder) {
     for (size_t i = 0; i < 3 * size_t(nt); ++i)
         if (idx[i] >= nv) {
             log_msg(st, "scene: vertex index %u out of range (%u vertices)", idx[i], nv);
@@ -127,7 +128,20 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
             p1[3 * t + k] = verts[3 * size_t(idx[3 * t + 1]) + k];
             p2[3 * t + k] = verts[3 * size_t(idx[3 * t + 2]) + k];
         }
-    BvhBuildResult bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, threads);
+    BvhBuildResult bvh;
+    const bool gpu = threads == 0 && (builder == Q_BVH_BUILD_GPU ||
+                                      (builder == Q_BVH_BUILD_AUTO && nt >= kGpuBuildMinTriangles));
+    out.builder = Q_BVH_BUILD_HOST;
+    out.device_ms = 0;
+    if (gpu) {
+        const char* why = "";
+        if (build_bvh_gpu(p0.data(), p1.data(), p2.data(), nt, st->stream, bvh, &out.device_ms, &why))
+            out.builder = Q_BVH_BUILD_GPU;
+        else
+            log_msg(st, "scene: GPU BVH build failed (%s); building on the host", why);
+    }
+    if (out.builder == Q_BVH_BUILD_HOST) bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, threads);
+    out.sah_cost = bvh.sah_cost;
     if (bvh.depth > megakernel_max_depth()) {
         log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", bvh.depth, megakernel_max_depth());
         return Q_EXT_ERR_UNSUPPORTED;
@@ -260,7 +274,7 @@ int32_t upload_scene(plugin_state* st) {
     auto t0 = std::chrono::steady_clock::now();
     SceneHost sh;
     int32_t rc = prepare_scene(st, st->verts.data(), nv, st->indices.data(), nt, st->albedo.data(),
-                               st->emission.data(), 0, sh);
+                               st->emission.data(), 0, sh, st->params.bvh_builder);
     if (rc != Q_EXT_OK) return rc;
     const uint32_t gentries = std::max(stack_overflow_entries(sh.depth, sh.depth4), st->batch.gstack_entries);
     if (!ensure_buffer(st, st->tris, std::max<size_t>(16, sh.tri4.size() * 4), "triangles") ||
@@ -285,6 +299,9 @@ int32_t upload_scene(plugin_state* st) {
     st->bvh_depth4 = sh.depth4;
     st->gstack_entries = gentries;
     ++st->scene_generation;
+    st->bvh_builder = sh.builder;
+    st->bvh_sah_cost = sh.sah_cost;
+    st->bvh_device_ms = sh.device_ms;
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return Q_EXT_OK;
 }
@@ -743,7 +760,8 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
         Q_render_params p = *in;
         p.struct_size = sizeof(Q_render_params);
         if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > kMaxDepth ||
-            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT) {
+            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT ||
+            p.bvh_builder > Q_BVH_BUILD_GPU) {
             log_msg(st, "set_params: unsupported parameter combination");
             return Q_EXT_ERR_UNSUPPORTED;
         }
@@ -803,6 +821,9 @@ Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out
     s.rays_extension = c.rays_extension;
     s.rays_shadow = c.rays_shadow;
     s.bvh_build_ms = st->bvh_build_ms;
+    s.bvh_builder = st->bvh_builder;
+    s.bvh_sah_cost = st->bvh_sah_cost;
+    s.bvh_device_ms = st->bvh_device_ms;
     s.render_ms = 0;
     for (uint32_t k = 0; k < K_COUNT; ++k)
         if (kRenderKernel[k] && k != K_BATCH_PASS) s.render_ms += st->kernel_ms[k];

@@ -47,6 +47,8 @@ struct SceneHost {
     std::vector<float> tri4, mat4, nodes, nodes4;
     std::vector<LightEntry> lights;
     uint32_t num_tris = 0, num_nodes = 0, num_nodes4 = 0, depth = 0, depth4 = 0;
+    uint32_t builder = Q_BVH_BUILD_HOST;   // where the BVH was built
+    double sah_cost = 0, device_ms = 0;
 };
 
 // Several scenes traced together (Q_plugin_set_batch).
@@ -97,6 +99,8 @@ struct plugin_state {
     uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
     uint32_t num_nodes4 = 0, bvh_depth4 = 0, gstack_entries = 0;
     double bvh_build_ms = 0;
+    uint32_t bvh_builder = Q_BVH_BUILD_HOST;
+    double bvh_sah_cost = 0, bvh_device_ms = 0;
 
     // framebuffer
     uint32_t width = 0, height = 0;
@@ -145,9 +149,12 @@ struct LaunchTimer {
 };
 
 // Prepares one mesh (BVH build, triangle / material / node / light arrays);
-// threads = 0 uses up to 16 host threads for the BVH build.
+// threads = 0 uses up to 16 host threads for a host BVH build and lets
+// builder (Q_BVH_BUILD_*) choose the GPU build; threads > 0 always builds on
+// the host with that many threads.
 int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
-                      const float* albedo, const float* emission, unsigned threads, SceneHost& out);
+                      const float* albedo, const float* emission, unsigned threads, SceneHost& out,
+                      uint32_t builder = Q_BVH_BUILD_HOST);
 
 // Overflow entries per thread that the persistent traversal stacks need for
 // a BVH2 of `depth` and a BVH4 of `depth4` levels.

@@ -314,3 +314,88 @@ def save_qscene(mesh: Mesh, path) -> None:
         f.write(np.ascontiguousarray(mesh.indices, dtype="<u4").tobytes())
         f.write(np.ascontiguousarray(mesh.albedo, dtype="<f4").tobytes())
         f.write(np.ascontiguousarray(mesh.emission, dtype="<f4").tobytes())
+
+
+# ---------------------------------------------------------------- large heightfield terrain
+TERRAIN_CAMERA = dict(position=(0.0, 1.6, 4.6), target=(0.0, 0.0, 0.6), up=(0.0, 1.0, 0.0), fov=45.0)
+
+
+def _lattice_hash(ix, iz, seed):
+    """Lattice values in [0, 1): the PCG hash of the two integer coordinates
+    and the octave seed (u32 arithmetic, vectorised)."""
+    def pcg(x):
+        state = (x * np.uint32(747796405) + np.uint32(2891336453)).astype(np.uint32)
+        word = (((state >> ((state >> np.uint32(28)) + np.uint32(4))) ^ state) * np.uint32(277803737)).astype(np.uint32)
+        return (word >> np.uint32(22)) ^ word
+    with np.errstate(over="ignore"):
+        h = pcg(ix.astype(np.uint32) ^ pcg(iz.astype(np.uint32) ^ pcg(np.full_like(ix, seed, dtype=np.uint32))))
+    return h.astype(np.float64) * (1.0 / 4294967296.0)
+
+
+def value_noise(x, z, seed):
+    """Value noise: hashed lattice values blended with the smoothstep of the
+    fractional coordinates."""
+    ix, iz = np.floor(x), np.floor(z)
+    fx, fz = x - ix, z - iz
+    ix, iz = ix.astype(np.int64), iz.astype(np.int64)
+    sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+    v00 = _lattice_hash(ix, iz, seed)
+    v10 = _lattice_hash(ix + 1, iz, seed)
+    v01 = _lattice_hash(ix, iz + 1, seed)
+    v11 = _lattice_hash(ix + 1, iz + 1, seed)
+    return (v00 * (1 - sx) + v10 * sx) * (1 - sz) + (v01 * (1 - sx) + v11 * sx) * sz
+
+
+def fbm(x, z, octaves=6, base_frequency=1.5, gain=0.5, seed=7):
+    """Fractal sum of value noise in [0, 1): octave k has frequency
+    base * 2^k and weight gain^k, normalised by the total weight."""
+    total = np.zeros_like(x)
+    norm = 0.0
+    f, a = base_frequency, 1.0
+    for k in range(octaves):
+        total += a * value_noise(x * f, z * f, seed + k)
+        norm += a
+        f *= 2.0
+        a *= gain
+    return total / norm
+
+
+def terrain(grid: int = 4096, size: float = 8.0, amplitude: float = 0.2, seed: int = 7) -> Mesh:
+    """Heightfield terrain: a grid x grid vertex lattice over [-size/2,
+    size/2]^2 (2 (grid-1)^2 triangles; grid 4096 gives 33,538,050), heights
+    amplitude * FBM (6 octaves of value noise), per-triangle albedo from the
+    noise at the triangle's first vertex; lit by a sun-sized disk (a 16-triangle
+    fan of radius 0.25 at distance 50, radiance 4000) and a 40 x 40 sky quad at
+    y = 8 (radiance (0.35, 0.45, 0.65))."""
+    g = int(grid)
+    xs = np.linspace(-size / 2, size / 2, g)
+    X, Z = np.meshgrid(xs, xs)                      # row = z, column = x
+    n = fbm(X, Z, seed=seed)
+    Y = amplitude * n
+    verts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(F32)
+    i = np.arange(g - 1, dtype=np.uint32)
+    r, c = np.meshgrid(i, i, indexing="ij")
+    v00 = (r * g + c).ravel()
+    v10 = v00 + 1
+    v01 = v00 + g
+    v11 = v01 + 1
+    tris = np.empty((2 * v00.size, 3), dtype=np.uint32)
+    tris[0::2, 0], tris[0::2, 1], tris[0::2, 2] = v00, v01, v11
+    tris[1::2, 0], tris[1::2, 1], tris[1::2, 2] = v00, v11, v10
+    t = n.reshape(-1)[tris[:, 0]].astype(F32)
+    alb = np.stack([0.18 + 0.45 * t, 0.22 + 0.40 * t, 0.12 + 0.25 * t], axis=1).astype(F32)
+    b = _Builder()
+    b.v, b.i, b.a, b.e = [verts], [tris], [alb], [np.zeros_like(alb)]
+    # sun: a fan facing the origin from direction d
+    d = np.array([0.45, 0.8, -0.4])
+    d /= np.linalg.norm(d)
+    centre = 50.0 * d
+    u = np.cross(d, [0.0, 0.0, 1.0])
+    u /= np.linalg.norm(u)
+    w = np.cross(d, u)
+    ang = np.linspace(0, 2 * np.pi, 17)
+    ring = centre + 0.25 * (np.cos(ang)[:, None] * u + np.sin(ang)[:, None] * w)
+    fan = np.stack([np.repeat(centre[None], 16, axis=0), ring[:-1], ring[1:]], axis=1).astype(F32)
+    b.triangles(fan, np.zeros((16, 3), dtype=F32), np.full((16, 3), 4000.0, dtype=F32))
+    b.quad((-20.0, 8.0, -20.0), (40.0, 0.0, 0.0), (0.0, 0.0, 40.0), (0.0, 0.0, 0.0), (0.35, 0.45, 0.65))
+    return b.mesh(TERRAIN_CAMERA, f"terrain{g}")
