@@ -309,11 +309,30 @@ BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes,
     if (ntri == 0 || num_nodes == 0) return res;
     struct Tmp {
         Box box;
+        double area;     // surface area of box
         uint32_t left, right, first, count;
     };
     std::vector<Tmp> nodes(num_nodes);
-    for (uint32_t i = 0; i < num_nodes; ++i)
-        nodes[i] = Tmp{box_of(tnodes[i]), tnodes[i].left, tnodes[i].right, tnodes[i].first, tnodes[i].count};
+    {
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        auto fill = [&](size_t b0, size_t b1) {
+            for (size_t i = b0; i < b1; ++i) {
+                const Box bx = box_of(tnodes[i]);
+                nodes[i] = Tmp{bx, bx.area(), tnodes[i].left, tnodes[i].right, tnodes[i].first, tnodes[i].count};
+            }
+        };
+        if (num_nodes < (1u << 16) || nth == 1) {
+            fill(0, num_nodes);
+        } else {
+            std::vector<std::thread> pool;
+            const size_t chunk = (size_t(num_nodes) + nth - 1) / nth;
+            for (unsigned t = 0; t < nth; ++t) {
+                const size_t b0 = t * chunk, b1 = std::min<size_t>(num_nodes, b0 + chunk);
+                if (b0 < b1) pool.emplace_back(fill, b0, b1);
+            }
+            for (auto& th : pool) th.join();
+        }
+    }
     struct {
         const std::vector<Tmp>& nodes;
         const uint32_t* prims;
@@ -388,7 +407,7 @@ BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes,
     // it is popped, and the parent's child slot is patched at that moment.
     struct Walk { uint32_t tmp; int64_t parent; uint32_t side; uint32_t depth; };
     uint32_t next_out = 0;
-    double root_area = std::max(root.box.area(), 1e-30);
+    double root_area = std::max(root.area, 1e-30);
     double cost = 0;
     std::vector<Walk> st{{0, -1, 0, 1}};
     while (!st.empty()) {
@@ -400,12 +419,12 @@ BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes,
         }
         const Tmp& nd = b.nodes[it.tmp];
         res.depth = std::max(res.depth, it.depth);
-        cost += nd.box.area() / root_area;
+        cost += nd.area / root_area;
         const Tmp& L = b.nodes[nd.left];
         const Tmp& R = b.nodes[nd.right];
         int32_t cl = 0, cr = 0;
-        if (L.count) { cl = encode_leaf(nd.left); cost += L.box.area() / root_area * L.count; }
-        if (R.count) { cr = encode_leaf(nd.right); cost += R.box.area() / root_area * R.count; }
+        if (L.count) { cl = encode_leaf(nd.left); cost += L.area / root_area * L.count; }
+        if (R.count) { cr = encode_leaf(nd.right); cost += R.area / root_area * R.count; }
         write_node(idx, L.box, R.box, cl, cr);
         if (!R.count) st.push_back({nd.right, int64_t(idx), 1, it.depth + 1});
         if (!L.count) st.push_back({nd.left, int64_t(idx), 0, it.depth + 1});
@@ -438,7 +457,7 @@ BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes,
             double best_area = -1.0;
             for (uint32_t k = 0; k < nk; ++k) {
                 const Tmp& c = b.nodes[kids[k]];
-                if (c.count == 0 && c.box.area() > best_area) { best_area = c.box.area(); pick = int(k); }
+                if (c.count == 0 && c.area > best_area) { best_area = c.area; pick = int(k); }
             }
             if (pick < 0) break;
             const Tmp& c = b.nodes[kids[pick]];

@@ -32,10 +32,10 @@ namespace quasi_hip {
 namespace {
 
 constexpr int kBins = kSahBins;             // 32
-constexpr uint32_t kLargeNode = 4096;       // larger nodes take the level-synchronous path
+constexpr uint32_t kLargeNode = 512;        // larger nodes take the level-synchronous path
 constexpr uint32_t kChunk = 8192;           // triangles per block in the large-node kernels
 constexpr int kBlock = 256;
-constexpr int kStack = 48;                  // subtree task stack (smaller child first: <= log2(4096) + 1 used)
+constexpr int kStack = 48;                  // subtree task stack (smaller child first: <= log2(512) + 1 used)
 constexpr uint32_t kIntMax = 0x7FFFFFFF;
 
 // Error flags

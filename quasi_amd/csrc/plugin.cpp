@@ -113,6 +113,27 @@ bool sync_render(plugin_state* st) {
     return hip_ok(st, hipStreamSynchronize(st->stream), "stream synchronise");
 }
 
+namespace {
+
+// f(begin, end) over [0, n) split across up to `threads` host threads.
+template <class F>
+void parallel_ranges(size_t n, unsigned threads, F&& f) {
+    if (threads <= 1 || n < (size_t(1) << 16)) {
+        f(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t chunk = (n + threads - 1) / threads;
+    for (unsigned i = 1; i < threads; ++i) {
+        const size_t b = i * chunk, e = std::min(n, b + chunk);
+        if (b < e) pool.emplace_back([&f, b, e] { f(b, e); });
+    }
+    f(size_t(0), std::min(n, chunk));
+    for (auto& t : pool) t.join();
+}
+
+}  // namespace
+
 int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                       const float* albedo, const float* emission, unsigned threads, SceneHost& out,
                       uint32_t builder) {
@@ -121,13 +142,16 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
             log_msg(st, "scene: vertex index %u out of range (%u vertices)", idx[i], nv);
             return Q_EXT_ERR_ARGUMENT;
         }
+    const unsigned host_threads = threads ? threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<float> p0(3 * size_t(nt)), p1(3 * size_t(nt)), p2(3 * size_t(nt));
-    for (size_t t = 0; t < nt; ++t)
-        for (int k = 0; k < 3; ++k) {
-            p0[3 * t + k] = verts[3 * size_t(idx[3 * t + 0]) + k];
-            p1[3 * t + k] = verts[3 * size_t(idx[3 * t + 1]) + k];
-            p2[3 * t + k] = verts[3 * size_t(idx[3 * t + 2]) + k];
-        }
+    parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
+        for (size_t t = tb; t < te; ++t)
+            for (int k = 0; k < 3; ++k) {
+                p0[3 * t + k] = verts[3 * size_t(idx[3 * t + 0]) + k];
+                p1[3 * t + k] = verts[3 * size_t(idx[3 * t + 1]) + k];
+                p2[3 * t + k] = verts[3 * size_t(idx[3 * t + 2]) + k];
+            }
+    });
     BvhBuildResult bvh;
     const bool gpu = threads == 0 && (builder == Q_BVH_BUILD_GPU ||
                                       (builder == Q_BVH_BUILD_AUTO && nt >= kGpuBuildMinTriangles));
@@ -149,22 +173,24 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
     std::vector<uint32_t> bvh_pos(nt);
     out.tri4.assign(12 * size_t(nt), 0.f);
     out.mat4.assign(8 * size_t(nt), 0.f);
-    for (uint32_t i = 0; i < nt; ++i) {
-        const uint32_t t = bvh.order[i];
-        bvh_pos[t] = i;
-        float* d = &out.tri4[12 * size_t(i)];
-        for (int k = 0; k < 3; ++k) {
-            d[k] = p0[3 * size_t(t) + k];
-            d[4 + k] = p1[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
-            d[8 + k] = p2[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+    parallel_ranges(nt, host_threads, [&](size_t ib, size_t ie) {
+        for (size_t i = ib; i < ie; ++i) {
+            const uint32_t t = bvh.order[i];
+            bvh_pos[t] = uint32_t(i);
+            float* d = &out.tri4[12 * i];
+            for (int k = 0; k < 3; ++k) {
+                d[k] = p0[3 * size_t(t) + k];
+                d[4 + k] = p1[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+                d[8 + k] = p2[3 * size_t(t) + k] - p0[3 * size_t(t) + k];
+            }
+            std::memcpy(&d[3], &t, 4);
+            float* m = &out.mat4[8 * i];
+            const float* a = &albedo[3 * size_t(t)];
+            const float* e = &emission[3 * size_t(t)];
+            m[0] = a[0]; m[1] = a[1]; m[2] = a[2]; m[3] = std::max(e[0], std::max(e[1], e[2]));
+            m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
         }
-        std::memcpy(&d[3], &t, 4);
-        float* m = &out.mat4[8 * size_t(i)];
-        const float* a = &albedo[3 * size_t(t)];
-        const float* e = &emission[3 * size_t(t)];
-        m[0] = a[0]; m[1] = a[1]; m[2] = a[2]; m[3] = std::max(e[0], std::max(e[1], e[2]));
-        m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
-    }
+    });
     // Light table in input-triangle order: weight = area * (Le.r + Le.g + Le.b),
     // prefix sums in double, cdf stored as float with the last entry set to 1.
     out.lights.clear();

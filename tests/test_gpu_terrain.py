@@ -1,0 +1,99 @@
+"""The large workload: a 4096 x 4096 heightfield (33,538,050 terrain
+triangles, FBM heights) lit by a sun disk and a sky quad, 2048 x 2048.
+
+Size-independent properties checked at full size:
+- the GPU-built BVH renders the same image and primary ids, bit for bit, as
+  the host-built BVH (512 x 512, 1 spp);
+- Euler's identity for the adjoint at 2048 x 2048, 16 spp, depth 6 without
+  Russian roulette: sum(albedo * dl/dalbedo) = sum_k (k-1) (l_k - l_{k-1}),
+  l_k the loss of GPU forward renders of depth k (each NEE term found at
+  bounce b carries b+1 albedo factors), to 1e-4 relative;
+- the 1M-parameter albedo grid (1024 x 1024 blocks of 4 x 4 cells, 32
+  triangles each) collects every terrain triangle's gradient once.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from quasi_amd import abi, scenes
+from quasi_amd.plugin import HipRenderer
+
+pytestmark = pytest.mark.gpu
+
+GRID = 4096
+NT_TERRAIN = 2 * (GRID - 1) ** 2
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    t0 = time.time()
+    m = scenes.terrain(GRID)
+    print(f"[terrain] {m.num_triangles} triangles generated in {time.time() - t0:.1f} s", flush=True)
+    return m
+
+
+def cam(r, m):
+    c = m.camera
+    r.set_camera(c["position"], c["target"], c["up"], c["fov"])
+
+
+def test_gpu_and_host_bvh_render_alike(terrain):
+    out = {}
+    for builder in (abi.Q_BVH_BUILD_GPU, abi.Q_BVH_BUILD_HOST):
+        with HipRenderer(512, 512) as r:
+            r.set_params(bvh_builder=builder)
+            t0 = time.time()
+            r.set_scene(terrain)
+            st = r.stats()
+            print(f"[terrain] builder {builder}: set_scene {time.time() - t0:.1f} s (build+upload "
+                  f"{st.bvh_build_ms:.0f} ms, device {st.bvh_device_ms:.0f} ms), SAH {st.bvh_sah_cost:.2f}, "
+                  f"nodes {st.bvh_nodes}, depth {st.bvh_depth}", flush=True)
+            assert st.bvh_builder == builder
+            cam(r, terrain)
+            r.set_params(spp_per_render=1, max_depth=6, rr_start=3, rng_mode=1, integrator=1, seed=2,
+                         engine=abi.Q_ENGINE_WAVEFRONT, bvh_width=4, flags=abi.Q_FLAG_RECORD_IDS, bvh_builder=builder)
+            r.render(1)
+            out[builder] = (r.readback(), r.primary_ids(), st.bvh_sah_cost)
+    (ia, ida, ca), (ib, idb, cb) = out[abi.Q_BVH_BUILD_GPU], out[abi.Q_BVH_BUILD_HOST]
+    assert np.array_equal(ia, ib) and np.array_equal(ida, idb)
+    assert ca <= 1.03 * cb
+
+
+def test_adjoint_euler_identity_full_size(terrain):
+    w = h = 2048
+    spp, D = 16, 6
+    g = np.random.Generator(np.random.PCG64(1)).normal(size=(h, w, 4)).astype(np.float32)
+    g[..., 3] = 0.0
+    with HipRenderer(w, h) as r:
+        r.set_scene(terrain)
+        cam(r, terrain)
+        losses = []
+        for k in range(1, D + 1):
+            r.set_params(spp_per_render=spp, max_depth=k, rr_start=D, rng_mode=1, integrator=1, seed=0,
+                         engine=abi.Q_ENGINE_WAVEFRONT, bvh_width=4, flags=abi.Q_FLAG_TIME_KERNELS)
+            r.reset_stats()
+            r.render(1)
+            img = r.readback().astype(np.float64)
+            losses.append(float((img[..., :3] * g[..., :3]).sum()))
+            if k == D:
+                st = r.stats()
+                rays = st.rays_camera + st.rays_extension + st.rays_shadow
+                print(f"[terrain] forward {w}x{h} {spp} spp depth {D}: {rays} rays, "
+                      f"{st.render_ms:.1f} ms kernels", flush=True)
+        t0 = time.time()
+        ga, gv, res = r.adjoint(g, spp)
+        print(f"[terrain] adjoint: {res.kernel_ms:.1f} ms device, {time.time() - t0:.2f} s wall", flush=True)
+    alb = terrain.albedo.astype(np.float64)
+    lhs = float((alb * ga).sum())
+    rhs = sum((k - 1) * (losses[k - 1] - losses[k - 2]) for k in range(2, D + 1))
+    print(f"[terrain] Euler: sum(albedo * grad) {lhs:.6f}, from forward renders {rhs:.6f}", flush=True)
+    assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
+    assert np.isfinite(gv).all() and np.abs(gv).sum() > 0
+    # 1024 x 1024 parameter grid over the (GRID-1)^2 cells, 2 triangles per cell
+    cells = np.arange(NT_TERRAIN) // 2
+    row, col = cells // (GRID - 1), cells % (GRID - 1)
+    group = np.minimum(row // 4, 1023) * 1024 + np.minimum(col // 4, 1023)
+    gg = np.stack([np.bincount(group, weights=ga[:NT_TERRAIN, c], minlength=1024 * 1024) for c in range(3)], axis=1)
+    assert gg.shape == (1024 * 1024, 3)
+    assert np.allclose(gg.sum(axis=0), ga[:NT_TERRAIN].sum(axis=0), rtol=1e-9, atol=1e-12)
