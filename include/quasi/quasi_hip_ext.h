@@ -6,7 +6,10 @@
  * never looks them up still works.
  *
  * Conventions (all functions):
- *   - every struct starts with uint32_t struct_size = sizeof(struct);
+ *   - every struct starts with uint32_t struct_size = sizeof(struct); structs only
+ *     grow at the end, and a caller built against an older header passes its
+ *     smaller size (Q_render_params and Q_render_stats: at least the size they
+ *     had without the bvh_builder fields);
  *   - int32_t return: Q_EXT_OK (0) or a negative Q_EXT_ERR_* code, with a message
  *     sent through Q_plugin_context.log;
  *   - inputs are caller-owned and copied before the call returns;
@@ -82,15 +85,16 @@ typedef struct Q_render_params {
     uint32_t rng_mode;         /* Q_RNG_* */
     uint32_t integrator;       /* Q_INTEGRATOR_* */
     uint32_t engine;           /* Q_ENGINE_* */
-    uint32_t bvh_width;        /* wavefront traversal: 2 or 4 (0 = 2); the megakernel walks BVH2 */
+    uint32_t bvh_width;        /* wavefront traversal: 2 or 4 (0 = 4); the megakernel walks BVH2 */
     uint32_t samples_per_pass; /* wavefront: samples per pixel in flight at once (0 = auto: about
                                   2^27 paths per pass; for a batch, the largest power of two that
                                   divides the samples requested and stays within that) */
     uint32_t block_size;       /* wavefront traversal threads per block: 64, 128, 256 (0 = 256) */
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
-    uint32_t queue_chunk;      /* wavefront: rays a wave takes per queue access, 64 or 256 (0 = 64);
-                                  1 = each lane fetches its next ray as soon as it is idle */
+    uint32_t queue_chunk;      /* wavefront: 1 (0 = 1) = each idle lane takes its next ray from the
+                                  queue (refilled whenever half a wave is idle); 64 or 256 = each
+                                  wave takes that many rays at a time */
     uint32_t bvh_builder;      /* Q_BVH_BUILD_*: where Q_plugin_set_scene builds the BVH */
     uint32_t reserved;
 } Q_render_params;

@@ -290,109 +290,6 @@ __device__ __forceinline__ TraceHit traverse4(const float4* __restrict__ nodes, 
     }
 }
 
-// Resumable traversal for the dynamic-fetch kernels: one call of trav_step
-// processes one node (internal or leaf) of the lane's current ray, so a wave
-// can interleave rays that started at different times.  Same visit order and
-// tie rule as traverse2 / traverse4.
-struct TravState {
-    Ray r;
-    f3 inv;
-    float tmin;
-    float best_t;
-    int best_tri;
-    uint32_t best_id;
-    int node;
-};
-
-__device__ __forceinline__ void trav_begin(TravState& s, const Ray& r, float tmin, float tmax) {
-    s.r = r;
-    s.inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
-    s.tmin = tmin;
-    s.best_t = tmax;
-    s.best_tri = -1;
-    s.best_id = 0u;
-    s.node = 0;
-}
-
-// Returns true when the ray's traversal has finished (result in s.best_*).
-template <int WIDTH, bool ANY, class Stack>
-__device__ __forceinline__ bool trav_step(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          TravState& s, Stack& st) {
-    if (s.node >= 0) {
-        if (WIDTH == 4) {
-            const float4* n = nodes + 8 * s.node;
-            const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-            const int4 ch = *reinterpret_cast<const int4*>(n + 6);
-            float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
-            slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.r, s.inv, s.tmin, s.best_t, t0n, t0f);
-            slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, s.r, s.inv, s.tmin, s.best_t, t1n, t1f);
-            slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, s.r, s.inv, s.tmin, s.best_t, t2n, t2f);
-            slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, s.r, s.inv, s.tmin, s.best_t, t3n, t3f);
-            constexpr float kMissKey = 3.0e38f;
-            float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
-            float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
-            float k2 = (t2n <= t2f && ch.z != kEmptyChild) ? t2n : kMissKey;
-            float k3 = (t3n <= t3f && ch.w != kEmptyChild) ? t3n : kMissKey;
-            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-            cswap(k0, c0, k1, c1);
-            cswap(k2, c2, k3, c3);
-            cswap(k0, c0, k2, c2);
-            cswap(k1, c1, k3, c3);
-            cswap(k1, c1, k2, c2);
-            if (k0 == kMissKey) {
-                if (st.empty()) return true;
-                s.node = st.pop();
-                return false;
-            }
-            if (k3 != kMissKey) st.push(c3);
-            if (k2 != kMissKey) st.push(c2);
-            if (k1 != kMissKey) st.push(c1);
-            s.node = c0;
-            return false;
-        } else {
-            const float4 a = nodes[4 * s.node + 0];
-            const float4 b = nodes[4 * s.node + 1];
-            const float4 c = nodes[4 * s.node + 2];
-            const float4 dd = nodes[4 * s.node + 3];
-            const int cl = __float_as_int(dd.x);
-            const int cr = __float_as_int(dd.y);
-            float tnl, tfl, tnr, tfr;
-            slab(a.x, a.y, a.z, a.w, c.x, c.y, s.r, s.inv, s.tmin, s.best_t, tnl, tfl);
-            slab(b.x, b.y, b.z, b.w, c.z, c.w, s.r, s.inv, s.tmin, s.best_t, tnr, tfr);
-            const bool hl = tnl <= tfl;
-            const bool hr = tnr <= tfr;
-            if (hl && hr) {
-                const bool left_first = tnl <= tnr;
-                st.push(left_first ? cr : cl);
-                s.node = left_first ? cl : cr;
-            } else if (hl || hr) {
-                s.node = hl ? cl : cr;
-            } else {
-                if (st.empty()) return true;
-                s.node = st.pop();
-            }
-            return false;
-        }
-    }
-    const uint32_t v = ~uint32_t(s.node);
-    const int first = int(v >> 2);
-    const int count = int(v & 3u) + 1;
-    for (int i = first; i < first + count; ++i) {
-        float t;
-        uint32_t id;
-        if (!hit_triangle(tris, i, s.r, s.tmin, s.best_t, t, id)) continue;
-        if (ANY) { s.best_t = t; s.best_tri = i; s.best_id = id; return true; }
-        if (s.best_tri < 0 || t < s.best_t || id > s.best_id) {
-            s.best_t = t;
-            s.best_tri = i;
-            s.best_id = id;
-        }
-    }
-    if (st.empty()) return true;
-    s.node = st.pop();
-    return false;
-}
-
 // Convenience wrappers with an LDS stack (megakernel).
 template <int STRIDE>
 __device__ __forceinline__ TraceHit trace_closest(const float4* __restrict__ nodes, const float4* __restrict__ tris,

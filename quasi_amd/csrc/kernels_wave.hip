@@ -237,12 +237,16 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
     }
 }
 
-// Persistent traversal with per-lane dynamic ray fetch: a lane whose ray has
-// finished takes the next one from the queue as soon as fewer than kRefill
-// lanes of its wave are still busy, instead of the whole wave waiting for its
-// slowest ray.  Every fetch is one atomic per wave on the bounce's cursor.
-// SHADOW selects the any-hit shadow queue; otherwise the closest-hit queue.
-constexpr uint32_t kRefill = 40;
+// Persistent traversal with per-lane dynamic ray fetch, in while-while form:
+// every lane descends through internal nodes until it reaches a leaf (the wave
+// loops while any lane is still descending), then the lanes at a leaf test its
+// triangles together; a lane whose ray has finished takes the next ray from
+// the queue as soon as kRefillIdle lanes of its wave are idle, so a wave never
+// waits for its slowest ray.  Every fetch is one atomic per wave on the
+// bounce's cursor.  SHADOW selects the any-hit shadow queue; otherwise the
+// closest-hit queue.  Same visit order and tie rule as traverse2 / traverse4.
+constexpr uint32_t kRefillIdle = 32;   // measured best of 4..48 on the 1M soup
+constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
@@ -261,37 +265,42 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     float4* rad = static_cast<float4*>(S.radiance);
     uint32_t* cursor = &S.fetch[2 * b + (SHADOW ? 1 : 0)];
 
-    TravState ts;
+    Ray r{mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 1.f)};
+    f3 inv = mk(0.f, 0.f, 0.f);
+    float tmin = kTMin, best_t = kTFar;
+    int best_tri = -1;
+    uint32_t best_id = 0u;
+    int node = kDone;
     bool busy = false, exhausted = false;
-    uint32_t idx = 0, sid = 0, path = 0;
+    uint32_t idx = 0, path = 0;
     const float4* nodes = nullptr;
     const float4* tris = nullptr;
     for (;;) {
-        // refill idle lanes (wave-uniform: every lane reaches the ballot)
-        const unsigned long long need = __ballot(!busy && !exhausted);
-        if (need) {
+        // refill idle lanes (wave-uniform: every lane reaches the ballots)
+        const unsigned long long idle = __ballot(!busy && !exhausted);
+        const unsigned long long act = __ballot(busy);
+        if (idle && (act == 0ull || uint32_t(__popcll(idle)) >= kRefillIdle)) {
             uint32_t base = 0;
-            if ((threadIdx.x & 63u) == 0u) base = atomicAdd(cursor, uint32_t(__popcll(need)));
+            if ((threadIdx.x & 63u) == 0u) base = atomicAdd(cursor, uint32_t(__popcll(idle)));
             base = __shfl(base, 0, 64);
             if (!busy && !exhausted) {
-                idx = base + lanes_below(need);
+                idx = base + lanes_below(idle);
                 if (idx < n) {
                     const float4 o = ro[idx];
                     const float4 d = rd[idx];
                     path = __float_as_uint(d.w);
-                    sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
+                    const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
                     const SceneDesc& sc = scene_at<BATCH>(P, sid);
                     nodes = scene_nodes<WIDTH, BATCH>(P, sid);
                     tris = static_cast<const float4*>(sc.tris);
-                    trav_begin(ts, Ray{xyz(o), xyz(d)}, kTMin, SHADOW ? o.w : kTFar);
+                    r = Ray{xyz(o), xyz(d)};
+                    inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+                    best_t = SHADOW ? o.w : kTFar;
+                    best_tri = -1;
+                    best_id = 0u;
                     st.sp = 0;
+                    node = sc.num_nodes ? 0 : kDone;
                     busy = true;
-                    if (sc.num_nodes == 0) {
-                        // nothing to intersect: the ray is finished already
-                        if (SHADOW) add_shadow(rad, sc_a, idx, path);
-                        else hit[idx] = make_uint2(__float_as_uint(kTFar), uint32_t(-1));
-                        busy = false;
-                    }
                 } else {
                     exhausted = true;
                 }
@@ -301,18 +310,85 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             if (__ballot(!exhausted) == 0ull) break;
             continue;
         }
-        // traverse until too few lanes are left busy
-        for (;;) {
-            if (busy && trav_step<WIDTH, SHADOW>(nodes, tris, ts, st)) {
-                busy = false;
-                if (SHADOW) {
-                    if (ts.best_tri < 0) add_shadow(rad, sc_a, idx, path);
+        // descend through internal nodes
+        while (busy && node >= 0) {
+            if (WIDTH == 4) {
+                const float4* nd = nodes + 8 * node;
+                const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+                const int4 ch = *reinterpret_cast<const int4*>(nd + 6);
+                float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
+                slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, inv, tmin, best_t, t0n, t0f);
+                slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, inv, tmin, best_t, t1n, t1f);
+                slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, inv, tmin, best_t, t2n, t2f);
+                slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, inv, tmin, best_t, t3n, t3f);
+                constexpr float kMissKey = 3.0e38f;
+                float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
+                float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
+                float k2 = (t2n <= t2f && ch.z != kEmptyChild) ? t2n : kMissKey;
+                float k3 = (t3n <= t3f && ch.w != kEmptyChild) ? t3n : kMissKey;
+                int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+                cswap(k0, c0, k1, c1);
+                cswap(k2, c2, k3, c3);
+                cswap(k0, c0, k2, c2);
+                cswap(k1, c1, k3, c3);
+                cswap(k1, c1, k2, c2);
+                if (k0 == kMissKey) {
+                    node = st.empty() ? kDone : st.pop();
                 } else {
-                    hit[idx] = make_uint2(__float_as_uint(ts.best_t), uint32_t(ts.best_tri));
+                    if (k3 != kMissKey) st.push(c3);
+                    if (k2 != kMissKey) st.push(c2);
+                    if (k1 != kMissKey) st.push(c1);
+                    node = c0;
+                }
+            } else {
+                const float4 a = nodes[4 * node + 0];
+                const float4 bb = nodes[4 * node + 1];
+                const float4 c = nodes[4 * node + 2];
+                const float4 dd = nodes[4 * node + 3];
+                const int cl = __float_as_int(dd.x);
+                const int cr = __float_as_int(dd.y);
+                float tnl, tfl, tnr, tfr;
+                slab(a.x, a.y, a.z, a.w, c.x, c.y, r, inv, tmin, best_t, tnl, tfl);
+                slab(bb.x, bb.y, bb.z, bb.w, c.z, c.w, r, inv, tmin, best_t, tnr, tfr);
+                const bool hl = tnl <= tfl;
+                const bool hr = tnr <= tfr;
+                if (hl && hr) {
+                    const bool left_first = tnl <= tnr;
+                    st.push(left_first ? cr : cl);
+                    node = left_first ? cl : cr;
+                } else if (hl || hr) {
+                    node = hl ? cl : cr;
+                } else {
+                    node = st.empty() ? kDone : st.pop();
                 }
             }
-            const uint32_t active = uint32_t(__popcll(__ballot(busy)));
-            if (active < kRefill) break;
+        }
+        // the lanes at a leaf test its triangles
+        if (busy && node != kDone) {
+            const uint32_t v = ~uint32_t(node);
+            const int first = int(v >> 2);
+            const int count = int(v & 3u) + 1;
+            bool found = false;
+            for (int i = first; i < first + count; ++i) {
+                float t;
+                uint32_t id;
+                if (!hit_triangle(tris, i, r, tmin, best_t, t, id)) continue;
+                if (SHADOW) { best_t = t; best_tri = i; best_id = id; found = true; break; }
+                if (best_tri < 0 || t < best_t || id > best_id) {
+                    best_t = t;
+                    best_tri = i;
+                    best_id = id;
+                }
+            }
+            node = (found || st.empty()) ? kDone : st.pop();
+        }
+        if (busy && node == kDone) {
+            if (SHADOW) {
+                if (best_tri < 0) add_shadow(rad, sc_a, idx, path);
+            } else {
+                hit[idx] = make_uint2(__float_as_uint(best_t), uint32_t(best_tri));
+            }
+            busy = false;
         }
     }
 }

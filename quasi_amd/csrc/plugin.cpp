@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -260,7 +261,7 @@ Q_render_params default_params() {
     p.rng_mode = Q_RNG_REFERENCE;  // plugin.mm:295
     p.integrator = Q_INTEGRATOR_REFERENCE;
     p.engine = Q_ENGINE_MEGAKERNEL;
-    p.bvh_width = 2;
+    p.bvh_width = 4;
     p.samples_per_pass = 0;
     p.block_size = 0;
     p.flags = 0;
@@ -625,7 +626,7 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
         hipDeviceProp_t prop{};
         const bool have_prop = hipGetDeviceProperties(&prop, dev) == hipSuccess;
         std::snprintf(st->arch, sizeof st->arch, "%s", have_prop ? prop.gcnArchName : "unknown");
-        st->trace_cfg = TraceConfig{2, 256, 64};
+        st->trace_cfg = TraceConfig{4, 256, 1};
         st->trace_blocks = persistent_blocks(dev, st->trace_cfg);
         if (ctx->gpu->queue) {
             st->stream = static_cast<hipStream_t>(ctx->gpu->queue);
@@ -773,17 +774,26 @@ Q_EXPORT int32_t Q_plugin_set_scene(Q_plugin_handle* handle, const Q_scene_desc*
     }
 }
 
+// Structs grow at the end; a caller built against an older header passes its
+// smaller struct_size and reads / writes only the fields it knows.
+constexpr size_t kParamsMinSize = offsetof(Q_render_params, bvh_builder);
+constexpr size_t kStatsMinSize = offsetof(Q_render_stats, bvh_builder);
+
 Q_EXPORT int32_t Q_plugin_get_params(Q_plugin_handle* handle, Q_render_params* out) {
-    if (!handle || !out || out->struct_size < sizeof(Q_render_params)) return Q_EXT_ERR_ARGUMENT;
-    *out = as_state(handle)->params;
+    if (!handle || !out || out->struct_size < kParamsMinSize) return Q_EXT_ERR_ARGUMENT;
+    const uint32_t size = out->struct_size;
+    Q_render_params p = as_state(handle)->params;
+    p.struct_size = size;
+    std::memcpy(out, &p, std::min<size_t>(size, sizeof p));
     return Q_EXT_OK;
 }
 
 Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_params* in) {
-    if (!handle || !in || in->struct_size < sizeof(Q_render_params)) return Q_EXT_ERR_ARGUMENT;
+    if (!handle || !in || in->struct_size < kParamsMinSize) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
     try {
-        Q_render_params p = *in;
+        Q_render_params p{};
+        std::memcpy(&p, in, std::min<size_t>(in->struct_size, sizeof p));
         p.struct_size = sizeof(Q_render_params);
         if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > kMaxDepth ||
             p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT ||
@@ -791,8 +801,8 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
             log_msg(st, "set_params: unsupported parameter combination");
             return Q_EXT_ERR_UNSUPPORTED;
         }
-        if (p.bvh_width == 0) p.bvh_width = 2;
-        const TraceConfig cfg{p.bvh_width, p.block_size ? p.block_size : 256u, p.queue_chunk ? p.queue_chunk : 64u};
+        if (p.bvh_width == 0) p.bvh_width = 4;
+        const TraceConfig cfg{p.bvh_width, p.block_size ? p.block_size : 256u, p.queue_chunk ? p.queue_chunk : 1u};
         if (!trace_config_supported(cfg)) {
             log_msg(st, "set_params: no traversal kernel for width %u, block %u, chunk %u", cfg.width, cfg.block,
                     cfg.chunk);
@@ -826,11 +836,12 @@ Q_EXPORT int32_t Q_plugin_readback_ids(Q_plugin_handle* handle, uint32_t* out, u
 }
 
 Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out) {
-    if (!handle || !out || out->struct_size < sizeof(Q_render_stats)) return Q_EXT_ERR_ARGUMENT;
+    if (!handle || !out || out->struct_size < kStatsMinSize) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
     (void)hipSetDevice(st->device);
     if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
     harvest_events(st);
+    const uint32_t out_size = out->struct_size;
     Counters c{};
     if (!hip_ok(st, hipMemcpy(&c, st->counters.ptr, sizeof c, hipMemcpyDeviceToHost), "counter readback"))
         return Q_EXT_ERR_DEVICE;
@@ -859,7 +870,8 @@ Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out
         s.kernels[k].launches = st->launches[k];
         s.kernels[k].total_ms = st->kernel_ms[k];
     }
-    *out = s;
+    s.struct_size = out_size;
+    std::memcpy(out, &s, std::min<size_t>(out_size, sizeof s));
     return Q_EXT_OK;
 }
 

@@ -112,7 +112,7 @@ struct plugin_state {
 
     // wavefront engine
     WaveBuffers wave;
-    TraceConfig trace_cfg{2, 256, 64};
+    TraceConfig trace_cfg{4, 256, 1};
     uint32_t trace_blocks = 0;
     hipEvent_t pending_start = nullptr;
     int num_cus = 256;

@@ -175,7 +175,10 @@ int main(int argc, char** argv) {
         }
         Q_render_params p{};
         p.struct_size = sizeof p;
-        if ((*get)(plugin.handle(), &p) != Q_EXT_OK) return EXIT_FAILURE;
+        if ((*get)(plugin.handle(), &p) != Q_EXT_OK) {
+            std::fprintf(stderr, "Q_plugin_get_params failed\n");
+            return EXIT_FAILURE;
+        }
         if (o.spp > 0) p.spp_per_render = uint32_t(o.spp);
         if (o.depth > 0) p.max_depth = uint32_t(o.depth);
         if (o.rr_start >= 0) p.rr_start = uint32_t(o.rr_start);

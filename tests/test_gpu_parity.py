@@ -266,3 +266,28 @@ def test_unsupported_trace_config_is_rejected():
         with pytest.raises(RuntimeError):
             r.set_params(queue_chunk=100)
         r.set_params(bvh_width=4, block_size=128, queue_chunk=256)
+
+
+def test_params_and_stats_accept_older_struct_sizes():
+    """A caller built against the header before the bvh_builder fields passes
+    the older struct sizes; the plugin reads and writes only those bytes."""
+    import ctypes as C
+    old_params = abi.Q_render_params.bvh_builder.offset
+    old_stats = abi.Q_render_stats.bvh_builder.offset
+    with HipRenderer(16, 16) as r:
+        p = abi.Q_render_params()
+        p.struct_size = old_params
+        p.bvh_builder = 77                      # beyond the old size: must stay untouched
+        assert r._get_params(r.plugin.handle, C.byref(p)) == abi.Q_EXT_OK
+        assert p.struct_size == old_params and p.bvh_builder == 77 and p.max_depth == 5
+        p.max_depth = 3
+        assert r._set_params(r.plugin.handle, C.byref(p)) == abi.Q_EXT_OK   # bvh_builder read as 0 (auto)
+        q = r.get_params()
+        assert q.max_depth == 3 and q.bvh_builder == 0
+        s = abi.Q_render_stats()
+        s.struct_size = old_stats
+        s.bvh_sah_cost = -1.0
+        assert r._get_stats(r.plugin.handle, C.byref(s)) == abi.Q_EXT_OK
+        assert s.bvh_sah_cost == -1.0 and s.num_triangles == 32
+        p.struct_size = old_params - 4
+        assert r._get_params(r.plugin.handle, C.byref(p)) == abi.Q_EXT_ERR_ARGUMENT
