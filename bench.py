@@ -194,6 +194,11 @@ def bench_adjoint(args):
             r.render(1)
             if mesh is target:
                 T = r.readback_aov()["beauty"]
+        # the forward at its fastest traversal configuration for this scene
+        tune = r.autotune(abi.Q_AUTOTUNE_SINGLE, probe_spp=spp, flags=abi.Q_AUTOTUNE_NO_STORE)
+        best = tune.entries[tune.best]
+        r.set_camera(init.camera["position"], init.camera["target"], init.camera["up"], init.camera["fov"])
+        r.render(1)
         r.adjoint(T, spp, mode=abi.Q_ADJOINT_L2_TARGET)        # warm-up
         fwd_wall, fwd_dev, adj_wall, adj_dev = [], [], [], []
         for _ in range(args.adjoint_runs):
@@ -219,7 +224,9 @@ def bench_adjoint(args):
            "ratio_device": round(med(adj_dev) / med(fwd_dev), 4),
            "loss": res.loss, "paths": int(res.paths), "forward_rays": int(st.rays_camera + st.rays_extension
                                                                          + st.rays_shadow),
-           "grad_albedo_l1": float(np.abs(ga).sum()), "grad_vertices_l1": float(np.abs(gv).sum())}
+           "grad_albedo_l1": float(np.abs(ga).sum()), "grad_vertices_l1": float(np.abs(gv).sum()),
+           "forward_traversal": {"bvh_width": best.bvh_width, "block_size": best.block_size,
+                                 "queue_chunk": best.queue_chunk}}
     log(f"adjoint: forward {out['forward_ms']:.2f} ms, adjoint {out['adjoint_ms']:.2f} ms "
         f"(device {out['forward_device_ms']:.2f} / {out['adjoint_device_ms']:.2f} ms), ratio {out['ratio']:.3f}")
     return out
