@@ -164,6 +164,18 @@ __device__ __forceinline__ void slab(float lx, float hx, float ly, float hy, flo
     tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * kSlabWiden;
 }
 
+// slab() with the entry (e*) and exit (f*) planes of each axis already chosen
+// from the sign of the direction: (e - o) * inv <= (f - o) * inv, so the
+// values are those slab() takes the min and the max of, bit for bit.
+__device__ __forceinline__ void slab_sel(float ex, float fx, float ey, float fy, float ez, float fz, const Ray& r,
+                                         const f3& inv, float tmin, float tmax, float& tn, float& tf) {
+    const float x0 = (ex - r.o.x) * inv.x, x1 = (fx - r.o.x) * inv.x;
+    const float y0 = (ey - r.o.y) * inv.y, y1 = (fy - r.o.y) * inv.y;
+    const float z0 = (ez - r.o.z) * inv.z, z1 = (fz - r.o.z) * inv.z;
+    tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, tmin));
+    tf = fminf(fminf(x1, y1), fminf(z1, tmax)) * kSlabWiden;
+}
+
 // Closest hit over the pair-node BVH2.  Children are visited near first.  On
 // equal t the larger input index wins (the reference's linear loop,
 // plugin.mm:204-209 with the inclusive `t > t_max` test of plugin.mm:177).

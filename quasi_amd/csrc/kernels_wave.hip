@@ -273,6 +273,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     int node = kDone;
     bool busy = false, exhausted = false;
     uint32_t idx = 0, path = 0;
+    uint32_t sel_x = 0, sel_y = 32, sel_z = 64;
     const float4* nodes = nullptr;
     const float4* tris = nullptr;
     for (;;) {
@@ -295,6 +296,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                     tris = static_cast<const float4*>(sc.tris);
                     r = Ray{xyz(o), xyz(d)};
                     inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+                    // byte offsets of the entry planes in a BVH4 node: lo (or hi
+                    // when the direction component is negative) of each axis
+                    sel_x = inv.x < 0.f ? 16u : 0u;
+                    sel_y = inv.y < 0.f ? 48u : 32u;
+                    sel_z = inv.z < 0.f ? 80u : 64u;
                     best_t = SHADOW ? o.w : kTFar;
                     best_tri = -1;
                     best_id = 0u;
@@ -313,14 +319,22 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
         // descend through internal nodes
         while (busy && node >= 0) {
             if (WIDTH == 4) {
-                const float4* nd = nodes + 8 * node;
-                const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-                const int4 ch = *reinterpret_cast<const int4*>(nd + 6);
+                // entry and exit planes chosen by the direction's signs: the same
+                // values slab() takes the min and max of, without the min/max
+                const char* nb = reinterpret_cast<const char*>(nodes);
+                const uint32_t off = uint32_t(node) * 128u;
+                const float4 ex = *reinterpret_cast<const float4*>(nb + (off + sel_x));
+                const float4 fx = *reinterpret_cast<const float4*>(nb + (off + (sel_x ^ 16u)));
+                const float4 ey = *reinterpret_cast<const float4*>(nb + (off + sel_y));
+                const float4 fy = *reinterpret_cast<const float4*>(nb + (off + (sel_y ^ 16u)));
+                const float4 ez = *reinterpret_cast<const float4*>(nb + (off + sel_z));
+                const float4 fz = *reinterpret_cast<const float4*>(nb + (off + (sel_z ^ 16u)));
+                const int4 ch = *reinterpret_cast<const int4*>(nb + (off + 96u));
                 float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
-                slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, inv, tmin, best_t, t0n, t0f);
-                slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, inv, tmin, best_t, t1n, t1f);
-                slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, inv, tmin, best_t, t2n, t2f);
-                slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, inv, tmin, best_t, t3n, t3f);
+                slab_sel(ex.x, fx.x, ey.x, fy.x, ez.x, fz.x, r, inv, tmin, best_t, t0n, t0f);
+                slab_sel(ex.y, fx.y, ey.y, fy.y, ez.y, fz.y, r, inv, tmin, best_t, t1n, t1f);
+                slab_sel(ex.z, fx.z, ey.z, fy.z, ez.z, fz.z, r, inv, tmin, best_t, t2n, t2f);
+                slab_sel(ex.w, fx.w, ey.w, fy.w, ez.w, fz.w, r, inv, tmin, best_t, t3n, t3f);
                 constexpr float kMissKey = 3.0e38f;
                 float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
                 float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
