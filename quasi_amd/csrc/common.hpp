@@ -97,7 +97,7 @@ struct SceneDesc {
     uint32_t width, height;
     uint32_t path_begin, pix_begin;
     uint32_t seed;
-    uint32_t pad;
+    uint32_t num_tris;
 };
 
 // Everything one sample pass needs (passed by value as a kernel argument).

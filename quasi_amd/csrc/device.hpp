@@ -131,10 +131,14 @@ struct ShortStack {
 // Moller-Trumbore against BVH-order triangle i; true with t when the hit lies
 // in [tmin, tmax].  Inclusive edges.
 __device__ __forceinline__ bool hit_triangle(const float4* __restrict__ tris, int i, const Ray& r, float tmin,
-                                             float tmax, float& t_out, uint32_t& id) {
-    const float4 t0 = tris[3 * i + 0];
-    const float4 t1 = tris[3 * i + 1];
-    const float4 t2 = tris[3 * i + 2];
+                                             float tmax, float& t_out, uint32_t& id);
+
+// The same test on the triangle record at tri (its three float4s).
+__device__ __forceinline__ bool hit_triangle_rec(const float4* __restrict__ tri, const Ray& r, float tmin,
+                                                 float tmax, float& t_out, uint32_t& id) {
+    const float4 t0 = tri[0];
+    const float4 t1 = tri[1];
+    const float4 t2 = tri[2];
     const f3 e1 = xyz(t1), e2 = xyz(t2);
     const f3 pv = cross(r.d, e2);
     const float det = dot(e1, pv);
@@ -151,6 +155,11 @@ __device__ __forceinline__ bool hit_triangle(const float4* __restrict__ tris, in
     t_out = t;
     id = __float_as_uint(t0.w);
     return true;
+}
+
+__device__ __forceinline__ bool hit_triangle(const float4* __restrict__ tris, int i, const Ray& r, float tmin,
+                                             float tmax, float& t_out, uint32_t& id) {
+    return hit_triangle_rec(tris + 3 * i, r, tmin, tmax, t_out, id);
 }
 
 // Slab distances of one box: returns (t_near, widened t_far) clipped to

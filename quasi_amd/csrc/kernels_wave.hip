@@ -245,11 +245,16 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // waits for its slowest ray.  Every fetch is one atomic per wave on the
 // bounce's cursor.  SHADOW selects the any-hit shadow queue; otherwise the
 // closest-hit queue.  Same visit order and tie rule as traverse2 / traverse4.
+// OFF32 (one scene whose node and triangle arrays each span less than 4 GiB):
+// nodes and triangles are addressed as the scene's base pointer, uniform over
+// the grid, plus a 32-bit byte offset, which the loads take as they are.
 constexpr uint32_t kRefillIdle = 32;   // measured best of 4..48 on the 1M soup
 constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
-template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH>
+template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
+    static_assert(!(BATCH && OFF32), "a batch addresses each lane's scene through its own pointers");
+    constexpr uint32_t kNodeShift = WIDTH == 4 ? 7u : 6u;   // 128-B BVH4 / 64-B BVH2 nodes
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
                                       gridDim.x * BLOCK, 0};
@@ -271,11 +276,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     int best_tri = -1;
     uint32_t best_id = 0u;
     int node = kDone;
+    int leaf = 0;   // postponed leaf reference (leaf references are negative; 0 = none)
     bool busy = false, exhausted = false;
     uint32_t idx = 0, path = 0;
+    // byte offsets in a BVH4 node of each axis's entry plane (lo, or hi when
+    // the direction component is negative); the exit plane is at offset ^ 16
     uint32_t sel_x = 0, sel_y = 32, sel_z = 64;
-    const float4* nodes = nullptr;
-    const float4* tris = nullptr;
+    // a batch's lanes may trace different scenes; one scene's arrays are uniform
+    const char* nodes = BATCH ? nullptr : reinterpret_cast<const char*>(scene_nodes<WIDTH, false>(P, 0));
+    const char* tris = BATCH ? nullptr : static_cast<const char*>(P.scene0.tris);
     for (;;) {
         // refill idle lanes (wave-uniform: every lane reaches the ballots)
         const unsigned long long idle = __ballot(!busy && !exhausted);
@@ -292,12 +301,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                     path = __float_as_uint(d.w);
                     const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
                     const SceneDesc& sc = scene_at<BATCH>(P, sid);
-                    nodes = scene_nodes<WIDTH, BATCH>(P, sid);
-                    tris = static_cast<const float4*>(sc.tris);
+                    if (BATCH) {
+                        nodes = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
+                        tris = static_cast<const char*>(sc.tris);
+                    }
                     r = Ray{xyz(o), xyz(d)};
                     inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
-                    // byte offsets of the entry planes in a BVH4 node: lo (or hi
-                    // when the direction component is negative) of each axis
                     sel_x = inv.x < 0.f ? 16u : 0u;
                     sel_y = inv.y < 0.f ? 48u : 32u;
                     sel_z = inv.z < 0.f ? 80u : 64u;
@@ -306,6 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                     best_id = 0u;
                     st.sp = 0;
                     node = sc.num_nodes ? 0 : kDone;
+                    leaf = 0;
                     busy = true;
                 } else {
                     exhausted = true;
@@ -316,20 +326,30 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             if (__ballot(!exhausted) == 0ull) break;
             continue;
         }
-        // descend through internal nodes
-        while (busy && node >= 0) {
+        // descend through internal nodes.  A lane that reaches a leaf keeps it
+        // in `leaf` and goes on descending from its stack while lanes of the
+        // wave without a leaf are still looking for one (postponed leaves:
+        // lanes that would idle do node work instead)
+        for (;;) {
+            const bool desc = busy && node >= 0;
+            if (__ballot(desc && leaf == 0) == 0ull) break;
+            if (!desc) continue;
+            // the node's record: base + 32-bit offset (OFF32) or a 64-bit address
+            const uint32_t off = uint32_t(node) << kNodeShift;
+            const char* rec = nodes + (OFF32 ? size_t(0) : (size_t(uint32_t(node)) << kNodeShift));
+            auto ld = [&](uint32_t o) -> float4 {
+                return OFF32 ? *reinterpret_cast<const float4*>(nodes + (off + o))
+                             : *reinterpret_cast<const float4*>(rec + o);
+            };
             if (WIDTH == 4) {
                 // entry and exit planes chosen by the direction's signs: the same
                 // values slab() takes the min and max of, without the min/max
-                const char* nb = reinterpret_cast<const char*>(nodes);
-                const uint32_t off = uint32_t(node) * 128u;
-                const float4 ex = *reinterpret_cast<const float4*>(nb + (off + sel_x));
-                const float4 fx = *reinterpret_cast<const float4*>(nb + (off + (sel_x ^ 16u)));
-                const float4 ey = *reinterpret_cast<const float4*>(nb + (off + sel_y));
-                const float4 fy = *reinterpret_cast<const float4*>(nb + (off + (sel_y ^ 16u)));
-                const float4 ez = *reinterpret_cast<const float4*>(nb + (off + sel_z));
-                const float4 fz = *reinterpret_cast<const float4*>(nb + (off + (sel_z ^ 16u)));
-                const int4 ch = *reinterpret_cast<const int4*>(nb + (off + 96u));
+                const float4 ex = ld(sel_x), fx = ld(sel_x ^ 16u);
+                const float4 ey = ld(sel_y), fy = ld(sel_y ^ 16u);
+                const float4 ez = ld(sel_z), fz = ld(sel_z ^ 16u);
+                const float4 chf = ld(96u);
+                const int4 ch = make_int4(__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                                          __float_as_int(chf.w));
                 float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
                 slab_sel(ex.x, fx.x, ey.x, fy.x, ez.x, fz.x, r, inv, tmin, best_t, t0n, t0f);
                 slab_sel(ex.y, fx.y, ey.y, fy.y, ez.y, fz.y, r, inv, tmin, best_t, t1n, t1f);
@@ -349,16 +369,25 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 if (k0 == kMissKey) {
                     node = st.empty() ? kDone : st.pop();
                 } else {
-                    if (k3 != kMissKey) st.push(c3);
-                    if (k2 != kMissKey) st.push(c2);
-                    if (k1 != kMissKey) st.push(c1);
+                    // push the hit children after the nearest, farthest first
+                    const int np = int(k1 != kMissKey) + int(k2 != kMissKey) + int(k3 != kMissKey);
+                    if (st.sp + 3 <= kShortStack) {
+                        // three unconditional LDS stores; slots above sp + np are
+                        // overwritten before they are read
+                        int* top = st.lds + st.sp * BLOCK;
+                        top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
+                        top[BLOCK] = np == 3 ? c2 : c1;
+                        top[2 * BLOCK] = c1;
+                        st.sp += np;
+                    } else {
+                        if (k3 != kMissKey) st.push(c3);
+                        if (k2 != kMissKey) st.push(c2);
+                        if (k1 != kMissKey) st.push(c1);
+                    }
                     node = c0;
                 }
             } else {
-                const float4 a = nodes[4 * node + 0];
-                const float4 bb = nodes[4 * node + 1];
-                const float4 c = nodes[4 * node + 2];
-                const float4 dd = nodes[4 * node + 3];
+                const float4 a = ld(0u), bb = ld(16u), c = ld(32u), dd = ld(48u);
                 const int cl = __float_as_int(dd.x);
                 const int cr = __float_as_int(dd.y);
                 float tnl, tfl, tnr, tfr;
@@ -376,17 +405,25 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                     node = st.empty() ? kDone : st.pop();
                 }
             }
+            if (leaf == 0 && node < 0 && node != kDone) {
+                leaf = node;
+                node = st.empty() ? kDone : st.pop();
+            }
         }
-        // the lanes at a leaf test its triangles
-        if (busy && node != kDone) {
-            const uint32_t v = ~uint32_t(node);
+        // the lanes holding a leaf test its triangles; a lane whose descent
+        // stopped at a second leaf takes that one next
+        if (busy && leaf != 0) {
+            const uint32_t v = ~uint32_t(leaf);
             const int first = int(v >> 2);
             const int count = int(v & 3u) + 1;
             bool found = false;
-            for (int i = first; i < first + count; ++i) {
+            uint32_t toff = uint32_t(first) * 48u;   // 48-B triangle records
+            const char* trec = tris + (OFF32 ? size_t(0) : size_t(uint32_t(first)) * 48u);
+            for (int i = first; i < first + count; ++i, toff += 48u, trec += OFF32 ? 0 : 48) {
                 float t;
                 uint32_t id;
-                if (!hit_triangle(tris, i, r, tmin, best_t, t, id)) continue;
+                const float4* tr = reinterpret_cast<const float4*>(OFF32 ? tris + toff : trec);
+                if (!hit_triangle_rec(tr, r, tmin, best_t, t, id)) continue;
                 if (SHADOW) { best_t = t; best_tri = i; best_id = id; found = true; break; }
                 if (best_tri < 0 || t < best_t || id > best_id) {
                     best_t = t;
@@ -394,9 +431,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                     best_id = id;
                 }
             }
-            node = (found || st.empty()) ? kDone : st.pop();
+            if (found) {
+                node = kDone;
+                leaf = 0;
+            } else if (node < 0 && node != kDone) {
+                leaf = node;
+                node = st.empty() ? kDone : st.pop();
+            } else {
+                leaf = 0;
+            }
         }
-        if (busy && node == kDone) {
+        if (busy && node == kDone && leaf == 0) {
             if (SHADOW) {
                 if (best_tri < 0) add_shadow(rad, sc_a, idx, path);
             } else {
@@ -470,10 +515,24 @@ int occupancy_of(K kernel, int block) {
 template <int WIDTH, int BLOCK, int CHUNK>
 struct TraceVariant {
     static constexpr int kStatic = CHUNK == 1 ? 64 : CHUNK;
+    // one scene whose node and triangle arrays each span less than 4 GiB
+    static bool off32(const PassParams& p) {
+        const uint64_t node_bytes = uint64_t(WIDTH == 4 ? p.num_nodes4 : p.num_nodes) * (WIDTH == 4 ? 128u : 64u);
+        return p.num_scenes == 0 && node_bytes < (1ull << 32) && uint64_t(p.scene0.num_tris) * 48u < (1ull << 32);
+    }
+    template <bool SHADOW, bool BATCH>
+    static void dynamic(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        if (BATCH)
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, true, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        else if (off32(p))
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, false, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        else
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, false, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+    }
     template <bool BATCH>
     static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
         if (CHUNK == 1)
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, false, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            dynamic<false, BATCH>(p, s, b, blocks, st);
         else
             hipLaunchKernelGGL((k_extend<WIDTH, BLOCK, kStatic, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         return hipGetLastError();
@@ -481,19 +540,22 @@ struct TraceVariant {
     template <bool BATCH>
     static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
         if (CHUNK == 1)
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, true, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            dynamic<true, BATCH>(p, s, b, blocks, st);
         else
             hipLaunchKernelGGL((k_shadow<WIDTH, BLOCK, kStatic, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         return hipGetLastError();
     }
+    // Resident blocks per CU of the single-scene kernels.  The grid is sized
+    // by it; kernels with fewer resident blocks (a batch's) start the rest as
+    // the first ones finish, which is harmless: no block waits for another.
     static int occupancy() {
         int a, c;
         if (CHUNK == 1) {
-            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, true>, BLOCK);
-            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, true>, BLOCK);
+            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, false, true>, BLOCK);
+            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, false, true>, BLOCK);
         } else {
-            a = occupancy_of(k_extend<WIDTH, BLOCK, kStatic, true>, BLOCK);
-            c = occupancy_of(k_shadow<WIDTH, BLOCK, kStatic, true>, BLOCK);
+            a = occupancy_of(k_extend<WIDTH, BLOCK, kStatic, false>, BLOCK);
+            c = occupancy_of(k_shadow<WIDTH, BLOCK, kStatic, false>, BLOCK);
         }
         return a < c ? a : c;
     }

@@ -554,6 +554,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
         sd.path_begin = 0;
         sd.pix_begin = 0;
         sd.seed = rp.seed;
+        sd.num_tris = st->num_tris;
         p.num_scenes = 0;
         p.total_pixels = st->width * st->height;
         const uint32_t per_pass = wave_samples_per_pass(st);

@@ -291,6 +291,7 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
         d.pix_begin = pix;
         d.path_begin = pix;
         d.seed = sc.seed;
+        d.num_tris = uint32_t(h.tri4.size() / 3);
         to += h.tri4.size();
         mo += h.mat4.size();
         no += h.nodes.size();
