@@ -248,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // OFF32 (one scene whose node and triangle arrays each span less than 4 GiB):
 // nodes and triangles are addressed as the scene's base pointer, uniform over
 // the grid, plus a 32-bit byte offset, which the loads take as they are.
-constexpr uint32_t kRefillIdle = 32;   // measured best of 4..48 on the 1M soup
+constexpr uint32_t kRefillIdle = 40;   // measured best of 8..48 on the 1M soup
 constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32>
