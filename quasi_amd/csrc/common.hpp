@@ -60,9 +60,25 @@ struct Counters {
 
 constexpr uint32_t kMaxDepth = 16;
 
+// Wavefront queues are split into kQueueSegments segments of seg_cap slots:
+// segment j of every queue holds slots [j * seg_cap, j * seg_cap + count_j),
+// and each segment has its own length and work cursor, kCounterStride words
+// (one 128-B line) apart.  A single counter per queue would serialise every
+// wave's atomic on one address (about 11 ns each on MI355X); 64 addresses
+// take them 50 times faster.  Paths keep their segment from bounce to bounce:
+// the continuation and the shadow ray of a ray of segment j go to segment j of
+// the next queues, so no segment can overflow.
+constexpr uint32_t kQueueSegments = 64;
+constexpr uint32_t kCounterStride = 32;
+
+// Word index of the counter of segment `seg` in counter group `group`.
+constexpr uint32_t queue_counter(uint32_t group, uint32_t seg) {
+    return (group * kQueueSegments + seg) * kCounterStride;
+}
+
 // Wavefront path state.  Per-path arrays are indexed by the path id
 // p = sample_in_pass * W*H + pixel; per-slot arrays by the position in the
-// compacted queue of the current bounce (double buffered by bounce parity).
+// segmented queue of the current bounce (double buffered by bounce parity).
 struct WaveState {
     void* ray_o[2];        // float4 per slot: origin.xyz, w = path id (bits)
     void* ray_d[2];        // float4 per slot: direction.xyz
@@ -75,10 +91,11 @@ struct WaveState {
     void* sh_o;            // float4 per shadow slot: origin.xyz, w = t max
     void* sh_d;            // float4 per shadow slot: direction.xyz, w = path id (bits)
     void* sh_c;            // float4 per shadow slot: unoccluded contribution
-    uint32_t* counts;      // [kMaxDepth + 1] extension queue length per bounce
-    uint32_t* shadow_counts;  // [kMaxDepth] shadow queue length per bounce
-    uint32_t* fetch;       // [2 * kMaxDepth] work-fetch cursors (extend, shadow) per bounce
+    uint32_t* counts;      // extension queue lengths: group b (0..kMaxDepth), per segment
+    uint32_t* shadow_counts;  // shadow queue lengths: group b (0..kMaxDepth-1), per segment
+    uint32_t* fetch;       // work cursors: group 2b (extend) and 2b+1 (shadow), per segment
     uint32_t paths;        // paths in this pass
+    uint32_t seg_cap;      // slots per queue segment: ceil(paths / kQueueSegments)
 };
 
 // One scene of a wavefront pass.  A single-scene pass carries it inline

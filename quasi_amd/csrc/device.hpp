@@ -480,14 +480,47 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     return base + lanes_below(mask);
 }
 
-// Wave-uniform work fetch: lane 0 reserves the next CHUNK items of a queue
-// of n; the wave then walks them 64 at a time.
-template <int CHUNK>
-__device__ __forceinline__ bool fetch_chunk(uint32_t* cursor, uint32_t n, uint32_t& base) {
-    uint32_t b = 0;
-    if ((threadIdx.x & 63u) == 0u) b = atomicAdd(cursor, uint32_t(CHUNK));
-    base = __shfl(b, 0, 64);
-    return base < n;
-}
+// Wave-uniform work fetch over a segmented queue (kQueueSegments segments of
+// `cap` slots, each with its length and cursor kCounterStride words apart).
+// A wave starts at its own segment and moves on to the next one when that is
+// exhausted; every lane of the wave calls take() together.
+struct SegmentFetch {
+    const uint32_t* counts;   // segment lengths of the queue
+    uint32_t* cursors;        // segment cursors of the queue
+    uint32_t cap, seg, tried, seg_n;
+
+    __device__ __forceinline__ void init(const uint32_t* c, uint32_t* cur, uint32_t cap_, uint32_t start) {
+        counts = c;
+        cursors = cur;
+        cap = cap_;
+        seg = start % kQueueSegments;
+        tried = 0;
+        seg_n = counts[seg * kCounterStride];
+    }
+    __device__ __forceinline__ bool live() const { return tried < kQueueSegments; }
+    __device__ __forceinline__ void advance() {
+        seg = seg + 1 == kQueueSegments ? 0u : seg + 1;
+        if (++tried < kQueueSegments) seg_n = counts[seg * kCounterStride];
+    }
+    // Reserves up to `want` (1..64 or a chunk) consecutive slots; returns how
+    // many (0 once every segment is exhausted) and the first slot's index.
+    __device__ __forceinline__ uint32_t take(uint32_t want, uint32_t& first) {
+        while (tried < kQueueSegments) {
+            if (seg_n != 0u) {
+                uint32_t base = 0;
+                if ((threadIdx.x & 63u) == 0u) base = atomicAdd(&cursors[seg * kCounterStride], want);
+                base = __builtin_amdgcn_readlane(base, 0);
+                if (base < seg_n) {
+                    const uint32_t got = seg_n - base < want ? seg_n - base : want;
+                    first = seg * cap + base;
+                    if (got < want) advance();
+                    return got;
+                }
+            }
+            advance();
+        }
+        return 0u;
+    }
+};
 
 }  // namespace quasi_hip

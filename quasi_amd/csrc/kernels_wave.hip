@@ -57,10 +57,19 @@ __device__ __forceinline__ uint32_t pass_sample_begin(const PassParams& P) {
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
-    if (blockIdx.x == 0 && threadIdx.x < 2 * kMaxDepth + 1) {
-        if (threadIdx.x <= kMaxDepth) S.counts[threadIdx.x] = threadIdx.x == 0 ? S.paths : 0u;
-        if (threadIdx.x < kMaxDepth) S.shadow_counts[threadIdx.x] = 0u;
-        if (threadIdx.x < 2 * kMaxDepth) S.fetch[threadIdx.x] = 0u;
+    // queue lengths and cursors of the pass: the camera rays fill segment j
+    // with paths [j * seg_cap, ...) at slots equal to their path ids
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 2 * kMaxDepth * kQueueSegments;
+         e += gridDim.x * blockDim.x) {
+        const uint32_t seg = e % kQueueSegments;
+        const uint64_t lo = uint64_t(seg) * S.seg_cap;
+        if (e < (kMaxDepth + 1) * kQueueSegments) {
+            uint32_t v = 0u;
+            if (e < kQueueSegments && lo < S.paths) v = uint32_t(S.paths - lo < S.seg_cap ? S.paths - lo : S.seg_cap);
+            S.counts[e * kCounterStride] = v;
+        }
+        if (e < kMaxDepth * kQueueSegments) S.shadow_counts[e * kCounterStride] = 0u;
+        S.fetch[e * kCounterStride] = 0u;
     }
     const uint32_t s0 = pass_sample_begin(P);
     float4* ro = static_cast<float4*>(S.ray_o[0]);
@@ -90,6 +99,18 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     }
 }
 
+__device__ __forceinline__ uint32_t wave_global() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+
+// Adds the length of a segmented queue to a ray counter (first wave of the
+// grid; one segment per lane).
+static_assert(kQueueSegments == 64, "one segment per lane of a wave");
+__device__ __forceinline__ void count_rays(const uint32_t* lengths, unsigned long long* counter) {
+    if (blockIdx.x == 0 && threadIdx.x < 64u) {
+        const uint32_t total = wave_sum(lengths[threadIdx.x * kCounterStride]);
+        if (threadIdx.x == 0u && total) atomicAdd(counter, (unsigned long long)total);
+    }
+}
+
 template <int WIDTH, bool BATCH>
 __device__ __forceinline__ const float4* scene_nodes(const PassParams& P, uint32_t sid) {
     const SceneDesc& sc = scene_at<BATCH>(P, sid);
@@ -103,17 +124,17 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uin
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
                                       gridDim.x * BLOCK, 0};
-    const uint32_t n = S.counts[b];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n)
-        atomicAdd(b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension, (unsigned long long)n);
+    count_rays(S.counts + queue_counter(b, 0), b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension);
     const float4* __restrict__ ro = static_cast<const float4*>(S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(S.ray_d[b & 1u]);
     uint2* hit = static_cast<uint2*>(S.hit);
-    uint32_t base;
-    while (fetch_chunk<CHUNK>(&S.fetch[2 * b], n, base)) {
+    SegmentFetch f;
+    f.init(S.counts + queue_counter(b, 0), S.fetch + queue_counter(2 * b, 0), S.seg_cap, wave_global());
+    uint32_t base = 0, got;
+    while ((got = f.take(uint32_t(CHUNK), base)) != 0u) {
         for (uint32_t j = 0; j < uint32_t(CHUNK); j += 64) {
             const uint32_t i = base + j + lane_id();
-            if (i < n) {
+            if (j + lane_id() < got) {
                 const float4 o = ro[i], d = rd[i];
                 const uint32_t sid = BATCH ? __float_as_uint(d.w) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
@@ -132,9 +153,20 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uin
     }
 }
 
-template <int BLOCK, bool BATCH>
+// One vertex per hit of the bounce's queue (shade_hit); compacts continuing
+// paths and shadow rays into the next queues.  FIRST: bounce 0, which also
+// records the first-hit AOVs (a separate instantiation keeps their registers
+// out of the other bounces).
+template <int BLOCK, bool BATCH, bool FIRST>
 __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint32_t b) {
-    const uint32_t n = S.counts[b];
+    // each wave serves one queue segment; the grid is a multiple of the
+    // segment count (in waves), so every segment has the same number of waves
+    const uint32_t waves = gridDim.x * (BLOCK / 64);
+    const uint32_t seg = wave_global() % kQueueSegments;
+    const uint32_t n = S.counts[queue_counter(b, seg)];
+    const uint32_t slot0 = seg * S.seg_cap;
+    uint32_t* next_len = S.counts + queue_counter(b + 1, seg);
+    uint32_t* shadow_len = S.shadow_counts + queue_counter(b, seg);
     const float4* __restrict__ ro = static_cast<const float4*>(S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(S.ray_d[b & 1u]);
     float4* no = static_cast<float4*>(S.ray_o[(b + 1) & 1u]);
@@ -147,14 +179,13 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
     float4* so_a = static_cast<float4*>(S.sh_o);
     float4* sd_a = static_cast<float4*>(S.sh_d);
     float4* sc_a = static_cast<float4*>(S.sh_c);
-    // wave-uniform grid-stride loop so that every lane reaches the ballots
-    const uint32_t wave_base = blockIdx.x * BLOCK + (threadIdx.x & ~63u);
-    for (uint32_t wb = wave_base; wb < n; wb += gridDim.x * BLOCK) {
-        const uint32_t i = wb + lane_id();
+    // wave-uniform loop over the segment so that every lane reaches the ballots
+    for (uint32_t wb = (wave_global() / kQueueSegments) * 64u; wb < n; wb += (waves / kQueueSegments) * 64u) {
+        const uint32_t i = slot0 + wb + lane_id();
         bool cont = false, shadow = false;
         ShadeOut so;
         uint32_t p = 0, sid = 0;
-        if (i < n) {
+        if (wb + lane_id() < n) {
             const float4 o = ro[i], d = rd[i];
             p = __float_as_uint(o.w);
             sid = BATCH ? __float_as_uint(d.w) : 0u;
@@ -168,8 +199,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
                 FirstHit first;
                 shade_hit(P, static_cast<const float4*>(sc.tris), static_cast<const float4*>(sc.mats), sc.lights,
                           sc.num_lights, b, Ray{xyz(o), xyz(d)}, __uint_as_float(h.x), tri, thr, L, rng,
-                          b == 0 ? &first : nullptr, so);
-                if (b == 0) {
+                          FIRST ? &first : nullptr, so);
+                if (FIRST) {
                     a0[p] = make_float4(first.albedo.x, first.albedo.y, first.albedo.z, first.t);
                     a1[p] = make_float4(first.normal.x, first.normal.y, first.normal.z, __uint_as_float(first.id));
                 }
@@ -180,12 +211,12 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
                 shadow = so.shadow;
             }
         }
-        const uint32_t j = wave_append(cont, &S.counts[b + 1]);
+        const uint32_t j = slot0 + wave_append(cont, next_len);
         if (cont) {
             no[j] = make_float4(so.next.o.x, so.next.o.y, so.next.o.z, __uint_as_float(p));
             nd[j] = make_float4(so.next.d.x, so.next.d.y, so.next.d.z, __uint_as_float(sid));
         }
-        const uint32_t k = wave_append(shadow, &S.shadow_counts[b]);
+        const uint32_t k = slot0 + wave_append(shadow, shadow_len);
         if (shadow) {
             so_a[k] = make_float4(so.sray.o.x, so.sray.o.y, so.sray.o.z, so.stmax);
             sd_a[k] = make_float4(so.sray.d.x, so.sray.d.y, so.sray.d.z, __uint_as_float(p));
@@ -209,18 +240,18 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
                                       gridDim.x * BLOCK, 0};
-    const uint32_t n = S.shadow_counts[b];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n)
-        atomicAdd(&P.counters->rays_shadow, (unsigned long long)n);
+    count_rays(S.shadow_counts + queue_counter(b, 0), &P.counters->rays_shadow);
     const float4* __restrict__ so_a = static_cast<const float4*>(S.sh_o);
     const float4* __restrict__ sd_a = static_cast<const float4*>(S.sh_d);
     const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
     float4* rad = static_cast<float4*>(S.radiance);
-    uint32_t base;
-    while (fetch_chunk<CHUNK>(&S.fetch[2 * b + 1], n, base)) {
+    SegmentFetch f;
+    f.init(S.shadow_counts + queue_counter(b, 0), S.fetch + queue_counter(2 * b + 1, 0), S.seg_cap, wave_global());
+    uint32_t base = 0, got;
+    while ((got = f.take(uint32_t(CHUNK), base)) != 0u) {
         for (uint32_t j = 0; j < uint32_t(CHUNK); j += 64) {
             const uint32_t k = base + j + lane_id();
-            if (k < n) {
+            if (j + lane_id() < got) {
                 const float4 o = so_a[k], d = sd_a[k];
                 const uint32_t sid = BATCH ? __float_as_uint(sc_a[k].w) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
@@ -243,8 +274,9 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // triangles together; a lane whose ray has finished takes the next ray from
 // the queue as soon as kRefillIdle lanes of its wave are idle, so a wave never
 // waits for its slowest ray.  Every fetch is one atomic per wave on the
-// bounce's cursor.  SHADOW selects the any-hit shadow queue; otherwise the
-// closest-hit queue.  Same visit order and tie rule as traverse2 / traverse4.
+// cursor of the wave's current queue segment.  SHADOW selects the any-hit
+// shadow queue; otherwise the closest-hit queue.  Same visit order and tie
+// rule as traverse2 / traverse4.
 // OFF32 (one scene whose node and triangle arrays each span less than 4 GiB):
 // nodes and triangles are addressed as the scene's base pointer, uniform over
 // the grid, plus a 32-bit byte offset, which the loads take as they are.
@@ -258,17 +290,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
                                       gridDim.x * BLOCK, 0};
-    const uint32_t n = SHADOW ? S.shadow_counts[b] : S.counts[b];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n)
-        atomicAdd(SHADOW ? &P.counters->rays_shadow
-                         : (b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension),
-                  (unsigned long long)n);
+    const uint32_t* lengths = SHADOW ? S.shadow_counts + queue_counter(b, 0) : S.counts + queue_counter(b, 0);
+    count_rays(lengths, SHADOW ? &P.counters->rays_shadow
+                               : (b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension));
     const float4* __restrict__ ro = static_cast<const float4*>(SHADOW ? S.sh_o : S.ray_o[b & 1u]);
     const float4* __restrict__ rd = static_cast<const float4*>(SHADOW ? S.sh_d : S.ray_d[b & 1u]);
     const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
     uint2* hit = static_cast<uint2*>(S.hit);
     float4* rad = static_cast<float4*>(S.radiance);
-    uint32_t* cursor = &S.fetch[2 * b + (SHADOW ? 1 : 0)];
+    SegmentFetch f;
+    f.init(lengths, S.fetch + queue_counter(2 * b + (SHADOW ? 1 : 0), 0), S.seg_cap, wave_global());
 
     Ray r{mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 1.f)};
     f3 inv = mk(0.f, 0.f, 0.f);
@@ -277,7 +308,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     uint32_t best_id = 0u;
     int node = kDone;
     int leaf = 0;   // postponed leaf reference (leaf references are negative; 0 = none)
-    bool busy = false, exhausted = false;
+    bool busy = false;
     uint32_t idx = 0, path = 0;
     // byte offsets in a BVH4 node of each axis's entry plane (lo, or hi when
     // the direction component is negative); the exit plane is at offset ^ 16
@@ -287,45 +318,43 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     const char* tris = BATCH ? nullptr : static_cast<const char*>(P.scene0.tris);
     for (;;) {
         // refill idle lanes (wave-uniform: every lane reaches the ballots)
-        const unsigned long long idle = __ballot(!busy && !exhausted);
+        const unsigned long long idle = __ballot(!busy);
         const unsigned long long act = __ballot(busy);
-        if (idle && (act == 0ull || uint32_t(__popcll(idle)) >= kRefillIdle)) {
-            uint32_t base = 0;
-            if ((threadIdx.x & 63u) == 0u) base = atomicAdd(cursor, uint32_t(__popcll(idle)));
-            base = __shfl(base, 0, 64);
-            if (!busy && !exhausted) {
-                idx = base + lanes_below(idle);
-                if (idx < n) {
-                    const float4 o = ro[idx];
-                    const float4 d = rd[idx];
-                    path = __float_as_uint(d.w);
-                    const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
-                    const SceneDesc& sc = scene_at<BATCH>(P, sid);
-                    if (BATCH) {
-                        nodes = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
-                        tris = static_cast<const char*>(sc.tris);
-                    }
-                    r = Ray{xyz(o), xyz(d)};
-                    inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
-                    sel_x = inv.x < 0.f ? 16u : 0u;
-                    sel_y = inv.y < 0.f ? 48u : 32u;
-                    sel_z = inv.z < 0.f ? 80u : 64u;
-                    best_t = SHADOW ? o.w : kTFar;
-                    best_tri = -1;
-                    best_id = 0u;
-                    st.sp = 0;
-                    node = sc.num_nodes ? 0 : kDone;
-                    leaf = 0;
-                    busy = true;
-                } else {
-                    exhausted = true;
+        unsigned long long want = f.live() && (act == 0ull || uint32_t(__popcll(idle)) >= kRefillIdle) ? idle : 0ull;
+        while (want) {
+            uint32_t first = 0;
+            const uint32_t got = f.take(uint32_t(__popcll(want)), first);
+            if (got == 0u) break;   // every segment exhausted
+            const uint32_t lb = lanes_below(want);
+            const bool mine = ((want >> (threadIdx.x & 63u)) & 1ull) && lb < got;
+            want &= ~__ballot(mine);
+            if (mine) {
+                idx = first + lb;
+                const float4 o = ro[idx];
+                const float4 d = rd[idx];
+                path = __float_as_uint(d.w);
+                const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
+                const SceneDesc& sc = scene_at<BATCH>(P, sid);
+                if (BATCH) {
+                    nodes = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
+                    tris = static_cast<const char*>(sc.tris);
                 }
+                r = Ray{xyz(o), xyz(d)};
+                inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+                sel_x = inv.x < 0.f ? 16u : 0u;
+                sel_y = inv.y < 0.f ? 48u : 32u;
+                sel_z = inv.z < 0.f ? 80u : 64u;
+                best_t = SHADOW ? o.w : kTFar;
+                best_tri = -1;
+                best_id = 0u;
+                st.sp = 0;
+                node = sc.num_nodes ? 0 : kDone;
+                leaf = 0;
+                busy = true;
             }
         }
-        if (__ballot(busy) == 0ull) {
-            if (__ballot(!exhausted) == 0ull) break;
-            continue;
-        }
+        // no lane busy after a refill: every segment is exhausted
+        if (__ballot(busy) == 0ull) break;
         // descend through internal nodes.  A lane that reaches a leaf keeps it
         // in `leaf` and goes on descending from its stack while lanes of the
         // wave without a leaf are still looking for one (postponed leaves:
@@ -617,7 +646,9 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
     if (!v || p.max_depth > kMaxDepth) return hipErrorInvalidValue;
     const bool batch = p.num_scenes > 0;
     const uint32_t gen_blocks = grid_for(s.paths, 256, 16384);
-    const uint32_t shade_blocks = grid_for(s.paths, kShadeBlock, 8192);
+    // a multiple of the queue segments in waves (one segment per wave)
+    constexpr uint32_t kShadeGrain = kQueueSegments / (kShadeBlock / 64);
+    const uint32_t shade_blocks = (grid_for(s.paths, kShadeBlock, 8192) + kShadeGrain - 1) / kShadeGrain * kShadeGrain;
     auto before = [&](uint32_t k) { if (tm && tm->before) tm->before(tm->user, k, stream); };
     auto after = [&](uint32_t k) { if (tm && tm->after) tm->after(tm->user, k, stream); };
     before(WAVE_K_RAYGEN);
@@ -633,8 +664,13 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
         after(WAVE_K_EXTEND);
         if (e != hipSuccess) return e;
         before(WAVE_K_SHADE);
-        if (batch) hipLaunchKernelGGL((k_shade<kShadeBlock, true>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
-        else hipLaunchKernelGGL((k_shade<kShadeBlock, false>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+        if (b == 0) {
+            if (batch) hipLaunchKernelGGL((k_shade<kShadeBlock, true, true>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+            else hipLaunchKernelGGL((k_shade<kShadeBlock, false, true>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+        } else {
+            if (batch) hipLaunchKernelGGL((k_shade<kShadeBlock, true, false>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+            else hipLaunchKernelGGL((k_shade<kShadeBlock, false, false>), dim3(shade_blocks), dim3(kShadeBlock), 0, stream, p, s, b);
+        }
         after(WAVE_K_SHADE);
         if (shadows) {
             before(WAVE_K_SHADOW);

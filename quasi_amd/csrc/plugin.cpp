@@ -382,19 +382,22 @@ bool ensure_wave(plugin_state* st, uint32_t paths) {
     WaveBuffers& w = st->wave;
     if (w.capacity >= paths) return true;
     const size_t n = paths;
+    // queue slots: kQueueSegments segments of ceil(n / kQueueSegments) slots
+    const size_t q = n + kQueueSegments;
+    const size_t group = size_t(kQueueSegments) * kCounterStride * 4;   // bytes per counter group
     bool ok = true;
     for (int k = 0; k < 2; ++k) {
-        ok = ok && ensure_buffer(st, w.ray_o[k], 16 * n, "wavefront rays");
-        ok = ok && ensure_buffer(st, w.ray_d[k], 16 * n, "wavefront rays");
+        ok = ok && ensure_buffer(st, w.ray_o[k], 16 * q, "wavefront rays");
+        ok = ok && ensure_buffer(st, w.ray_d[k], 16 * q, "wavefront rays");
     }
-    ok = ok && ensure_buffer(st, w.hit, 8 * n, "wavefront hits") && ensure_buffer(st, w.thr, 16 * n, "throughput") &&
+    ok = ok && ensure_buffer(st, w.hit, 8 * q, "wavefront hits") && ensure_buffer(st, w.thr, 16 * n, "throughput") &&
          ensure_buffer(st, w.radiance, 16 * n, "radiance") && ensure_buffer(st, w.rng, 4 * n, "rng states") &&
          ensure_buffer(st, w.aov0, 16 * n, "first-hit data") && ensure_buffer(st, w.aov1, 16 * n, "first-hit data") &&
-         ensure_buffer(st, w.sh_o, 16 * n, "shadow rays") && ensure_buffer(st, w.sh_d, 16 * n, "shadow rays") &&
-         ensure_buffer(st, w.sh_c, 16 * n, "shadow contributions") &&
-         ensure_buffer(st, w.counts, 4 * (kMaxDepth + 1), "queue lengths") &&
-         ensure_buffer(st, w.shadow_counts, 4 * kMaxDepth, "queue lengths") &&
-         ensure_buffer(st, w.fetch, 8 * kMaxDepth, "work cursors");
+         ensure_buffer(st, w.sh_o, 16 * q, "shadow rays") && ensure_buffer(st, w.sh_d, 16 * q, "shadow rays") &&
+         ensure_buffer(st, w.sh_c, 16 * q, "shadow contributions") &&
+         ensure_buffer(st, w.counts, group * (kMaxDepth + 1), "queue lengths") &&
+         ensure_buffer(st, w.shadow_counts, group * kMaxDepth, "queue lengths") &&
+         ensure_buffer(st, w.fetch, group * 2 * kMaxDepth, "work cursors");
     if (!ok) { free_wave(w); return false; }
     w.capacity = paths;
     return true;
@@ -416,6 +419,7 @@ WaveState wave_state(const WaveBuffers& w, uint32_t paths) {
     s.shadow_counts = static_cast<uint32_t*>(w.shadow_counts.ptr);
     s.fetch = static_cast<uint32_t*>(w.fetch.ptr);
     s.paths = paths;
+    s.seg_cap = (paths + kQueueSegments - 1) / kQueueSegments;
     return s;
 }
 
