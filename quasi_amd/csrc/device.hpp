@@ -133,7 +133,11 @@ struct ShortStack {
 __device__ __forceinline__ bool hit_triangle(const float4* __restrict__ tris, int i, const Ray& r, float tmin,
                                              float tmax, float& t_out, uint32_t& id);
 
-// The same test on the triangle record at tri (its three float4s).
+// The same test on the triangle record at tri (its three float4s).  Written
+// without early exits: a wave's lanes test different triangles, so every lane
+// runs the whole test anyway, and the three loads issue together.  The
+// predicate is the early-exit form's, NaN cases included: a value is rejected
+// only when one of its comparisons is true.
 __device__ __forceinline__ bool hit_triangle_rec(const float4* __restrict__ tri, const Ray& r, float tmin,
                                                  float tmax, float& t_out, uint32_t& id) {
     const float4 t0 = tri[0];
@@ -142,16 +146,14 @@ __device__ __forceinline__ bool hit_triangle_rec(const float4* __restrict__ tri,
     const f3 e1 = xyz(t1), e2 = xyz(t2);
     const f3 pv = cross(r.d, e2);
     const float det = dot(e1, pv);
-    if (det == 0.0f) return false;
     const float idet = 1.0f / det;
     const f3 s = r.o - xyz(t0);
     const float u = dot(s, pv) * idet;
-    if (u < 0.0f || u > 1.0f) return false;
     const f3 q = cross(s, e1);
     const float vv = dot(r.d, q) * idet;
-    if (vv < 0.0f || u + vv > 1.0f) return false;
     const float t = dot(e2, q) * idet;
-    if (t < tmin || t > tmax) return false;
+    const bool reject = det == 0.0f || u < 0.0f || u > 1.0f || vv < 0.0f || u + vv > 1.0f || t < tmin || t > tmax;
+    if (reject) return false;
     t_out = t;
     id = __float_as_uint(t0.w);
     return true;
