@@ -381,8 +381,8 @@ BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes,
         for (int k = 0; k < 4; ++k) {
             const bool used = refs[k] != kEmptyChild;
             for (int a = 0; a < 3; ++a) {
-                res.nodes4[8 * a + k] = used ? root.box.lo[a] : 0.f;
-                res.nodes4[8 * a + 4 + k] = used ? root.box.hi[a] : 0.f;
+                res.nodes4[8 * a + k] = used ? root.box.lo[a] : kEmptyLo;
+                res.nodes4[8 * a + 4 + k] = used ? root.box.hi[a] : kEmptyHi;
             }
             std::memcpy(&res.nodes4[24 + k], &refs[k], 4);
         }
@@ -473,6 +473,8 @@ BvhBuildResult convert_binary_tree(const BinaryNode* tnodes, uint32_t num_nodes,
                 const Tmp& c = b.nodes[kids[k]];
                 for (int a = 0; a < 3; ++a) { n[8 * a + k] = c.box.lo[a]; n[8 * a + 4 + k] = c.box.hi[a]; }
                 ref = c.count ? leaf_ref[kids[k]] : 0;
+            } else {
+                for (int a = 0; a < 3; ++a) { n[8 * a + k] = kEmptyLo; n[8 * a + 4 + k] = kEmptyHi; }
             }
             std::memcpy(&n[24 + k], &ref, 4);
         }

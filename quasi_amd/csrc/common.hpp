@@ -23,7 +23,8 @@
 //              [2] = lo.y                    [3] = hi.y
 //              [4] = lo.z                    [5] = hi.z
 //              [6] = child references (int bits, same encoding as BVH2;
-//                    kEmptyChild marks an unused slot)
+//                    kEmptyChild marks an unused slot, whose box is
+//                    lo = +inf, hi = -inf so that every slab test rejects it)
 //              [7] = unused
 //   lights : one LightEntry per emissive triangle
 #pragma once
@@ -36,6 +37,8 @@ constexpr int kLeafMax = 4;
 constexpr int kSahBins = 32;
 constexpr uint32_t kMissId = 0xFFFFFFFFu;
 constexpr int32_t kEmptyChild = 0x7FFFFFFF;
+constexpr float kEmptyLo = __builtin_huge_valf();    // box of an unused BVH4 slot
+constexpr float kEmptyHi = -__builtin_huge_valf();
 
 struct LightEntry {
     uint32_t tri;    // BVH-order triangle index

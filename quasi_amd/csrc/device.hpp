@@ -110,12 +110,18 @@ struct LdsStack {
 // ShortStack: the top LDS_N entries in LDS, deeper entries in a per-thread
 // global overflow area (lane-interleaved with stride gstride).  Keeps the LDS
 // footprint small enough for full occupancy; the overflow is rarely touched.
+// LDS pointer type: accesses through it are ds_* instructions (a generic
+// int* that may point to either space becomes a flat access).
+using lds_int_ptr = __attribute__((address_space(3))) int*;
+
 template <int LDS_N, int STRIDE>
 struct ShortStack {
-    int* lds;
+    lds_int_ptr lds;
     int* glob;
     uint32_t gstride;
     int sp;
+    __device__ __forceinline__ ShortStack(int* lds_base, int* glob_base, uint32_t stride, int sp0)
+        : lds((lds_int_ptr)lds_base), glob(glob_base), gstride(stride), sp(sp0) {}
     __device__ __forceinline__ void push(int v) {
         if (sp < LDS_N) lds[sp * STRIDE] = v;
         else glob[size_t(sp - LDS_N) * gstride] = v;

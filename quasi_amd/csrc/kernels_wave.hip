@@ -384,11 +384,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 slab_sel(ex.y, fx.y, ey.y, fy.y, ez.y, fz.y, r, inv, tmin, best_t, t1n, t1f);
                 slab_sel(ex.z, fx.z, ey.z, fy.z, ez.z, fz.z, r, inv, tmin, best_t, t2n, t2f);
                 slab_sel(ex.w, fx.w, ey.w, fy.w, ez.w, fz.w, r, inv, tmin, best_t, t3n, t3f);
+                // unused slots hold inverted boxes (lo = +inf, hi = -inf), which
+                // the sign-selected slab test always rejects
                 constexpr float kMissKey = 3.0e38f;
-                float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
-                float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
-                float k2 = (t2n <= t2f && ch.z != kEmptyChild) ? t2n : kMissKey;
-                float k3 = (t3n <= t3f && ch.w != kEmptyChild) ? t3n : kMissKey;
+                float k0 = t0n <= t0f ? t0n : kMissKey;
+                float k1 = t1n <= t1f ? t1n : kMissKey;
+                float k2 = t2n <= t2f ? t2n : kMissKey;
+                float k3 = t3n <= t3f ? t3n : kMissKey;
                 int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
                 cswap(k0, c0, k1, c1);
                 cswap(k2, c2, k3, c3);
@@ -403,7 +405,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                     if (st.sp + 3 <= kShortStack) {
                         // three unconditional LDS stores; slots above sp + np are
                         // overwritten before they are read
-                        int* top = st.lds + st.sp * BLOCK;
+                        lds_int_ptr top = st.lds + st.sp * BLOCK;
                         top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
                         top[BLOCK] = np == 3 ? c2 : c1;
                         top[2 * BLOCK] = c1;

@@ -234,6 +234,8 @@ def check_layout4(mesh, nodes4, order):
         for k in range(4):
             c = int(refs[i, k])
             if c == EMPTY:
+                # an inverted box, which the sign-selected slab test rejects
+                assert (lo[i, k] == np.inf).all() and (hi[i, k] == -np.inf).all()
                 continue
             if c >= 0:
                 assert 0 < c < len(nodes4)
