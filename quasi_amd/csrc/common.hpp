@@ -153,9 +153,17 @@ struct PassParams {
     uint32_t num_scenes;
     uint32_t total_pixels;             // framebuffer pixels over all scenes
     const uint32_t* dyn_sample_begin;  // when set, the pass's first sample index is read here
+    // Path records of a wavefront pass run for the adjoint (null otherwise; the
+    // pass then adds nothing to the framebuffer).  For bounce b and path p of
+    // the pass, at [b * paths + p]: rec_hit = the bounce's hit as the traversal
+    // returned it (t bits, BVH triangle or -1), rec_vis = 1 when the vertex's
+    // shadow ray was unoccluded.
+    void* rec_hit;      // uint2 per (bounce, path)
+    uint8_t* rec_vis;
 };
 
-// One adjoint launch (path replay of pass.sample_begin .. + pass.sample_count - 1).
+// One adjoint launch: path replay of pass.sample_begin .. + pass.sample_count - 1
+// from the records of a wavefront pass over the same samples.
 struct AdjointParams {
     PassParams pass;              // scene, camera, integrator settings, replayed samples
     const void* dimg;             // float4 per pixel: dL/dI (rgb; w unused)
@@ -165,6 +173,7 @@ struct AdjointParams {
     float* grad_albedo;           // replicas x 3 x num_tris
     float* grad_vertices;         // replicas x 3 x num_vertices
     uint32_t num_tris, num_vertices;
+    uint32_t rec_paths;           // paths of the recorded pass (pass.rec_hit / rec_vis stride)
 };
 
 // Launch shape of the persistent traversal kernels (autotuned).

@@ -268,22 +268,26 @@ __device__ __forceinline__ TraceHit traverse4(const float4* __restrict__ nodes, 
                                               const Ray& r, float tmin, float tmax, Stack& st) {
     TraceHit best{tmax, -1, 0u};
     const f3 inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+    // entry-plane float4 of each axis in a node (lo, or hi for a negative
+    // direction component); the exit plane is the other one of the pair
+    const uint32_t sx = inv.x < 0.f ? 1u : 0u, sy = inv.y < 0.f ? 3u : 2u, sz = inv.z < 0.f ? 5u : 4u;
     constexpr float kMissKey = 3.0e38f;
     int node = 0;
     for (;;) {
         while (node >= 0) {
             const float4* n = nodes + 8 * node;
-            const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+            const float4 ex = n[sx], fx = n[sx ^ 1u], ey = n[sy], fy = n[sy ^ 1u], ez = n[sz], fz = n[sz ^ 1u];
             const int4 ch = *reinterpret_cast<const int4*>(n + 6);
             float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
-            slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, inv, tmin, best.t, t0n, t0f);
-            slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, inv, tmin, best.t, t1n, t1f);
-            slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, inv, tmin, best.t, t2n, t2f);
-            slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, inv, tmin, best.t, t3n, t3f);
-            float k0 = (t0n <= t0f && ch.x != kEmptyChild) ? t0n : kMissKey;
-            float k1 = (t1n <= t1f && ch.y != kEmptyChild) ? t1n : kMissKey;
-            float k2 = (t2n <= t2f && ch.z != kEmptyChild) ? t2n : kMissKey;
-            float k3 = (t3n <= t3f && ch.w != kEmptyChild) ? t3n : kMissKey;
+            slab_sel(ex.x, fx.x, ey.x, fy.x, ez.x, fz.x, r, inv, tmin, best.t, t0n, t0f);
+            slab_sel(ex.y, fx.y, ey.y, fy.y, ez.y, fz.y, r, inv, tmin, best.t, t1n, t1f);
+            slab_sel(ex.z, fx.z, ey.z, fy.z, ez.z, fz.z, r, inv, tmin, best.t, t2n, t2f);
+            slab_sel(ex.w, fx.w, ey.w, fy.w, ez.w, fz.w, r, inv, tmin, best.t, t3n, t3f);
+            // unused slots hold inverted boxes, which slab_sel always rejects
+            float k0 = t0n <= t0f ? t0n : kMissKey;
+            float k1 = t1n <= t1f ? t1n : kMissKey;
+            float k2 = t2n <= t2f ? t2n : kMissKey;
+            float k3 = t3n <= t3f ? t3n : kMissKey;
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
             cswap(k0, c0, k1, c1);
             cswap(k2, c2, k3, c3);

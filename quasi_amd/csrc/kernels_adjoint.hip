@@ -5,10 +5,13 @@
 // for theta = per-triangle albedo and vertex positions, where C_s is the
 // radiance of one path with its discrete decisions held fixed (triangles hit,
 // facing sides, light choices, shadow visibility, Russian-roulette outcomes
-// and probabilities).  Nothing is kept from the forward render: each thread
-// re-traces its path from the RNG seed with the forward's own code
-// (camera_ray, traverse4, shade_hit), records the path's ray segments in LDS
-// (at most max_depth of them), and then runs reverse mode over the record in
+// and probabilities).  Nothing is kept from the forward render.  A wavefront
+// pass over the same samples (the forward's own kernels) first records, per
+// path and bounce, the hit (distance and triangle) and the shadow ray's
+// visibility; then each thread of k_adjoint replays its path from the RNG
+// seed with the forward's code (camera_ray, the recorded hits, shade_hit)
+// without any BVH traversal, records the path's ray segments in
+// LDS (at most max_depth of them), and runs reverse mode over that record in
 // the same thread.
 //
 // Radiance of a path with hit vertices j = 0..K-1:
@@ -123,14 +126,12 @@ __device__ __forceinline__ void scatter_grads(bool act, uint32_t key, const floa
 
 template <int MAXD>
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
-    __shared__ int s_stack[kShortStack * kAdjBlock];
     __shared__ uint32_t s_rec[MAXD * kRecWords * kAdjBlock];
     const PassParams& P = A.pass;
-    ShortStack<kShortStack, kAdjBlock> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * kAdjBlock + threadIdx.x),
-                                          gridDim.x * kAdjBlock, 0};
     const float4* __restrict__ tris = static_cast<const float4*>(P.tris);
     const float4* __restrict__ mats = static_cast<const float4*>(P.mats);
-    const float4* __restrict__ nodes4 = static_cast<const float4*>(P.nodes4);
+    const uint2* __restrict__ rec_hit = static_cast<const uint2*>(P.rec_hit);
+    const uint8_t* __restrict__ rec_vis = P.rec_vis;
     const float4* __restrict__ dimg = static_cast<const float4*>(A.dimg);
     const uint32_t W = P.width, H = P.height;
     const uint32_t tiles_x = (W + 15u) / 16u;
@@ -157,9 +158,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
                 uint32_t rng = path_seed(P.rng_mode, x, y, W, H, P.sample_begin + s, P.seed);
                 Ray ray = camera_ray(P.cam, W, H, x, y, rng);
                 f3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
+                // the path's id in the recorded pass (sample-major, as the wavefront engine numbers paths)
+                const size_t pid = (size_t(s) * H + y) * W + x;
                 for (uint32_t b = 0; b < P.max_depth; ++b) {
-                    st.sp = 0;
-                    const TraceHit h = traverse4<false>(nodes4, tris, ray, kTMin, kTFar, st);
+                    // the recorded closest hit, distance and triangle as traced
+                    const uint2 rh = rec_hit[size_t(b) * A.rec_paths + pid];
+                    const TraceHit h{__uint_as_float(rh.x), int(rh.y), 0u};
                     if (h.tri < 0) break;
                     REC(b, 0) = __float_as_uint(ray.o.x);
                     REC(b, 1) = __float_as_uint(ray.o.y);
@@ -175,10 +179,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
                     ShadeOut out;
                     shade_hit(P, tris, mats, P.lights, P.num_lights, b, ray, h.t, h.tri, thr, L, rng, nullptr, out);
                     uint32_t fl = 0;
-                    if (out.shadow) {
-                        st.sp = 0;
-                        if (traverse4<true>(nodes4, tris, out.sray, kTMin, out.stmax, st).tri < 0) fl |= kRecNee;
-                    }
+                    if (out.shadow && rec_vis[size_t(b) * A.rec_paths + pid]) fl |= kRecNee;
                     REC(b, 11) = fl;
                     K = int(b) + 1;
                     if (!out.cont) break;
@@ -344,7 +345,9 @@ uint32_t adjoint_block_size() { return kAdjBlock; }
 uint32_t adjoint_max_depth() { return kMaxDepth; }
 
 hipError_t launch_adjoint(const AdjointParams& p, uint32_t blocks, hipStream_t stream) {
-    if (blocks == 0 || p.pass.max_depth == 0 || p.pass.max_depth > kMaxDepth || p.replicas == 0)
+    if (blocks == 0 || p.pass.max_depth == 0 || p.pass.max_depth > kMaxDepth || p.replicas == 0 ||
+        !p.pass.rec_hit || !p.pass.rec_vis ||
+        uint64_t(p.pass.sample_count) * p.pass.width * p.pass.height > p.rec_paths)
         return hipErrorInvalidValue;
     if (p.pass.max_depth <= 4)
         hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(kAdjBlock), 0, stream, p);

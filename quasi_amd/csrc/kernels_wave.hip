@@ -191,6 +191,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
             sid = BATCH ? __float_as_uint(d.w) : 0u;
             const uint2 h = hit[i];
             const int tri = int(h.y);
+            if (P.rec_hit) static_cast<uint2*>(P.rec_hit)[size_t(b) * S.paths + p] = h;
             if (tri >= 0) {
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 float4 t4 = thr_a[p], l4 = rad_a[p];
@@ -263,6 +264,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
                                  (WIDTH == 4 ? traverse4<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0
                                              : traverse2<true>(nodes, tris, ray, kTMin, o.w, st).tri >= 0);
                 if (!occ) add_shadow(rad, sc_a, k, __float_as_uint(d.w));
+                if (P.rec_vis) P.rec_vis[size_t(b) * S.paths + __float_as_uint(d.w)] = occ ? 0u : 1u;
             }
         }
     }
@@ -475,6 +477,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
         if (busy && node == kDone && leaf == 0) {
             if (SHADOW) {
                 if (best_tri < 0) add_shadow(rad, sc_a, idx, path);
+                if (P.rec_vis) P.rec_vis[size_t(b) * S.paths + path] = best_tri < 0 ? 1u : 0u;
             } else {
                 hit[idx] = make_uint2(__float_as_uint(best_t), uint32_t(best_tri));
             }
@@ -681,6 +684,7 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
             if (e != hipSuccess) return e;
         }
     }
+    if (p.rec_hit) return hipGetLastError();   // a recorded pass for the adjoint: no accumulation
     before(WAVE_K_ACCUMULATE);
     const uint32_t acc_blocks = grid_for(p.total_pixels, 256, 16384);
     if (batch) hipLaunchKernelGGL(k_accumulate<true>, dim3(acc_blocks), dim3(256), 0, stream, p, s);

@@ -17,7 +17,9 @@ constexpr uint32_t kAdjointReplicas = 8;   // FP32 gradient copies, one per XCD 
 
 void free_adjoint(plugin_state* st) {
     AdjointState& a = st->adjoint;
-    for (auto* b : {&a.indices, &a.image, &a.dimg, &a.grads, &a.out, &a.partials}) free_buffer(*b);
+    for (auto* b : {&a.indices, &a.image, &a.dimg, &a.grads, &a.out, &a.partials, &a.rec_hit, &a.rec_vis,
+                    &a.counters})
+        free_buffer(*b);
     a.indices_generation = 0;
 }
 
@@ -51,6 +53,14 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     const size_t nt = st->num_tris;
     const size_t nv = st->verts.size() / 3;
     const uint32_t nblk = l2_image_blocks(npix);
+    // samples per recorded pass: the wavefront engine's pass size
+    const uint32_t per_pass = std::min<uint32_t>(
+        d->spp, std::max<uint32_t>(1u, kAutoPassPaths / std::max<uint32_t>(1u, uint32_t(npix))));
+    const size_t rec_paths = size_t(per_pass) * npix;
+    if (!ensure_buffer(st, a.rec_hit, 8 * size_t(rp.max_depth) * rec_paths, "adjoint path records") ||
+        !ensure_buffer(st, a.rec_vis, size_t(rp.max_depth) * rec_paths, "adjoint path records") ||
+        !ensure_buffer(st, a.counters, sizeof(Counters), "adjoint ray counters") || !ensure_wave(st, uint32_t(rec_paths)))
+        return Q_EXT_ERR_OUT_OF_MEMORY;
     if (!ensure_buffer(st, a.indices, 4 * 3 * nt, "adjoint indices") ||
         !ensure_buffer(st, a.image, 16 * npix, "adjoint image") ||
         !ensure_buffer(st, a.dimg, 16 * npix, "adjoint image gradient") ||
@@ -102,6 +112,23 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     p.num_nodes4 = st->num_nodes4;
     p.gstack = static_cast<int*>(st->gstack.ptr);
     p.gstack_entries = st->gstack_entries;
+    p.counters = static_cast<Counters*>(a.counters.ptr);
+    p.rec_hit = a.rec_hit.ptr;
+    p.rec_vis = static_cast<uint8_t*>(a.rec_vis.ptr);
+    SceneDesc& sd = p.scene0;
+    sd.tris = p.tris;
+    sd.mats = p.mats;
+    sd.nodes = p.nodes;
+    sd.nodes4 = p.nodes4;
+    sd.lights = p.lights;
+    sd.num_nodes = p.num_nodes;
+    sd.num_lights = p.num_lights;
+    sd.cam = p.cam;
+    sd.width = p.width;
+    sd.height = p.height;
+    sd.seed = p.seed;
+    sd.num_tris = st->num_tris;
+    p.total_pixels = uint32_t(npix);
     A.dimg = dimg;
     A.scale = 1.0f / float(d->spp);
     A.replicas = kAdjointReplicas;
@@ -115,7 +142,18 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     const uint32_t cap = uint32_t(st->num_cus) * (kMaxThreadsPerCU / adjoint_block_size());
     const uint32_t blocks = std::max(1u, std::min(tiles, cap));
     double* out = static_cast<double*>(a.out.ptr);
-    ok = ok && hip_ok(st, launch_adjoint(A, blocks, s), "adjoint") &&
+    // per pass: the forward's wavefront kernels record the paths' hits and
+    // shadow visibility, then k_adjoint replays and differentiates them
+    for (uint32_t done = 0; ok && done < d->spp; done += per_pass) {
+        const uint32_t n = std::min(per_pass, d->spp - done);
+        p.sample_begin = d->sample_begin + done;
+        p.sample_count = n;
+        A.rec_paths = uint32_t(size_t(n) * npix);
+        const WaveState ws = wave_state(st->wave, A.rec_paths);
+        ok = hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, nullptr), "adjoint record pass") &&
+             hip_ok(st, launch_adjoint(A, blocks, s), "adjoint");
+    }
+    ok = ok &&
          hip_ok(st, launch_replica_reduce(A.grad_albedo, 3 * nt, kAdjointReplicas, out, s), "albedo reduction") &&
          hip_ok(st, launch_replica_reduce(A.grad_vertices, 3 * nv, kAdjointReplicas, out + 3 * nt, s),
                 "vertex reduction") &&

@@ -75,6 +75,9 @@ struct AdjointState {
     DeviceBuffer grads;          // FP32 replicas: albedo then vertices
     DeviceBuffer out;            // FP64 reduced gradients: albedo then vertices
     DeviceBuffer partials;       // FP32 per-block loss sums
+    DeviceBuffer rec_hit;        // path records of the replayed pass: hit (t, triangle) per bounce and path
+    DeviceBuffer rec_vis;        // ... and shadow-ray visibility (bytes)
+    DeviceBuffer counters;       // ray counters of the recorded passes (kept out of the render statistics)
     uint64_t indices_generation = 0;
 };
 
