@@ -68,8 +68,9 @@ def cpu_baseline(mesh, args, seconds: float):
 def bench_batch(args):
     """64 scenes of uneven size (room + soup, 1k..500k triangles, 128^2..1024^2),
     32 spp, depth 6, NEE: every scene rendered one after another with the
-    plain wavefront engine, then all of them as one batch (shared queues,
-    captured pass graph, autotuned trace configuration).  Scene preparation
+    plain wavefront engine (non-persistent traversal, and the persistent
+    default), then all of them as one batch (shared queues, captured pass
+    graph, autotuned trace configuration).  Scene preparation
     (BVH builds, uploads) is outside both timed regions; every scene's batched
     beauty image is compared with its plain one."""
     import numpy as np
@@ -93,17 +94,31 @@ def bench_batch(args):
     plain_imgs = []
     prog_s = 0.0
     prog_rays = 0
+    flat_s = 0.0
+    flat_rays = 0
+    # the plain wavefront tracer: no persistent waves, one thread per queue slot
+    flat = dict(common, bvh_width=4, block_size=128, queue_chunk=abi.Q_QUEUE_FLAT)
     with HipRenderer(64, 64, device=0) as r:
         for i, (mesh, w, h, seed) in enumerate(batch):
             r.set_scene(mesh)
             r.resize(w, h)
             c = mesh.camera
             r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
-            r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **common)
-            if i == 0:          # untimed warm-up of the kernels, then a cleared accumulation
+            if i == 0:          # untimed warm-up of the kernels
+                for kw in ((flat,) if args.batch_flat else ()) + (common,):
+                    r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **kw)
+                    r.render(1)
+                    r.synchronize()
+            if args.batch_flat:
+                r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **flat)
+                r.reset_stats()
+                r.synchronize()
+                t0 = time.perf_counter()
                 r.render(1)
                 r.synchronize()
-                r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **common)
+                flat_s += time.perf_counter() - t0
+                flat_rays += rays_of(r.stats())
+            r.set_params(spp_per_render=spp, seed=seed, samples_per_pass=0, flags=0, **common)
             r.reset_stats()
             r.synchronize()
             t0 = time.perf_counter()
@@ -123,6 +138,8 @@ def bench_batch(args):
                 prog_s += time.perf_counter() - t0
                 prog_rays += rays_of(r.stats())
         log(f"batch: plain sequential {plain_s:.3f} s, {plain_rays / plain_s / 1e6:.1f} Mrays/s")
+        if flat_s:
+            log(f"batch: plain non-persistent sequential {flat_s:.3f} s, {flat_rays / flat_s / 1e6:.1f} Mrays/s")
 
     with HipRenderer(64, 64, device=0) as r:
         r.set_params(spp_per_render=1, seed=0, samples_per_pass=args.batch_samples_per_pass, flags=0, **common)
@@ -163,7 +180,9 @@ def bench_batch(args):
                           "queue_chunk": best.queue_chunk},
             "max_rel_l1_vs_plain": err,
             "plain_progressive_mrays": round(prog_rays / prog_s / 1e6, 3) if prog_s else None,
-            "speedup_vs_progressive": round(batched / (prog_rays / prog_s / 1e6), 4) if prog_s else None}
+            "speedup_vs_progressive": round(batched / (prog_rays / prog_s / 1e6), 4) if prog_s else None,
+            "plain_nonpersistent_mrays": round(flat_rays / flat_s / 1e6, 3) if flat_s else None,
+            "speedup_vs_nonpersistent": round(batched / (flat_rays / flat_s / 1e6), 4) if flat_s else None}
 
 
 def bench_adjoint(args):
@@ -256,6 +275,8 @@ def main():
     ap.add_argument("--batch-samples-per-pass", type=int, default=0, help="0: the plugin's automatic pass size")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--adjoint-runs", type=int, default=5)
+    ap.add_argument("--batch-flat", type=int, default=1,
+                    help="also time the plain non-persistent wavefront tracer per scene (1/0)")
     ap.add_argument("--batch-progressive", type=int, default=1,
                     help="also time the plain engine at one sample per render call (the reference host's usage)")
     args = ap.parse_args()

@@ -93,8 +93,10 @@ typedef struct Q_render_params {
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
     uint32_t queue_chunk;      /* wavefront: 1 (0 = 1) = each idle lane takes its next ray from the
-                                  queue (refilled whenever half a wave is idle); 64 or 256 = each
-                                  wave takes that many rays at a time */
+                                  queue (persistent waves, refilled when 40 lanes are idle); 64 or
+                                  256 = each persistent wave takes that many rays at a time;
+                                  Q_QUEUE_FLAT = no persistence: one thread per queue slot, the
+                                  plain wavefront tracer (block 128 only) */
     uint32_t bvh_builder;      /* Q_BVH_BUILD_*: where Q_plugin_set_scene builds the BVH */
     uint32_t reserved;
 } Q_render_params;
@@ -102,6 +104,9 @@ typedef struct Q_render_params {
 /* Q_render_params.bvh_builder (the same binned-SAH algorithm either way; any
  * valid tree renders the same image, because the closest hit does not
  * depend on the tree) */
+/* Q_render_params.queue_chunk: the plain (non-persistent) wavefront traversal */
+#define Q_QUEUE_FLAT 0xFFFFFFFFu
+
 #define Q_BVH_BUILD_AUTO 0u   /* on the GPU from 2^20 triangles, else on the host */
 #define Q_BVH_BUILD_HOST 1u   /* host threads */
 #define Q_BVH_BUILD_GPU  2u   /* GPU kernels (falls back to the host when the GPU build fails) */

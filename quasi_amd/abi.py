@@ -91,6 +91,7 @@ class Q_render_params(C.Structure):
 
 
 Q_BVH_BUILD_AUTO, Q_BVH_BUILD_HOST, Q_BVH_BUILD_GPU = 0, 1, 2
+Q_QUEUE_FLAT = 0xFFFFFFFF   # queue_chunk: the plain (non-persistent) wavefront traversal
 
 
 class Q_kernel_stat(C.Structure):

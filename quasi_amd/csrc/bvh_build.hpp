@@ -26,8 +26,9 @@ struct BvhBuildResult {
 };
 
 // p0/p1/p2: vertex positions of each triangle, 3 floats each, in input order.
+// cut_depth as for convert_binary_tree.
 BvhBuildResult build_bvh(const float* p0, const float* p1, const float* p2, uint32_t ntri,
-                         unsigned threads);
+                         unsigned threads, uint32_t cut_depth = 0);
 
 // A node of a binary build tree: internal (count == 0: children left, right)
 // or leaf (count 1..kLeafMax triangles prims[first .. first + count)).  Node 0
@@ -41,14 +42,19 @@ struct BinaryNode {
 // The pair-node and 4-wide layouts of a binary build tree (shared by the host
 // build and the GPU build); prims maps the tree's triangle positions to input
 // triangles.  The output is independent of how the tree's node slots were
-// numbered.
+// numbered and of `threads` (0 = the machine's, at most 16).  cut_depth: the
+// depth below which subtrees are converted in parallel (0 = 12 for trees of
+// 2^18 nodes or more, else one sequential walk; ~0u = always sequential); the
+// layouts are the same for every cut, and the SAH cost agrees to rounding.
 BvhBuildResult convert_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, const uint32_t* prims,
-                                   uint32_t ntri);
+                                   uint32_t ntri, unsigned threads = 0, uint32_t cut_depth = 0);
 
 // True when the tree rooted at node 0 reaches every one of its num_nodes
 // nodes exactly once, internal children are in range, leaves hold 1..kLeafMax
 // triangles, the leaves cover tree positions 0..ntri-1 exactly once and prims
-// is a permutation of 0..ntri-1.
+// is a permutation of 0..ntri-1.  Requires (and checks) that every child has a
+// larger index than its parent, which both builders guarantee (slots are
+// taken from a counter when a node is split), so the checks run in parallel.
 bool validate_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, const uint32_t* prims, uint32_t ntri);
 
 }  // namespace quasi_hip

@@ -140,6 +140,7 @@ struct PassParams {
     uint32_t num_lights;
     uint32_t num_nodes;
     uint32_t record_ids, id_sample;
+    uint32_t num_tris;     // triangles (a batch: of all scenes, whose arrays are concatenated)
     // 4-wide BVH over the same triangles, and the traversal-stack overflow
     // area of the persistent kernels (gstack_entries words per thread,
     // lane-interleaved; sized by the host for the deepest possible stack)
@@ -180,8 +181,11 @@ struct AdjointParams {
 struct TraceConfig {
     uint32_t width;   // BVH width: 2 or 4
     uint32_t block;   // threads per block: 64, 128 or 256
-    uint32_t chunk;   // rays fetched per wave per queue access: 64 or 256; 1 = per-lane dynamic fetch
+    uint32_t chunk;   // rays fetched per wave per queue access: 64 or 256; 1 = per-lane dynamic fetch;
+                      // kChunkFlat = no persistence (one thread per queue slot)
 };
+constexpr uint32_t kChunkFlat = 0xFFFFFFFFu;
+constexpr int kFlatStack = 48;        // LDS stack entries per thread of the non-persistent traversal
 
 constexpr int kShortStack = 16;       // LDS entries per lane in the persistent kernels
 

@@ -97,11 +97,16 @@ constexpr float kSlabWiden = 1.000001f;
 // lanes of a wave touch 64 consecutive words (distinct LDS banks, coalesced
 // global lines).
 //
+// LDS pointer type: accesses through it are ds_* instructions (a generic
+// int* that may point to either space becomes a flat access).
+using lds_int_ptr = __attribute__((address_space(3))) int*;
+
 // LdsStack: the whole stack in LDS (bounded by the BVH depth).
 template <int STRIDE>
 struct LdsStack {
-    int* lds;
+    lds_int_ptr lds;
     int sp;
+    __device__ __forceinline__ LdsStack(int* base, int sp0) : lds((lds_int_ptr)base), sp(sp0) {}
     __device__ __forceinline__ void push(int v) { lds[sp * STRIDE] = v; ++sp; }
     __device__ __forceinline__ int pop() { --sp; return lds[sp * STRIDE]; }
     __device__ __forceinline__ bool empty() const { return sp == 0; }
@@ -110,10 +115,6 @@ struct LdsStack {
 // ShortStack: the top LDS_N entries in LDS, deeper entries in a per-thread
 // global overflow area (lane-interleaved with stride gstride).  Keeps the LDS
 // footprint small enough for full occupancy; the overflow is rarely touched.
-// LDS pointer type: accesses through it are ds_* instructions (a generic
-// int* that may point to either space becomes a flat access).
-using lds_int_ptr = __attribute__((address_space(3))) int*;
-
 template <int LDS_N, int STRIDE>
 struct ShortStack {
     lds_int_ptr lds;

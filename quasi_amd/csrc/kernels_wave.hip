@@ -279,15 +279,16 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // cursor of the wave's current queue segment.  SHADOW selects the any-hit
 // shadow queue; otherwise the closest-hit queue.  Same visit order and tie
 // rule as traverse2 / traverse4.
-// OFF32 (one scene whose node and triangle arrays each span less than 4 GiB):
-// nodes and triangles are addressed as the scene's base pointer, uniform over
-// the grid, plus a 32-bit byte offset, which the loads take as they are.
+// OFF32 (node and triangle arrays, a batch's concatenated ones included, each
+// under 4 GiB): nodes and triangles are addressed as the array's base pointer,
+// uniform over the grid, plus a 32-bit byte offset (in a batch: the lane's
+// scene's offset plus the record's), which the loads take as they are;
+// otherwise each lane holds 64-bit pointers to its scene's arrays.
 constexpr uint32_t kRefillIdle = 40;   // measured best of 8..48 on the 1M soup
 constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
-    static_assert(!(BATCH && OFF32), "a batch addresses each lane's scene through its own pointers");
     constexpr uint32_t kNodeShift = WIDTH == 4 ? 7u : 6u;   // 128-B BVH4 / 64-B BVH2 nodes
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
@@ -315,9 +316,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     // byte offsets in a BVH4 node of each axis's entry plane (lo, or hi when
     // the direction component is negative); the exit plane is at offset ^ 16
     uint32_t sel_x = 0, sel_y = 32, sel_z = 64;
-    // a batch's lanes may trace different scenes; one scene's arrays are uniform
-    const char* nodes = BATCH ? nullptr : reinterpret_cast<const char*>(scene_nodes<WIDTH, false>(P, 0));
-    const char* tris = BATCH ? nullptr : static_cast<const char*>(P.scene0.tris);
+    // base of the node and triangle arrays (a batch: the concatenated ones)
+    const char* const nbase = static_cast<const char*>(WIDTH == 4 ? P.nodes4 : P.nodes);
+    const char* const tbase = static_cast<const char*>(P.tris);
+    uint32_t nofs = 0, tofs = 0;                    // OFF32 batch: the lane's scene's byte offsets
+    const char* nodes = BATCH ? nullptr : nbase;    // otherwise: the lane's scene's arrays
+    const char* tris = BATCH ? nullptr : tbase;
     for (;;) {
         // refill idle lanes (wave-uniform: every lane reaches the ballots)
         const unsigned long long idle = __ballot(!busy);
@@ -338,8 +342,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 if (BATCH) {
-                    nodes = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
-                    tris = static_cast<const char*>(sc.tris);
+                    const char* sn = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
+                    const char* stt = static_cast<const char*>(sc.tris);
+                    if (OFF32) {
+                        nofs = uint32_t(sn - nbase);
+                        tofs = uint32_t(stt - tbase);
+                    } else {
+                        nodes = sn;
+                        tris = stt;
+                    }
                 }
                 r = Ray{xyz(o), xyz(d)};
                 inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
@@ -366,10 +377,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             if (__ballot(desc && leaf == 0) == 0ull) break;
             if (!desc) continue;
             // the node's record: base + 32-bit offset (OFF32) or a 64-bit address
-            const uint32_t off = uint32_t(node) << kNodeShift;
+            const uint32_t off = nofs + (uint32_t(node) << kNodeShift);
             const char* rec = nodes + (OFF32 ? size_t(0) : (size_t(uint32_t(node)) << kNodeShift));
             auto ld = [&](uint32_t o) -> float4 {
-                return OFF32 ? *reinterpret_cast<const float4*>(nodes + (off + o))
+                return OFF32 ? *reinterpret_cast<const float4*>(nbase + (off + o))
                              : *reinterpret_cast<const float4*>(rec + o);
             };
             if (WIDTH == 4) {
@@ -450,12 +461,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             const int first = int(v >> 2);
             const int count = int(v & 3u) + 1;
             bool found = false;
-            uint32_t toff = uint32_t(first) * 48u;   // 48-B triangle records
+            uint32_t toff = tofs + uint32_t(first) * 48u;   // 48-B triangle records
             const char* trec = tris + (OFF32 ? size_t(0) : size_t(uint32_t(first)) * 48u);
             for (int i = first; i < first + count; ++i, toff += 48u, trec += OFF32 ? 0 : 48) {
                 float t;
                 uint32_t id;
-                const float4* tr = reinterpret_cast<const float4*>(OFF32 ? tris + toff : trec);
+                const float4* tr = reinterpret_cast<const float4*>(OFF32 ? tbase + toff : trec);
                 if (!hit_triangle_rec(tr, r, tmin, best_t, t, id)) continue;
                 if (SHADOW) { best_t = t; best_tri = i; best_id = id; found = true; break; }
                 if (best_tri < 0 || t < best_t || id > best_id) {
@@ -483,6 +494,42 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             }
             busy = false;
         }
+    }
+}
+
+// The plain wavefront traversal: no persistence and no work fetch, one thread
+// per slot of the bounce's (segmented) queue; slots past their segment's
+// length exit at once.  The whole stack lives in LDS (kFlatStack entries, which
+// the host checks against the BVH depth).  SHADOW: any hit on the shadow queue.
+template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_trace_flat(PassParams P, WaveState S, uint32_t b) {
+    __shared__ int s_stack[kFlatStack * BLOCK];
+    const uint32_t* lengths = SHADOW ? S.shadow_counts + queue_counter(b, 0) : S.counts + queue_counter(b, 0);
+    count_rays(lengths, SHADOW ? &P.counters->rays_shadow
+                               : (b == 0 ? &P.counters->rays_camera : &P.counters->rays_extension));
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t seg = i / S.seg_cap;
+    if (seg >= kQueueSegments || i - seg * S.seg_cap >= lengths[seg * kCounterStride]) return;
+    const float4* __restrict__ ro = static_cast<const float4*>(SHADOW ? S.sh_o : S.ray_o[b & 1u]);
+    const float4* __restrict__ rd = static_cast<const float4*>(SHADOW ? S.sh_d : S.ray_d[b & 1u]);
+    const float4* __restrict__ sc_a = static_cast<const float4*>(S.sh_c);
+    const float4 o = ro[i], d = rd[i];
+    const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[i].w : d.w) : 0u;
+    const SceneDesc& sc = scene_at<BATCH>(P, sid);
+    const float4* nodes = scene_nodes<WIDTH, BATCH>(P, sid);
+    const float4* tris = static_cast<const float4*>(sc.tris);
+    LdsStack<BLOCK> st{s_stack + threadIdx.x, 0};
+    const Ray ray{xyz(o), xyz(d)};
+    TraceHit h{SHADOW ? o.w : kTFar, -1, 0u};
+    if (sc.num_nodes)
+        h = WIDTH == 4 ? traverse4<SHADOW>(nodes, tris, ray, kTMin, SHADOW ? o.w : kTFar, st)
+                       : traverse2<SHADOW>(nodes, tris, ray, kTMin, SHADOW ? o.w : kTFar, st);
+    if (SHADOW) {
+        const uint32_t path = __float_as_uint(d.w);
+        if (h.tri < 0) add_shadow(static_cast<float4*>(S.radiance), sc_a, i, path);
+        if (P.rec_vis) P.rec_vis[size_t(b) * S.paths + path] = h.tri < 0 ? 1u : 0u;
+    } else {
+        static_cast<uint2*>(S.hit)[i] = make_uint2(__float_as_uint(h.t), uint32_t(h.tri));
     }
 }
 
@@ -545,26 +592,34 @@ int occupancy_of(K kernel, int block) {
     return a;
 }
 
-// CHUNK 64 / 256: static chunks per wave; CHUNK 1: per-lane dynamic fetch.
+// CHUNK 64 / 256: static chunks per wave; CHUNK 1: per-lane dynamic fetch;
+// CHUNK 0 (kChunkFlat in a TraceConfig): the non-persistent traversal.
 template <int WIDTH, int BLOCK, int CHUNK>
 struct TraceVariant {
-    static constexpr int kStatic = CHUNK == 1 ? 64 : CHUNK;
-    // one scene whose node and triangle arrays each span less than 4 GiB
+    static constexpr int kStatic = CHUNK <= 1 ? 64 : CHUNK;
+    // one thread per queue slot; the stack must hold the deepest walk
+    template <bool SHADOW, bool BATCH>
+    static hipError_t flat(const PassParams& p, const WaveState& s, uint32_t b, hipStream_t st) {
+        if (p.gstack_entries + uint32_t(kShortStack) > uint32_t(kFlatStack)) return hipErrorInvalidValue;
+        const uint32_t blocks = uint32_t((uint64_t(s.seg_cap) * kQueueSegments + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL((k_trace_flat<WIDTH, BLOCK, SHADOW, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        return hipGetLastError();
+    }
+    // node and triangle arrays (a batch's concatenated ones) each under 4 GiB
     static bool off32(const PassParams& p) {
         const uint64_t node_bytes = uint64_t(WIDTH == 4 ? p.num_nodes4 : p.num_nodes) * (WIDTH == 4 ? 128u : 64u);
-        return p.num_scenes == 0 && node_bytes < (1ull << 32) && uint64_t(p.scene0.num_tris) * 48u < (1ull << 32);
+        return node_bytes < (1ull << 32) && uint64_t(p.num_tris) * 48u < (1ull << 32);
     }
     template <bool SHADOW, bool BATCH>
     static void dynamic(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        if (BATCH)
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, true, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
-        else if (off32(p))
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, false, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        if (off32(p))
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         else
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, false, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
     }
     template <bool BATCH>
     static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        if (CHUNK == 0) return flat<false, BATCH>(p, s, b, st);
         if (CHUNK == 1)
             dynamic<false, BATCH>(p, s, b, blocks, st);
         else
@@ -573,6 +628,7 @@ struct TraceVariant {
     }
     template <bool BATCH>
     static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
+        if (CHUNK == 0) return flat<true, BATCH>(p, s, b, st);
         if (CHUNK == 1)
             dynamic<true, BATCH>(p, s, b, blocks, st);
         else
@@ -584,7 +640,10 @@ struct TraceVariant {
     // the first ones finish, which is harmless: no block waits for another.
     static int occupancy() {
         int a, c;
-        if (CHUNK == 1) {
+        if (CHUNK == 0) {
+            a = occupancy_of(k_trace_flat<WIDTH, BLOCK, false, false>, BLOCK);
+            c = occupancy_of(k_trace_flat<WIDTH, BLOCK, true, false>, BLOCK);
+        } else if (CHUNK == 1) {
             a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, false, true>, BLOCK);
             c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, false, true>, BLOCK);
         } else {
@@ -612,12 +671,14 @@ const VariantEntry kVariants[] = {
     QH_VARIANT(4, 64, 256), QH_VARIANT(4, 128, 256), QH_VARIANT(4, 256, 256),
     QH_VARIANT(2, 64, 1),   QH_VARIANT(2, 128, 1),   QH_VARIANT(2, 256, 1),
     QH_VARIANT(4, 64, 1),   QH_VARIANT(4, 128, 1),   QH_VARIANT(4, 256, 1),
+    QH_VARIANT(2, 128, 0),  QH_VARIANT(4, 128, 0),
 };
 #undef QH_VARIANT
 
 const VariantEntry* find_variant(const TraceConfig& c) {
+    const uint32_t chunk = c.chunk == kChunkFlat ? 0u : c.chunk;
     for (const VariantEntry& v : kVariants)
-        if (v.width == c.width && v.block == c.block && v.chunk == c.chunk) return &v;
+        if (v.width == c.width && v.block == c.block && v.chunk == chunk) return &v;
     return nullptr;
 }
 
@@ -629,7 +690,7 @@ uint32_t trace_config_count() { return uint32_t(sizeof(kVariants) / sizeof(kVari
 
 TraceConfig trace_config_at(uint32_t i) {
     const VariantEntry& v = kVariants[i % trace_config_count()];
-    return TraceConfig{v.width, v.block, v.chunk};
+    return TraceConfig{v.width, v.block, v.chunk == 0 ? kChunkFlat : v.chunk};
 }
 
 uint32_t persistent_blocks(int device, const TraceConfig& c) {

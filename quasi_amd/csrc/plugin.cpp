@@ -537,6 +537,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
     p.integrator = rp.integrator;
     p.num_lights = st->num_lights;
     p.num_nodes = st->num_nodes;
+    p.num_tris = st->num_tris;
     p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
     p.id_sample = rp.id_sample;
     p.nodes4 = st->nodes4.ptr;

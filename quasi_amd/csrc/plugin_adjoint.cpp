@@ -108,6 +108,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     p.integrator = rp.integrator;
     p.num_lights = st->num_lights;
     p.num_nodes = st->num_nodes;
+    p.num_tris = st->num_tris;
     p.nodes4 = st->nodes4.ptr;
     p.num_nodes4 = st->num_nodes4;
     p.gstack = static_cast<int*>(st->gstack.ptr);

@@ -97,6 +97,15 @@ PassParams batch_params(plugin_state* st, uint32_t spp_pass) {
     p.id_sample = rp.id_sample;
     p.gstack = static_cast<int*>(st->gstack.ptr);
     p.gstack_entries = b.gstack_entries;
+    // the concatenated arrays: every scene's pointers in the table lie inside them
+    p.tris = b.tris.ptr;
+    p.mats = b.mats.ptr;
+    p.nodes = b.nodes.ptr;
+    p.nodes4 = b.nodes4.ptr;
+    p.lights = static_cast<const LightEntry*>(b.lights.ptr);
+    p.num_tris = b.total_tris;
+    p.num_nodes = b.total_nodes;
+    p.num_nodes4 = b.total_nodes4;
     p.scenes = static_cast<const SceneDesc*>(b.table.ptr);
     p.num_scenes = b.num_scenes;
     p.total_pixels = b.total_pixels;
@@ -302,6 +311,8 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     b.num_scenes = n;
     b.total_pixels = uint32_t(total_pix);
     b.total_tris = uint32_t(total_tris);
+    b.total_nodes = uint32_t(node_f / 16);     // 16 floats per BVH2 pair node
+    b.total_nodes4 = uint32_t(node4_f / 32);   // 32 floats per BVH4 node
     b.gstack_entries = std::max(gentries, st->gstack_entries);
     st->gstack_entries = b.gstack_entries;
     b.active = true;

@@ -58,6 +58,7 @@ struct BatchState {
     DeviceBuffer tris, mats, nodes, nodes4, lights, table;
     DeviceBuffer accum[Q_AOV_COUNT], ids, resolved, sample_begin;
     uint32_t num_scenes = 0, total_pixels = 0, total_tris = 0;
+    uint32_t total_nodes = 0, total_nodes4 = 0;   // BVH2 / BVH4 nodes of the concatenated arrays
     uint32_t sample_count = 0;
     uint32_t gstack_entries = 0;
     double build_ms = 0;

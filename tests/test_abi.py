@@ -52,6 +52,18 @@ def test_symbol_name_constants():
     assert abi.Q_PLUGIN_ABI_VERSION == 3
 
 
+def test_header_constants_match_the_python_mirror():
+    # every numeric #define of the headers that quasi_amd.abi also names has the same value
+    checked = 0
+    for h in (ROOT / "include" / "quasi").glob("*.h"):
+        for m in re.finditer(r"^#define\s+(Q_\w+)\s+(-?(?:0x[0-9A-Fa-f]+|\d+))u?\b", h.read_text(), re.M):
+            name, value = m.group(1), int(m.group(2), 0)
+            if hasattr(abi, name):
+                assert getattr(abi, name) == value, name
+                checked += 1
+    assert checked >= 20 and abi.Q_QUEUE_FLAT == 0xFFFFFFFF
+
+
 LAYOUT_STRUCTS = ["Q_gpu_context", "Q_camera", "Q_render_frame", "Q_plugin_version", "Q_plugin_info",
                   "Q_plugin_context", "Q_readback_result", "Q_aov_buffer", "Q_readback_aov_result",
                   "Q_scene_desc", "Q_render_params", "Q_kernel_stat", "Q_render_stats", "Q_batch_scene",

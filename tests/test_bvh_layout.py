@@ -27,6 +27,11 @@ def probe(tmp_path_factory):
                     str(ROOT / "quasi_amd" / "csrc" / "bvh_build.cpp"), "-o", str(so)], check=True)
     lib = C.CDLL(str(so))
     P = C.POINTER
+    lib.probe_build_cut.restype = C.c_int
+    lib.probe_build_cut.argtypes = [P(C.c_float)] * 3 + [C.c_uint, C.c_uint, P(C.c_float), C.c_uint, P(C.c_uint),
+                                                          P(C.c_uint), P(C.c_float), C.c_uint, P(C.c_double)]
+    lib.probe_validate.restype = C.c_int
+    lib.probe_validate.argtypes = [C.c_void_p, C.c_uint, C.c_void_p, C.c_uint]
     lib.probe_build.restype = C.c_int
     lib.probe_build.argtypes = [P(C.c_float)] * 3 + [C.c_uint, C.c_uint, P(C.c_float), C.c_uint, P(C.c_uint),
                                                       P(C.c_uint), P(C.c_float)]
@@ -321,3 +326,76 @@ def test_layout4_large_soup(probe):
     assert internal.max() < len(nodes4)
     assert meta["nodes4"] < meta["nodes"]
     assert 3 * meta["depth4"] <= 16 + 64     # LDS short stack + global overflow capacity
+
+
+def build_cut(probe, mesh, cut, threads=4):
+    tri = mesh.triangles().astype(np.float32)
+    n = tri.shape[0]
+    p = [np.ascontiguousarray(tri[:, k, :]) for k in range(3)]
+    cap = max(1, 2 * n)
+    nodes = np.zeros((cap, 16), dtype=np.float32)
+    nodes4 = np.zeros((cap, 32), dtype=np.float32)
+    order = np.zeros(max(1, n), dtype=np.uint32)
+    info = np.zeros(6, dtype=np.uint32)
+    sah = C.c_double(0.0)
+    rc = probe.probe_build_cut(*[x.ctypes.data_as(C.POINTER(C.c_float)) for x in p], n, threads,
+                               nodes.ctypes.data_as(C.POINTER(C.c_float)), cap,
+                               order.ctypes.data_as(C.POINTER(C.c_uint)), info.ctypes.data_as(C.POINTER(C.c_uint)),
+                               nodes4.ctypes.data_as(C.POINTER(C.c_float)), cut, C.byref(sah))
+    assert rc == 0
+    return nodes[:info[0]], nodes4[:info[4]], order[:n], tuple(int(v) for v in info), sah.value
+
+
+@pytest.mark.parametrize("cut", [1, 3, 6, 0])
+def test_parallel_conversion_equals_one_walk(probe, cut):
+    """Converting the subtrees below a cut depth on several threads gives the
+    layouts of one sequential walk bit for bit (cut 1: almost everything in
+    subtrees; 0: the automatic cut, which a 300k soup's tree is large enough
+    to use); the SAH cost is the same sum in another association."""
+    mesh = scenes.soup(300_000, seed=12)
+    seq = build_cut(probe, mesh, 0xFFFFFFFF)
+    par = build_cut(probe, mesh, cut)
+    for a, b in zip(seq[:3], par[:3]):
+        assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert seq[3] == par[3]
+    assert abs(seq[4] - par[4]) <= 1e-12 * seq[4]
+    # and the parallel output is a valid layout
+    check_layout4(mesh, par[1], par[2])
+
+
+BINARY_NODE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("left", "<u4"), ("right", "<u4"), ("first", "<u4"),
+                        ("count", "<u4")])
+
+
+def small_tree():
+    """Root 0 -> internal 1 and leaf 2; node 1 -> leaves 3 and 4; 7 triangles."""
+    t = np.zeros(5, dtype=BINARY_NODE)
+    t["hi"] = 1.0
+    t[0]["left"], t[0]["right"] = 1, 2
+    t[1]["left"], t[1]["right"] = 3, 4
+    t[2]["first"], t[2]["count"] = 0, 3
+    t[3]["first"], t[3]["count"] = 3, 2
+    t[4]["first"], t[4]["count"] = 5, 2
+    return t, np.array([6, 5, 4, 3, 2, 1, 0], dtype=np.uint32)
+
+
+def validate(probe, t, prims, ntri=None):
+    return probe.probe_validate(t.ctypes.data, len(t), prims.ctypes.data, len(prims) if ntri is None else ntri)
+
+
+def test_tree_validation_accepts_a_tree_and_rejects_each_defect(probe):
+    assert BINARY_NODE.itemsize == 40
+    t, prims = small_tree()
+    assert validate(probe, t, prims) == 1
+    bad = []
+    x = t.copy(); x[1]["left"] = 0; bad.append((x, prims))                    # child index not above the parent
+    x = t.copy(); x[1]["left"] = 2; bad.append((x, prims))                    # node 2 has two parents
+    x = t.copy(); x[0]["right"] = 9; bad.append((x, prims))                   # child out of range
+    x = t.copy(); x[4]["first"] = 4; bad.append((x, prims))                   # position 4 covered twice, 6 never
+    x = t.copy(); x[2]["count"] = 5; bad.append((x, prims))                   # leaf larger than kLeafMax
+    x = t.copy(); x[4]["count"] = 3; bad.append((x, prims))                   # leaf past the last triangle
+    p2 = prims.copy(); p2[0] = 5; bad.append((t, p2))                         # prims not a permutation
+    x = np.concatenate([t, t[4:5]]); bad.append((x, prims))                   # node 5 unreachable (no parent)
+    for x, p in bad:
+        assert validate(probe, x, p) == 0
+

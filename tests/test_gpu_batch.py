@@ -134,7 +134,7 @@ def test_autotune_batch(tmp_path, monkeypatch):
         r.set_params(spp_per_render=1, seed=0, **PARAMS)
         r.set_batch(batch)
         res = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=1)
-        assert res.num_entries == 18 and res.best < res.num_entries and not res.from_cache
+        assert res.num_entries == 20 and res.best < res.num_entries and not res.from_cache
         best = res.entries[res.best]
         assert all(best.ms <= res.entries[k].ms for k in range(res.num_entries))
         p = r.get_params()

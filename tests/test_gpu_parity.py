@@ -213,13 +213,15 @@ def test_wavefront_pass_split_and_determinism():
         assert np.allclose(im, imgs[2], rtol=2e-6, atol=1e-6)
 
 
-TRACE_CONFIGS = [(w, b, c) for w in (2, 4) for b in (64, 128, 256) for c in (1, 64, 256)]
+TRACE_CONFIGS = ([(w, b, c) for w in (2, 4) for b in (64, 128, 256) for c in (1, 64, 256)] +
+                 [(w, 128, abi.Q_QUEUE_FLAT) for w in (2, 4)])
 
 
 def test_all_trace_configs_render_identically():
     """Every compiled traversal variant (BVH width 2/4, block 64/128/256, queue
-    chunk 64/256) yields the same image bit for bit: the closest hit is unique
-    under the tie rule and occlusion does not depend on which blocker is found."""
+    chunk: per-lane fetch, 64, 256, and the non-persistent form) yields the same
+    image bit for bit: the closest hit is unique under the tie rule and
+    occlusion does not depend on which blocker is found."""
     mesh = scenes.soup(150_000, seed=8)
     w, h = 96, 64
     say("[trace-configs]")
