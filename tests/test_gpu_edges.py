@@ -1,0 +1,96 @@
+"""GPU edge cases against the FP64 oracle, through the C ABI: ragged and tiny
+images, a one-triangle scene without lights, a camera that sees nothing,
+depth 1, and a sample count that does not divide into whole passes.  Same bars
+as test_gpu_parity.py (relative L1 <= 1e-4 on the same samples, ids >= 99.99%
+with only near-tie disagreements, exact camera-ray counts)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quasi_amd import abi, scenes
+from quasi_amd.plugin import HipRenderer, make_camera
+from tests.parity import check_ids, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+REL_BAR = 1e-4
+ENGINES = [abi.Q_ENGINE_MEGAKERNEL, abi.Q_ENGINE_WAVEFRONT]
+ENGINE_IDS = ["megakernel", "wavefront"]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def cam_of(mesh):
+    c = mesh.camera
+    return make_camera(c["position"], c["target"], c["up"], c["fov"])
+
+
+def render_and_compare(mesh, w, h, spp, engine, label, **kw):
+    params = dict(max_depth=kw.pop("max_depth", 4), rr_start=kw.pop("rr_start", 3), rng_mode=1,
+                  integrator=kw.pop("integrator", 1), seed=kw.pop("seed", 3))
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        r.camera = cam_of(mesh)
+        r.set_params(spp_per_render=spp, flags=abi.Q_FLAG_RECORD_IDS, id_sample=0, engine=engine, **params, **kw)
+        r.render(1)
+        aov = r.readback_aov()
+        ids = r.primary_ids()
+        st = r.stats()
+    s = O.Scene(mesh)
+    ref = s.render(w, h, spp, precision=64, **params)
+    errs = {k: rel_l1(aov[k], ref.mean(k)) for k in ("beauty", "albedo", "normal", "depth")}
+    frac, nbad, not_ties = check_ids(ids, ref.ids, mesh, s,
+                                     dict(width=w, height=h, rng_mode=1, seed=params["seed"], camera=mesh.camera))
+    say(f"[{label}] {w}x{h} spp={spp} rel-L1 {errs} ids {frac:.6f} rays {st.rays_camera}/{st.rays_extension}/"
+        f"{st.rays_shadow} oracle {ref.counts.camera_rays}/{ref.counts.extension_rays}/{ref.counts.shadow_rays}")
+    for k, e in errs.items():
+        assert e <= REL_BAR, (label, k, e)
+    assert frac >= 0.9999 and not not_ties, (label, frac, not_ties[:3])
+    assert st.rays_camera == ref.counts.camera_rays == w * h * spp
+    return aov, st, ref
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 5), (33, 17)])
+def test_tiny_and_ragged_images(w, h, engine):
+    render_and_compare(scenes.cornell36(2), w, h, 5, engine, "ragged")
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_one_triangle_without_lights(engine, integrator):
+    """NEE with an empty light table and the reference loop on a scene of one
+    triangle: nothing emits, so every image is black, but the hits, AOVs and
+    ray counts still follow the oracle."""
+    m = scenes.Mesh(vertices=np.array([[-1, 0, -1], [1, 0, -1], [0, 1.5, -1]], dtype=np.float32),
+                    indices=np.array([[0, 1, 2]], dtype=np.uint32),
+                    albedo=np.array([[0.7, 0.5, 0.3]], dtype=np.float32),
+                    emission=np.zeros((1, 3), dtype=np.float32),
+                    camera=dict(position=(0.0, 0.6, 2.0), target=(0.0, 0.6, -1.0), up=(0.0, 1.0, 0.0), fov=40.0),
+                    name="one")
+    aov, st, ref = render_and_compare(m, 24, 16, 3, engine, "one-triangle", integrator=integrator)
+    assert float(np.abs(aov["beauty"][..., :3]).max()) == 0.0
+    assert (aov["depth"][..., 0] > 0).any()
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+def test_camera_that_sees_nothing(engine):
+    m = scenes.cornell36(1)
+    m.camera = dict(position=(0.0, 1.0, 3.5), target=(0.0, 1.0, 10.0), up=(0.0, 1.0, 0.0), fov=40.0)
+    aov, st, ref = render_and_compare(m, 20, 12, 2, engine, "miss")
+    assert st.rays_extension == 0 and st.rays_shadow == 0
+    assert float(np.abs(aov["beauty"][..., :3]).max()) == 0.0
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+def test_depth_one(engine):
+    aov, st, ref = render_and_compare(scenes.cornell36(3), 32, 24, 4, engine, "depth1", max_depth=1)
+    assert st.rays_extension == 0 and st.rays_shadow == 0
+
+
+def test_ragged_sample_passes():
+    """7 samples in passes of 3 (3 + 3 + 1) on the wavefront engine."""
+    render_and_compare(scenes.cornell36(4), 40, 30, 7, abi.Q_ENGINE_WAVEFRONT, "passes", samples_per_pass=3,
+                       max_depth=5)
