@@ -38,6 +38,7 @@ PLUGIN_SOURCES = [
     ("kernels_util.hip", DEVICE),
     ("kernels_adjoint.hip", DEVICE),
     ("bvh_gpu.hip", DEVICE),
+    ("bvh_layout.hip", DEVICE),
     ("plugin.cpp", HOSTONLY),
     ("plugin_batch.cpp", HOSTONLY),
     ("plugin_adjoint.cpp", HOSTONLY),
@@ -96,6 +97,22 @@ def build_host(verbose: bool = False) -> Path | None:
     return HOST_BIN
 
 
+LAYOUT_PROBE = ROOT / "tests" / "native" / "liblayoutprobe.so"
+
+
+def build_test_probes(verbose: bool = False) -> Path:
+    """tests/native/liblayoutprobe.so: the GPU scene build's layout next to the
+    host converter, from the plugin's own objects (build_plugin first)."""
+    obj = BUILD / "layout_probe.cpp.o"
+    _run([HIPCC, *COMMON, *HOSTONLY, "-I", ROOT / "include", "-I", CSRC, "-c",
+          ROOT / "tests" / "native" / "layout_probe.cpp", "-o", obj], verbose)
+    parts = [BUILD / "bvh_gpu.hip.o", BUILD / "bvh_layout.hip.o", BUILD / "bvh_build.cpp.o"]
+    tmp = LAYOUT_PROBE.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj, *parts, "-pthread"], verbose)
+    os.replace(tmp, LAYOUT_PROBE)
+    return LAYOUT_PROBE
+
+
 def build_oracle(verbose: bool = False) -> None:
     """The checker used by tests (oracle/liboracle.so and, when the reference
     sources are present, oracle/_ref)."""
@@ -108,6 +125,7 @@ def build_all(verbose: bool = False) -> None:
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     build_plugin(verbose)
+    build_test_probes(verbose)
     build_host(verbose)
     build_oracle(verbose)
 

@@ -4,8 +4,10 @@
 // bins on each axis, the cheapest of the 31 x 3 candidate planes by the SAH
 // (traversal cost 1, triangle cost 1), at most kLeafMax triangles per leaf, a
 // median split when every centroid coincides.  The output is a binary tree of
-// BinaryNode plus the triangle order, converted to the traversal layouts by
-// convert_binary_tree on the host.
+// BinaryNode plus the triangle order, which bvh_layout.hip turns into the
+// traversal layouts on the device; the triangle and material arrays are then
+// written in BVH order on the device too (build_scene_gpu), so after the mesh
+// upload nothing of the scene passes through host memory.
 //
 // Two phases:
 //  - large nodes (more than kLargeNode triangles) are split one tree level at
@@ -21,11 +23,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
 #include "bvh_build.hpp"
+#include "bvh_gpu.hpp"
 #include "common.hpp"
+#include "device_buffer.hpp"
 
 namespace quasi_hip {
 
@@ -589,25 +594,6 @@ __global__ void k_init_bins(Bins* bins, uint32_t ntasks) {
     }
 }
 
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    bool alloc(size_t count) {
-        release();
-        if (count == 0) count = 1;
-        if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) { p = nullptr; return false; }
-        n = count;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
 #define BVH_CHECK(x)                     \
     do {                                 \
         if ((x) != hipSuccess) return false; \
@@ -615,38 +601,40 @@ struct DevBuf {
 
 }  // namespace
 
-// Builds on the current device and stream; false on any failure (the caller
-// then uses the host build).  timing_ms receives the device build time.
-bool build_bvh_gpu(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
-                   BvhBuildResult& out, double* build_ms, const char** why) {
-    *why = "";
-    if (ntri < 8 || ntri > (1u << 29)) { *why = "size"; return false; }
-    const uint32_t n = ntri;
-    const uint32_t cap = 2 * n;
-    DevBuf<float> dp[3];
-    DevBuf<float4> plo, phi;
-    DevBuf<uint32_t> ids_a, ids_b, order, counter, err;
+namespace {
+
+// The binary tree of one build, on the device.
+struct DeviceTree {
     DevBuf<BinaryNode> nodes;
-    if (!dp[0].alloc(3 * size_t(n)) || !dp[1].alloc(3 * size_t(n)) || !dp[2].alloc(3 * size_t(n)) ||
-        !plo.alloc(n) || !phi.alloc(n) || !ids_a.alloc(n) || !ids_b.alloc(n) || !order.alloc(n) ||
+    DevBuf<uint32_t> prims;   // tree triangle position -> input triangle
+    uint32_t used = 0;        // nodes in use, root 0
+};
+
+// Builds the tree of triangles d0/d1/d2 (device arrays, 3 floats per
+// triangle) on stream s; device_ms receives the kernels' time.
+bool build_tree(const float* d0, const float* d1, const float* d2, uint32_t n, hipStream_t s, DeviceTree& tree,
+                double* device_ms, const char** why) {
+    if (n < 8 || n > (1u << 29)) { *why = "size"; return false; }
+    const uint32_t cap = 2 * n;
+    DevBuf<float4> plo, phi;
+    DevBuf<uint32_t> ids_a, ids_b, counter, err;
+    DevBuf<BinaryNode>& nodes = tree.nodes;
+    DevBuf<uint32_t>& order = tree.prims;
+    if (!plo.alloc(n) || !phi.alloc(n) || !ids_a.alloc(n) || !ids_b.alloc(n) || !order.alloc(n) ||
         !counter.alloc(1) || !err.alloc(1) || !nodes.alloc(cap)) {
         *why = "allocation";
         return false;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BVH_CHECK(hipEventCreate(&e0));
-    BVH_CHECK(hipEventCreate(&e1));
-    struct EvGuard { hipEvent_t a, b; ~EvGuard() { if (a) (void)hipEventDestroy(a); if (b) (void)hipEventDestroy(b); } } eg{e0, e1};
-    const float* src[3] = {p0, p1, p2};
-    for (int k = 0; k < 3; ++k)
-        BVH_CHECK(hipMemcpyAsync(dp[k].p, src[k], 12 * size_t(n), hipMemcpyHostToDevice, s));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return false; }
+    struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
     BVH_CHECK(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(k_prim_bounds, dim3(std::min<uint32_t>((n + 255) / 256, 65536)), dim3(256), 0, s, dp[0].p,
-                       dp[1].p, dp[2].p, n, plo.p, phi.p, ids_a.p);
+    hipLaunchKernelGGL(k_prim_bounds, dim3(std::min<uint32_t>((n + 255) / 256, 65536)), dim3(256), 0, s, d0, d1, d2,
+                       n, plo.p, phi.p, ids_a.p);
     hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, counter.p, size_t(1), 1u);
     hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, err.p, size_t(1), 0u);
     BVH_CHECK(hipGetLastError());
-
     // large-node levels
     std::vector<NodeTask> medium;
     std::vector<NodeTask> level{NodeTask{0, 0, n, 0}};
@@ -723,15 +711,190 @@ bool build_bvh_gpu(const float* p0, const float* p1, const float* p2, uint32_t n
     BVH_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    if (build_ms) *build_ms = ms;
+    if (device_ms) *device_ms = ms;
     if (herr || used == 0 || used > cap) { *why = "subtree pass"; return false; }
-    std::vector<BinaryNode> tree(used);
-    std::vector<uint32_t> prims(n);
-    BVH_CHECK(hipMemcpyAsync(tree.data(), nodes.p, used * sizeof(BinaryNode), hipMemcpyDeviceToHost, s));
-    BVH_CHECK(hipMemcpyAsync(prims.data(), order.p, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    tree.used = used;
+    return true;
+}
+
+// Corner positions of each triangle (3 floats per corner) from the indexed
+// mesh; an index out of range sets err and reads vertex 0.
+__global__ void k_gather_corners(const float* __restrict__ verts, uint32_t nv, const uint32_t* __restrict__ idx,
+                                 uint32_t nt, float* __restrict__ d0, float* __restrict__ d1, float* __restrict__ d2,
+                                 uint32_t* __restrict__ err) {
+    bool bad = false;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        float* dst[3] = {d0, d1, d2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            uint32_t v = idx[3 * size_t(t) + c];
+            if (v >= nv) { bad = true; v = 0; }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dst[c][3 * size_t(t) + k] = verts[3 * size_t(v) + k];
+        }
+    }
+    if (bad) atomicOr(err, 1u);
+}
+
+// Triangle and material arrays in BVH order (layouts of common.hpp) and the
+// inverse order; a triangle that occurs twice or is out of range sets err.
+__global__ void k_scene_arrays(const uint32_t* __restrict__ order, uint32_t n, const float* __restrict__ d0,
+                               const float* __restrict__ d1, const float* __restrict__ d2,
+                               const float* __restrict__ albedo, const float* __restrict__ emission,
+                               float4* __restrict__ tris, float4* __restrict__ mats, uint32_t* __restrict__ inv,
+                               uint32_t* __restrict__ err) {
+    bool bad = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t t = order[i];
+        if (t >= n || atomicExch(&inv[t], i) != 0xFFFFFFFFu) { bad = true; continue; }
+        const size_t o = 3 * size_t(t);
+        const float ax = d0[o], ay = d0[o + 1], az = d0[o + 2];
+        tris[3 * size_t(i)] = make_float4(ax, ay, az, __uint_as_float(t));
+        tris[3 * size_t(i) + 1] = make_float4(d1[o] - ax, d1[o + 1] - ay, d1[o + 2] - az, 0.f);
+        tris[3 * size_t(i) + 2] = make_float4(d2[o] - ax, d2[o + 1] - ay, d2[o + 2] - az, 0.f);
+        const float e0 = emission[o], e1 = emission[o + 1], e2 = emission[o + 2];
+        const float m12 = e1 < e2 ? e2 : e1;          // std::max(e0, std::max(e1, e2)) of the host layout
+        const float m = e0 < m12 ? m12 : e0;
+        mats[2 * size_t(i)] = make_float4(albedo[o], albedo[o + 1], albedo[o + 2], m);
+        mats[2 * size_t(i) + 1] = make_float4(e0, e1, e2, 0.f);
+    }
+    if (bad) atomicOr(err, 2u);
+}
+
+__global__ void k_gather_positions(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ query, uint32_t nq,
+                                   uint32_t* __restrict__ pos) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) pos[q] = inv[query[q]];
+}
+
+uint32_t grid256(size_t items) { return uint32_t(std::max<size_t>(1, std::min<size_t>((items + 255) / 256, 16384))); }
+
+}  // namespace
+
+void DeviceScene::release() {
+    if (tris) (void)hipFree(tris);
+    if (mats) (void)hipFree(mats);
+    if (nodes) (void)hipFree(nodes);
+    if (nodes4) (void)hipFree(nodes4);
+    tris = mats = nullptr;
+    nodes = nodes4 = nullptr;
+}
+
+bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,
+                     const float* emission, const uint32_t* query, uint32_t nquery, uint32_t* query_pos,
+                     hipStream_t s, DeviceScene& out, BvhGpuTimes* times, const char** why) {
+    *why = "";
+    out.release();
+    if (nt < 8 || nt > (1u << 29) || nv == 0) { *why = "size"; return false; }
+    for (uint32_t q = 0; q < nquery; ++q)
+        if (query[q] >= nt) { *why = "query"; return false; }
+    BvhGpuTimes tm;
+    using clock = std::chrono::steady_clock;
+    clock::time_point tp = clock::now();
+    auto lap = [&tp](double& into) {
+        const clock::time_point now = clock::now();
+        into = std::chrono::duration<double, std::milli>(now - tp).count();
+        tp = now;
+    };
+    const size_t n3 = 3 * size_t(nt);
+    DevBuf<float> dverts, dalb, demi, dp[3];
+    DevBuf<uint32_t> didx, err;
+    if (!dverts.alloc(3 * size_t(nv)) || !didx.alloc(n3) || !dalb.alloc(n3) || !demi.alloc(n3) || !dp[0].alloc(n3) ||
+        !dp[1].alloc(n3) || !dp[2].alloc(n3) || !err.alloc(1)) {
+        *why = "allocation";
+        return false;
+    }
+    lap(tm.alloc_ms);
+    BVH_CHECK(hipMemcpyAsync(dverts.p, verts, 12 * size_t(nv), hipMemcpyHostToDevice, s));
+    BVH_CHECK(hipMemcpyAsync(didx.p, idx, 4 * n3, hipMemcpyHostToDevice, s));
+    BVH_CHECK(hipMemcpyAsync(dalb.p, albedo, 4 * n3, hipMemcpyHostToDevice, s));
+    BVH_CHECK(hipMemcpyAsync(demi.p, emission, 4 * n3, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, err.p, size_t(1), 0u);
+    hipLaunchKernelGGL(k_gather_corners, dim3(grid256(nt)), dim3(256), 0, s, dverts.p, nv, didx.p, nt, dp[0].p,
+                       dp[1].p, dp[2].p, err.p);
+    BVH_CHECK(hipGetLastError());
+    uint32_t herr = 0;
+    BVH_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipStreamSynchronize(s));
-    if (!validate_binary_tree(tree.data(), used, prims.data(), n)) { *why = "validation"; return false; }
-    out = convert_binary_tree(tree.data(), used, prims.data(), n);
+    if (herr) { *why = "vertex index out of range"; return false; }
+    dverts.release();
+    didx.release();
+    lap(tm.upload_ms);
+
+    DeviceTree tree;
+    if (!build_tree(dp[0].p, dp[1].p, dp[2].p, nt, s, tree, &tm.device_ms, why)) {
+        if (!**why) *why = "HIP call";
+        return false;
+    }
+    lap(tm.build_ms);
+
+    DeviceLayout lay;
+    struct LayGuard { DeviceLayout& l; ~LayGuard() { l.release(); } } lg{lay};
+    if (!layout_tree_gpu(tree.nodes.p, tree.used, tree.prims.p, nt, s, lay, why)) return false;
+    tree.nodes.release();
+    tree.prims.release();
+    DevBuf<float4> tris, mats;
+    DevBuf<uint32_t> inv, dq, dpos;
+    if (!tris.alloc(3 * size_t(nt)) || !mats.alloc(2 * size_t(nt)) || !inv.alloc(nt) ||
+        (nquery && (!dq.alloc(nquery) || !dpos.alloc(nquery)))) {
+        *why = "allocation";
+        return false;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BVH_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return false; }
+    struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
+    BVH_CHECK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(grid256(nt)), dim3(256), 0, s, inv.p, size_t(nt), 0xFFFFFFFFu);
+    hipLaunchKernelGGL(k_scene_arrays, dim3(grid256(nt)), dim3(256), 0, s, lay.order, nt, dp[0].p, dp[1].p, dp[2].p,
+                       dalb.p, demi.p, tris.p, mats.p, inv.p, err.p);
+    if (nquery) {
+        BVH_CHECK(hipMemcpyAsync(dq.p, query, 4 * size_t(nquery), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_positions, dim3(grid256(nquery)), dim3(256), 0, s, inv.p, dq.p, nquery, dpos.p);
+        BVH_CHECK(hipMemcpyAsync(query_pos, dpos.p, 4 * size_t(nquery), hipMemcpyDeviceToHost, s));
+    }
+    BVH_CHECK(hipGetLastError());
+    BVH_CHECK(hipEventRecord(e1, s));
+    BVH_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipStreamSynchronize(s));
+    if (herr) { *why = "triangle order"; return false; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    tm.layout_device_ms = lay.device_ms + ms;
+    out.tris = tris.detach();
+    out.mats = mats.detach();
+    out.nodes = lay.nodes;
+    out.nodes4 = lay.nodes4;
+    lay.nodes = lay.nodes4 = nullptr;
+    out.num_nodes = lay.num_nodes;
+    out.num_leaves = lay.num_leaves;
+    out.depth = lay.depth;
+    out.num_nodes4 = lay.num_nodes4;
+    out.depth4 = lay.depth4;
+    out.sah_cost = lay.sah_cost;
+    lap(tm.layout_ms);
+    if (times) *times = tm;
+    return true;
+}
+
+bool build_tree_gpu_host(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
+                         std::vector<BinaryNode>& tree_out, std::vector<uint32_t>& prims_out, const char** why) {
+    *why = "";
+    DevBuf<float> dp[3];
+    const float* src[3] = {p0, p1, p2};
+    for (int k = 0; k < 3; ++k) {
+        if (!dp[k].alloc(3 * size_t(ntri))) { *why = "allocation"; return false; }
+        BVH_CHECK(hipMemcpyAsync(dp[k].p, src[k], 12 * size_t(ntri), hipMemcpyHostToDevice, s));
+    }
+    DeviceTree tree;
+    if (!build_tree(dp[0].p, dp[1].p, dp[2].p, ntri, s, tree, nullptr, why)) {
+        if (!**why) *why = "HIP call";
+        return false;
+    }
+    tree_out.resize(tree.used);
+    prims_out.resize(ntri);
+    BVH_CHECK(hipMemcpyAsync(tree_out.data(), tree.nodes.p, tree.used * sizeof(BinaryNode), hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipMemcpyAsync(prims_out.data(), tree.prims.p, 4 * size_t(ntri), hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipStreamSynchronize(s));
     return true;
 }
 

@@ -78,13 +78,6 @@ hipError_t launch_l2_image(const float4* sum, uint32_t n, const float4* target, 
 
 namespace quasi_hip {
 
-// Binned-SAH build on the current device (stream s) of the triangles p0/p1/p2
-// (host arrays, 3 floats per triangle): true with the result in out and the
-// device build time in build_ms, or false with a reason in why (the caller
-// then builds on the host).
-bool build_bvh_gpu(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
-                   BvhBuildResult& out, double* build_ms, const char** why);
-
 // Triangle count from which Q_BVH_BUILD_AUTO builds on the GPU.
 constexpr uint32_t kGpuBuildMinTriangles = 1u << 20;
 

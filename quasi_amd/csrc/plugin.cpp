@@ -133,17 +133,102 @@ void parallel_ranges(size_t n, unsigned threads, F&& f) {
     for (auto& t : pool) t.join();
 }
 
+// Emissive triangles (max emission > 0.1), in input order.
+std::vector<uint32_t> emissive_triangles(const float* emission, uint32_t nt, unsigned threads) {
+    const unsigned parts = std::max(1u, std::min(threads, 16u));
+    std::vector<std::vector<uint32_t>> found(parts);
+    const size_t chunk = (size_t(nt) + parts - 1) / parts;
+    parallel_ranges(parts, nt >= (1u << 16) ? parts : 1u, [&](size_t pb, size_t pe) {
+        for (size_t part = pb; part < pe; ++part)
+            for (size_t t = part * chunk; t < std::min<size_t>(nt, (part + 1) * chunk); ++t) {
+                const float* e = &emission[3 * t];
+                if (std::max(e[0], std::max(e[1], e[2])) > 0.1f) found[part].push_back(uint32_t(t));
+            }
+    });
+    std::vector<uint32_t> all;
+    for (auto& f : found) all.insert(all.end(), f.begin(), f.end());
+    return all;
+}
+
+// Light table in input-triangle order: weight = area * (Le.r + Le.g + Le.b),
+// prefix sums in double, cdf stored as float with the last entry set to 1.
+// pos[i] is the BVH-order position of triangle lights[i].
+void light_table(const float* verts, const uint32_t* idx, const float* emission, const std::vector<uint32_t>& tris,
+                 const uint32_t* pos, std::vector<LightEntry>& out) {
+    out.clear();
+    std::vector<double> wts;
+    for (size_t i = 0; i < tris.size(); ++i) {
+        const uint32_t t = tris[i];
+        const float* v0 = &verts[3 * size_t(idx[3 * size_t(t)])];
+        const float* v1 = &verts[3 * size_t(idx[3 * size_t(t) + 1])];
+        const float* v2 = &verts[3 * size_t(idx[3 * size_t(t) + 2])];
+        double a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = double(v1[k]) - double(v0[k]);
+            b[k] = double(v2[k]) - double(v0[k]);
+        }
+        const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        const float* e = &emission[3 * size_t(t)];
+        out.push_back({pos[i], 0.f, 0.f, float(area)});
+        wts.push_back(area * (double(e[0]) + double(e[1]) + double(e[2])));
+    }
+    double total = 0;
+    for (double w : wts) total += w;
+    double run = 0;
+    for (size_t i = 0; i < out.size(); ++i) {
+        run += wts[i];
+        out[i].cdf = total > 0 ? float(run / total) : 0.f;
+        out[i].psel = total > 0 ? float(wts[i] / total) : 0.f;
+    }
+    if (!out.empty()) out.back().cdf = 1.0f;
+}
+
 }  // namespace
 
 int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                       const float* albedo, const float* emission, unsigned threads, SceneHost& out,
                       uint32_t builder) {
+    const unsigned host_threads = threads ? threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    using clock = std::chrono::steady_clock;
+    auto ms_since = [](clock::time_point t) { return std::chrono::duration<double, std::milli>(clock::now() - t).count(); };
+    out.builder = Q_BVH_BUILD_HOST;
+    out.device_ms = 0;
+    const std::vector<uint32_t> lights = emissive_triangles(emission, nt, host_threads);
+    const bool gpu = threads == 0 && (builder == Q_BVH_BUILD_GPU ||
+                                      (builder == Q_BVH_BUILD_AUTO && nt >= kGpuBuildMinTriangles));
+    if (gpu) {
+        // the whole scene on the device: gather, tree, layouts, BVH-order arrays
+        std::vector<uint32_t> lpos(lights.size());
+        const char* why = "";
+        if (build_scene_gpu(verts, nv, idx, nt, albedo, emission, lights.data(), uint32_t(lights.size()),
+                            lpos.data(), st->stream, out.device, &out.gpu_times, &why)) {
+            const DeviceScene& d = out.device;
+            if (d.depth > megakernel_max_depth()) {
+                log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", d.depth, megakernel_max_depth());
+                out.device.release();
+                return Q_EXT_ERR_UNSUPPORTED;
+            }
+            out.builder = Q_BVH_BUILD_GPU;
+            out.device_ms = out.gpu_times.device_ms;
+            out.sah_cost = d.sah_cost;
+            out.num_tris = nt;
+            out.num_nodes = d.num_nodes;
+            out.num_nodes4 = d.num_nodes4;
+            out.depth = d.depth;
+            out.depth4 = d.depth4;
+            light_table(verts, idx, emission, lights, lpos.data(), out.lights);
+            return Q_EXT_OK;
+        }
+        log_msg(st, "scene: GPU build failed (%s); building on the host", why);
+    }
+
     for (size_t i = 0; i < 3 * size_t(nt); ++i)
         if (idx[i] >= nv) {
             log_msg(st, "scene: vertex index %u out of range (%u vertices)", idx[i], nv);
             return Q_EXT_ERR_ARGUMENT;
         }
-    const unsigned host_threads = threads ? threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    clock::time_point t0 = clock::now();
     std::vector<float> p0(3 * size_t(nt)), p1(3 * size_t(nt)), p2(3 * size_t(nt));
     parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
         for (size_t t = tb; t < te; ++t)
@@ -153,20 +238,12 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
                 p2[3 * t + k] = verts[3 * size_t(idx[3 * t + 2]) + k];
             }
     });
-    BvhBuildResult bvh;
-    const bool gpu = threads == 0 && (builder == Q_BVH_BUILD_GPU ||
-                                      (builder == Q_BVH_BUILD_AUTO && nt >= kGpuBuildMinTriangles));
-    out.builder = Q_BVH_BUILD_HOST;
-    out.device_ms = 0;
-    if (gpu) {
-        const char* why = "";
-        if (build_bvh_gpu(p0.data(), p1.data(), p2.data(), nt, st->stream, bvh, &out.device_ms, &why))
-            out.builder = Q_BVH_BUILD_GPU;
-        else
-            log_msg(st, "scene: GPU BVH build failed (%s); building on the host", why);
-    }
-    if (out.builder == Q_BVH_BUILD_HOST) bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, threads);
+    out.gather_ms = ms_since(t0);
+    t0 = clock::now();
+    BvhBuildResult bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, threads);
     out.sah_cost = bvh.sah_cost;
+    out.bvh_ms = ms_since(t0);
+    t0 = clock::now();
     if (bvh.depth > megakernel_max_depth()) {
         log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", bvh.depth, megakernel_max_depth());
         return Q_EXT_ERR_UNSUPPORTED;
@@ -192,34 +269,12 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
             m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
         }
     });
-    // Light table in input-triangle order: weight = area * (Le.r + Le.g + Le.b),
-    // prefix sums in double, cdf stored as float with the last entry set to 1.
-    out.lights.clear();
-    std::vector<double> wts;
-    for (uint32_t t = 0; t < nt; ++t) {
-        const float* e = &emission[3 * size_t(t)];
-        if (!(std::max(e[0], std::max(e[1], e[2])) > 0.1f)) continue;
-        double a[3], b[3];
-        for (int k = 0; k < 3; ++k) {
-            a[k] = double(p1[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
-            b[k] = double(p2[3 * size_t(t) + k]) - double(p0[3 * size_t(t) + k]);
-        }
-        const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
-        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-        out.lights.push_back({bvh_pos[t], 0.f, 0.f, float(area)});
-        wts.push_back(area * (double(e[0]) + double(e[1]) + double(e[2])));
-    }
-    double total = 0;
-    for (double w : wts) total += w;
-    double run = 0;
-    for (size_t i = 0; i < out.lights.size(); ++i) {
-        run += wts[i];
-        out.lights[i].cdf = total > 0 ? float(run / total) : 0.f;
-        out.lights[i].psel = total > 0 ? float(wts[i] / total) : 0.f;
-    }
-    if (!out.lights.empty()) out.lights.back().cdf = 1.0f;
+    std::vector<uint32_t> lpos(lights.size());
+    for (size_t i = 0; i < lights.size(); ++i) lpos[i] = bvh_pos[lights[i]];
+    light_table(verts, idx, emission, lights, lpos.data(), out.lights);
     out.nodes = std::move(bvh.nodes);
     out.nodes4 = std::move(bvh.nodes4);
+    out.layout_ms = ms_since(t0);
     out.num_tris = nt;
     out.num_nodes = bvh.num_nodes;
     out.num_nodes4 = bvh.num_nodes4;
@@ -303,6 +358,17 @@ int32_t upload_scene(plugin_state* st) {
     int32_t rc = prepare_scene(st, st->verts.data(), nv, st->indices.data(), nt, st->albedo.data(),
                                st->emission.data(), 0, sh, st->params.bvh_builder);
     if (rc != Q_EXT_OK) return rc;
+    if (sh.builder == Q_BVH_BUILD_GPU) {
+        // adopt the device arrays of the GPU build (the stream is idle: the build synchronised it)
+        DeviceScene& d = sh.device;
+        for (DeviceBuffer* b : {&st->tris, &st->mats, &st->nodes, &st->nodes4}) free_buffer(*b);
+        st->tris = {d.tris, 48 * size_t(nt)};
+        st->mats = {d.mats, 32 * size_t(nt)};
+        st->nodes = {d.nodes, 64 * size_t(d.num_nodes)};
+        st->nodes4 = {d.nodes4, 128 * size_t(d.num_nodes4)};
+        d.tris = d.mats = nullptr;
+        d.nodes = d.nodes4 = nullptr;
+    }
     const uint32_t gentries = std::max(stack_overflow_entries(sh.depth, sh.depth4), st->batch.gstack_entries);
     if (!ensure_buffer(st, st->tris, std::max<size_t>(16, sh.tri4.size() * 4), "triangles") ||
         !ensure_buffer(st, st->mats, std::max<size_t>(16, sh.mat4.size() * 4), "materials") ||
@@ -312,8 +378,8 @@ int32_t upload_scene(plugin_state* st) {
         !ensure_buffer(st, st->lights, std::max<size_t>(16, sh.lights.size() * sizeof(LightEntry)), "lights"))
         return Q_EXT_ERR_OUT_OF_MEMORY;
     if (!hip_ok(st, hipStreamSynchronize(st->stream), "scene upload") ||
-        (nt && !hip_ok(st, hipMemcpy(st->tris.ptr, sh.tri4.data(), sh.tri4.size() * 4, hipMemcpyHostToDevice), "triangle upload")) ||
-        (nt && !hip_ok(st, hipMemcpy(st->mats.ptr, sh.mat4.data(), sh.mat4.size() * 4, hipMemcpyHostToDevice), "material upload")) ||
+        (!sh.tri4.empty() && !hip_ok(st, hipMemcpy(st->tris.ptr, sh.tri4.data(), sh.tri4.size() * 4, hipMemcpyHostToDevice), "triangle upload")) ||
+        (!sh.mat4.empty() && !hip_ok(st, hipMemcpy(st->mats.ptr, sh.mat4.data(), sh.mat4.size() * 4, hipMemcpyHostToDevice), "material upload")) ||
         (!sh.nodes.empty() && !hip_ok(st, hipMemcpy(st->nodes.ptr, sh.nodes.data(), sh.nodes.size() * 4, hipMemcpyHostToDevice), "BVH upload")) ||
         (!sh.nodes4.empty() && !hip_ok(st, hipMemcpy(st->nodes4.ptr, sh.nodes4.data(), sh.nodes4.size() * 4, hipMemcpyHostToDevice), "BVH4 upload")) ||
         (!sh.lights.empty() && !hip_ok(st, hipMemcpy(st->lights.ptr, sh.lights.data(), sh.lights.size() * sizeof(LightEntry), hipMemcpyHostToDevice), "light upload")))
@@ -330,6 +396,16 @@ int32_t upload_scene(plugin_state* st) {
     st->bvh_sah_cost = sh.sah_cost;
     st->bvh_device_ms = sh.device_ms;
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (nt >= (1u << 20) && sh.builder == Q_BVH_BUILD_GPU) {
+        const BvhGpuTimes& g = sh.gpu_times;
+        log_msg(st, "scene: %u triangles built on the GPU in %.0f ms: allocate %.0f, mesh upload and corners %.0f, "
+                "tree %.0f (kernels %.0f), layouts and triangle arrays %.0f (kernels %.0f)", nt, st->bvh_build_ms,
+                g.alloc_ms, g.upload_ms, g.build_ms, g.device_ms, g.layout_ms, g.layout_device_ms);
+    } else if (nt >= (1u << 20)) {
+        log_msg(st, "scene: %u triangles built on the host in %.0f ms: corners %.0f, BVH %.0f, layout %.0f, "
+                "upload %.0f", nt, st->bvh_build_ms, sh.gather_ms, sh.bvh_ms, sh.layout_ms,
+                st->bvh_build_ms - sh.gather_ms - sh.bvh_ms - sh.layout_ms);
+    }
     return Q_EXT_OK;
 }
 

@@ -10,6 +10,7 @@
 
 #include "../../include/quasi/plugin_interface.h"
 #include "../../include/quasi/quasi_hip_ext.h"
+#include "bvh_gpu.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -42,13 +43,18 @@ struct TimedLaunch {
     hipEvent_t start, stop;
 };
 
-// One mesh prepared for upload in the layouts of common.hpp.
+// One mesh prepared in the layouts of common.hpp: host arrays to upload
+// (host build), or arrays already on the device (GPU build: tri4 .. nodes4
+// stay empty and `device` holds them until upload_scene adopts them).
 struct SceneHost {
     std::vector<float> tri4, mat4, nodes, nodes4;
     std::vector<LightEntry> lights;
+    DeviceScene device;
     uint32_t num_tris = 0, num_nodes = 0, num_nodes4 = 0, depth = 0, depth4 = 0;
     uint32_t builder = Q_BVH_BUILD_HOST;   // where the BVH was built
     double sah_cost = 0, device_ms = 0;
+    double gather_ms = 0, bvh_ms = 0, layout_ms = 0;   // host phases of a host build
+    BvhGpuTimes gpu_times;                               // phases of a GPU build
 };
 
 // Several scenes traced together (Q_plugin_set_batch).

@@ -29,6 +29,11 @@ def _ensure_built():
     if not (ROOT / "quasi_amd" / "libquasi_hip.so").exists():
         from quasi_amd import build
         build.build_plugin()
+    if not (ROOT / "tests" / "native" / "liblayoutprobe.so").exists():
+        from quasi_amd import build
+        if not (build.BUILD / "bvh_layout.hip.o").exists():
+            build.build_plugin()
+        build.build_test_probes()
 
 
 _ensure_built()

@@ -49,6 +49,9 @@ def test_gpu_and_host_bvh_render_alike(terrain):
             print(f"[terrain] builder {builder}: set_scene {time.time() - t0:.1f} s (build+upload "
                   f"{st.bvh_build_ms:.0f} ms, device {st.bvh_device_ms:.0f} ms), SAH {st.bvh_sah_cost:.2f}, "
                   f"nodes {st.bvh_nodes}, depth {st.bvh_depth}", flush=True)
+            for msg in r.messages:
+                if msg.startswith("scene:"):
+                    print(f"[terrain] {msg}", flush=True)
             assert st.bvh_builder == builder
             cam(r, terrain)
             r.set_params(spp_per_render=1, max_depth=6, rr_start=3, rng_mode=1, integrator=1, seed=2,
