@@ -359,6 +359,59 @@ __global__ __launch_bounds__(kBlock) void k_large_partition(LargeTask* tasks, co
     }
 }
 
+// SAH sweep over the 32 LDS bins of axis k, bins on lanes (each half-wave
+// holds the whole axis): prefix and suffix unions of the bin boxes and counts
+// by shuffles, then the first cheapest split plane s (bins < s go left) by a
+// reduction.  Returns split -1 when no plane has triangles on both sides.  The
+// costs are the serial sweep's: 1 + (A_left n_left + A_right n_right) / parent.
+__device__ __forceinline__ void sweep_axis(const Bins& lb, int k, float parent, float& out_cost, int& out_split,
+                                           uint32_t& out_nl) {
+    const int b = threadIdx.x & 31;
+    const uint32_t cnt = lb.cnt[k][b];
+    float pl[3], ph[3], sl[3], sh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        pl[a] = sl[a] = cnt ? funkey(lb.lo[k][b][a]) : INFINITY;
+        ph[a] = sh[a] = cnt ? funkey(lb.hi[k][b][a]) : -INFINITY;
+    }
+    uint32_t pc = cnt, scnt = cnt;   // bins 0..b and b..31
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const uint32_t up = __shfl_up(pc, o, 32), dn = __shfl_down(scnt, o, 32);
+        const bool take_up = b >= o, take_dn = b + o < 32;
+        if (take_up) pc += up;
+        if (take_dn) scnt += dn;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float ul = __shfl_up(pl[a], o, 32), uh = __shfl_up(ph[a], o, 32);
+            const float dl = __shfl_down(sl[a], o, 32), dh = __shfl_down(sh[a], o, 32);
+            if (take_up) { pl[a] = fminf(pl[a], ul); ph[a] = fmaxf(ph[a], uh); }
+            if (take_dn) { sl[a] = fminf(sl[a], dl); sh[a] = fmaxf(sh[a], dh); }
+        }
+    }
+    // plane s = b: left = bins 0..b-1 (the prefix one lane down), right = bins b..31
+    const uint32_t lrun = __shfl_up(pc, 1, 32);
+    float ll[3], lh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { ll[a] = __shfl_up(pl[a], 1, 32); lh[a] = __shfl_up(ph[a], 1, 32); }
+    float c = INFINITY;
+    if (b >= 1 && lrun != 0 && scnt != 0)
+        c = 1.f + (area_of(ll[0], ll[1], ll[2], lh[0], lh[1], lh[2]) * float(lrun) +
+                   area_of(sl[0], sl[1], sl[2], sh[0], sh[1], sh[2]) * float(scnt)) / parent;
+    int split = c < INFINITY ? b : 64;
+    uint32_t nl = lrun;
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+        const float c2 = __shfl_xor(c, o, 32);
+        const int s2 = __shfl_xor(split, o, 32);
+        const uint32_t n2 = __shfl_xor(nl, o, 32);
+        if (c2 < c || (c2 == c && s2 < split)) { c = c2; split = s2; nl = n2; }
+    }
+    out_cost = c;
+    out_split = split < 32 ? split : -1;
+    out_nl = nl;
+}
+
 // ---------------------------------------------------------------- subtrees (one wave each)
 struct SubTask {
     uint32_t node, begin, count, buf;
@@ -371,9 +424,6 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                                                 uint32_t* err) {
     __shared__ Bins lb;
     __shared__ SubTask stack[kStack];
-    __shared__ float cost_s[3];
-    __shared__ int split_s[3];
-    __shared__ uint32_t nl_s[3];
     const int lane = threadIdx.x;
     for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
         const NodeTask root = tasks[ti];
@@ -391,6 +441,7 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
             uint32_t* out = cur.buf ? ids_a : ids_b;
             // bounds
             int v[12];
+#pragma unroll
             for (int k = 0; k < 3; ++k) {
                 v[k] = kIntMax; v[3 + k] = -int(kIntMax) - 1; v[6 + k] = kIntMax; v[9 + k] = -int(kIntMax) - 1;
             }
@@ -399,6 +450,7 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                 if (p >= n) continue;
                 const float4 l = plo[p], h = phi[p];
                 const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const float c = 0.5f * (lo[k] + hi[k]);
                     v[k] = min(v[k], fkey(lo[k]));
@@ -407,9 +459,11 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     v[9 + k] = max(v[9 + k], fkey(c));
                 }
             }
+#pragma unroll
             for (int k = 0; k < 12; ++k)
                 v[k] = (k < 3 || (k >= 6 && k < 9)) ? wave_min_i(v[k]) : wave_max_i(v[k]);
             float blo[3], bhi[3], clo[3], sc[3];
+#pragma unroll
             for (int k = 0; k < 3; ++k) {
                 blo[k] = funkey(v[k]);
                 bhi[k] = funkey(v[3 + k]);
@@ -432,6 +486,7 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     if (p >= n) continue;
                     const float4 l = plo[p], h = phi[p];
                     const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
+#pragma unroll
                     for (int k = 0; k < 3; ++k) {
                         if (sc[k] == 0.f) continue;
                         const int b = bin_of(0.5f * (lo[k] + hi[k]), clo[k], sc[k]);
@@ -443,60 +498,24 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     }
                 }
                 __syncthreads();
-                // lanes 0..2: the sweep over one axis each
-                if (lane < 3) {
-                    const int k = lane;
-                    float bc = INFINITY;
-                    int bs = -1;
-                    uint32_t bnl = 0;
-                    if (sc[k] != 0.f) {
-                        const float parent = fmaxf(area_of(blo[0], blo[1], blo[2], bhi[0], bhi[1], bhi[2]), 1e-30f);
-                        float rarea[kBins];
-                        uint32_t rcnt[kBins];
-                        float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                        uint32_t run = 0;
-                        for (int b = kBins - 1; b >= 1; --b) {
-                            if (lb.cnt[k][b]) {
-                                for (int a = 0; a < 3; ++a) {
-                                    rl[a] = fminf(rl[a], funkey(lb.lo[k][b][a]));
-                                    rh[a] = fmaxf(rh[a], funkey(lb.hi[k][b][a]));
-                                }
-                            }
-                            run += lb.cnt[k][b];
-                            rarea[b] = area_of(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]);
-                            rcnt[b] = run;
-                        }
-                        float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                        uint32_t lrun = 0;
-                        for (int s = 1; s < kBins; ++s) {
-                            if (lb.cnt[k][s - 1]) {
-                                for (int a = 0; a < 3; ++a) {
-                                    ll[a] = fminf(ll[a], funkey(lb.lo[k][s - 1][a]));
-                                    lh[a] = fmaxf(lh[a], funkey(lb.hi[k][s - 1][a]));
-                                }
-                            }
-                            lrun += lb.cnt[k][s - 1];
-                            if (lrun == 0 || rcnt[s] == 0) continue;
-                            const float c = 1.f + (area_of(ll[0], ll[1], ll[2], lh[0], lh[1], lh[2]) * float(lrun) +
-                                                   rarea[s] * float(rcnt[s])) / parent;
-                            if (c < bc) { bc = c; bs = s; bnl = lrun; }
-                        }
-                    }
-                    cost_s[k] = bc;
-                    split_s[k] = bs;
-                    nl_s[k] = bnl;
+                // the sweep: one axis at a time, bins on lanes (both half-waves hold the same axis)
+                const float parent = fmaxf(area_of(blo[0], blo[1], blo[2], bhi[0], bhi[1], bhi[2]), 1e-30f);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (sc[k] == 0.f) continue;
+                    float c;
+                    int split;
+                    uint32_t nl;
+                    sweep_axis(lb, k, parent, c, split, nl);
+                    if (split >= 0 && c < best_cost) { best_cost = c; best_axis = k; best_split = split; best_nl = nl; }
                 }
-                __syncthreads();
-                for (int k = 0; k < 3; ++k)
-                    if (split_s[k] >= 0 && cost_s[k] < best_cost) {
-                        best_cost = cost_s[k]; best_axis = k; best_split = split_s[k]; best_nl = nl_s[k];
-                    }
                 __syncthreads();
             }
             const bool leaf = cnt <= 1u || (cnt <= uint32_t(kLeafMax) && (best_axis < 0 || float(cnt) <= best_cost));
             BinaryNode& nd = nodes[cur.node];
             if (leaf) {
                 if (lane == 0) {
+#pragma unroll
                     for (int k = 0; k < 3; ++k) { nd.lo[k] = blo[k]; nd.hi[k] = bhi[k]; }
                     nd.left = 0;
                     nd.right = 0;
@@ -508,8 +527,9 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                 const uint32_t nl = best_axis >= 0 ? best_nl : cnt / 2;
                 // stable partition into the other buffer
                 uint32_t lc = 0, rc = 0;
-                const float pclo = best_axis >= 0 ? clo[best_axis] : 0.f;
-                const float psc = best_axis >= 0 ? sc[best_axis] : 0.f;
+                // (selects rather than indexing keeps clo / sc in registers)
+                const float pclo = best_axis == 0 ? clo[0] : best_axis == 1 ? clo[1] : best_axis == 2 ? clo[2] : 0.f;
+                const float psc = best_axis == 0 ? sc[0] : best_axis == 1 ? sc[1] : best_axis == 2 ? sc[2] : 0.f;
                 const unsigned long long below = (1ull << lane) - 1ull;
                 for (uint32_t r0 = cur.begin; r0 < cur.begin + cnt; r0 += 64) {
                     const uint32_t i = r0 + lane;
@@ -546,23 +566,28 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     break;
                 }
                 if (lane == 0) {
+#pragma unroll
                     for (int k = 0; k < 3; ++k) { nd.lo[k] = blo[k]; nd.hi[k] = bhi[k]; }
                     nd.left = l;
                     nd.right = l + 1;
                     nd.first = 0;
                     nd.count = 0;
                 }
-                const SubTask a{l, cur.begin, nl, cur.buf ^ 1u};
-                const SubTask b{l + 1, cur.begin + nl, cnt - nl, cur.buf ^ 1u};
-                // continue with the smaller child, keep the larger
-                const bool a_small = a.count <= b.count;
+                // children: left (l, first nl triangles) and right (l + 1, the rest); continue
+                // with the smaller, keep the larger (field-wise selects keep both in registers)
+                const bool left_small = nl <= cnt - nl;
+                const uint32_t buf = cur.buf ^ 1u;
+                const SubTask big{left_small ? l + 1 : l, left_small ? cur.begin + nl : cur.begin,
+                                  left_small ? cnt - nl : nl, buf};
+                const SubTask small{left_small ? l : l + 1, left_small ? cur.begin : cur.begin + nl,
+                                    left_small ? nl : cnt - nl, buf};
                 if (sp >= kStack) {
                     if (lane == 0) atomicOr(err, kErrStack);
                     break;
                 }
-                if (lane == 0) stack[sp] = a_small ? b : a;
+                if (lane == 0) stack[sp] = big;
                 ++sp;
-                cur = a_small ? a : b;
+                cur = small;
                 __syncthreads();
                 continue;
             }
