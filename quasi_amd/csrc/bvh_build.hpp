@@ -55,6 +55,9 @@ BvhBuildResult convert_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, 
 // is a permutation of 0..ntri-1.  Requires (and checks) that every child has a
 // larger index than its parent, which both builders guarantee (slots are
 // taken from a counter when a node is split), so the checks run in parallel.
+// The GPU scene build applies the same checks on the device while it lays
+// the tree out (bvh_layout.hip); this host version states them for the CPU
+// tests of malformed trees.
 bool validate_binary_tree(const BinaryNode* nodes, uint32_t num_nodes, const uint32_t* prims, uint32_t ntri);
 
 }  // namespace quasi_hip

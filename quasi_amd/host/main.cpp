@@ -11,13 +11,21 @@
 //              [--spp-per-render S] [--depth D] [--rr-start R] [--seed X]
 //              [--integrator reference|nee] [--rng reference|hashed]
 //              [--engine megakernel|wavefront] [--compression zip|none]
-//              [--single-layer]
+//              [--single-layer] [--watch] [--reload-at K]
 //
 // Without a plugin path the library next to this executable's directory
 // (../libquasi_hip.so) is used.  The camera is the reference's orbit camera at
 // rest (main.cpp:38-45, 104-115), or the scene file's; it is marked dirty on
 // the first frame only, so the N frames accumulate.  The render parameters and
 // the scene go through the plugin's extension exports when given.
+//
+// The plugin is loaded through the hot-reload manager (manager.hpp, after the
+// reference's plugin/manager.hpp): from a private copy, so the library may be
+// rebuilt while the host runs.  --watch polls the library between frames and
+// reloads it when it changes; --reload-at K reloads it once before frame K.
+// Around a reload the host waits for the GPU, and afterwards hands the new
+// plugin the scene and parameters again and marks the camera dirty, so the
+// accumulation restarts from the reload.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -29,6 +37,7 @@
 #include "exr_writer.hpp"
 #include "hip_context.hpp"
 #include "loader.hpp"
+#include "manager.hpp"
 #include "quasi/quasi_hip_ext.h"
 #include "scene_file.hpp"
 
@@ -45,6 +54,8 @@ struct Options {
     long spp = -1, depth = -1, rr_start = -1, seed = -1, integrator = -1, rng = -1, engine = -1;
     Q::io::exr_compression compression = Q::io::exr_compression::zip;
     bool single_layer = false;
+    bool watch = false;
+    int reload_at = -1;
 };
 
 int usage(const char* argv0) {
@@ -52,7 +63,8 @@ int usage(const char* argv0) {
                  "usage: %s [plugin.so] --render N [--out file.exr | --out-dir DIR] [--width W] [--height H]\n"
                  "       [--device K] [--scene file.qscene] [--spp-per-render S] [--depth D] [--rr-start R]\n"
                  "       [--seed X] [--integrator reference|nee] [--rng reference|hashed]\n"
-                 "       [--engine megakernel|wavefront] [--compression zip|none] [--single-layer]\n",
+                 "       [--engine megakernel|wavefront] [--compression zip|none] [--single-layer]\n"
+                 "       [--watch] [--reload-at K]\n",
                  argv0);
     return EXIT_FAILURE;
 }
@@ -63,6 +75,7 @@ bool parse(int argc, char** argv, Options& o) {
         auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
         const char* v = nullptr;
         if (a == "--single-layer") { o.single_layer = true; continue; }
+        if (a == "--watch") { o.watch = true; continue; }
         if (a.size() > 2 && a.substr(0, 2) == "--") {
             v = next();
             if (!v) return false;
@@ -74,6 +87,7 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--height") o.height = uint32_t(std::atol(v));
         else if (a == "--device") o.device = std::atoi(v);
         else if (a == "--scene") o.scene = v;
+        else if (a == "--reload-at") o.reload_at = std::atoi(v);
         else if (a == "--spp-per-render") o.spp = std::atol(v);
         else if (a == "--depth") o.depth = std::atol(v);
         else if (a == "--rr-start") o.rr_start = std::atol(v);
@@ -109,13 +123,21 @@ int main(int argc, char** argv) {
     }
     auto& hip = *ctx_result;
 
-    auto lib_result = Q::plugin::dynamic_library::open(o.plugin);
-    if (!lib_result) {
-        std::fprintf(stderr, "Failed to load plugin library: %s (%s)\n",
-                     std::string(Q::plugin::to_string(lib_result.error())).c_str(), o.plugin.c_str());
-        return EXIT_FAILURE;
+    // camera: the reference orbit camera at rest, or the scene's
+    Q_camera cam{{0.0f, 1.0f, 3.5f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, 40.0f};
+    Q::io::scene_data scene;
+    if (!o.scene.empty()) {
+        auto sc = Q::io::read_scene(o.scene);
+        if (!sc) {
+            std::fprintf(stderr, "Scene %s: %s\n", o.scene.c_str(), Q::io::to_string(sc.error()));
+            return EXIT_FAILURE;
+        }
+        scene = std::move(*sc);
+        std::memcpy(cam.position, scene.camera, 3 * sizeof(float));
+        std::memcpy(cam.target, scene.camera + 3, 3 * sizeof(float));
+        std::memcpy(cam.up, scene.camera + 6, 3 * sizeof(float));
+        cam.fov = scene.camera[9];
     }
-    auto& lib = *lib_result;
 
     Q_plugin_context ctx{};
     ctx.viewport_width = o.width;
@@ -124,80 +146,106 @@ int main(int argc, char** argv) {
     ctx.gpu = hip.gpu();
     ctx.log = plugin_log;
     ctx.request_shutdown = plugin_request_shutdown;
-    auto plugin_result = Q::plugin::loader::load(lib, &ctx);
-    if (!plugin_result) {
-        std::fprintf(stderr, "Failed to load plugin: %s\n", Q::plugin::to_string(plugin_result.error()));
+
+    // The scene and parameters the plugin gets after every load.
+    Q::plugin::manager* mgr_ptr = nullptr;
+    bool camera_dirty = true;
+    auto configure = [&]() -> bool {
+        Q::plugin::manager& m = *mgr_ptr;
+        Q::plugin::loader* plugin = m.plugin();
+        const auto& lib = m.library();
+        if (!o.scene.empty()) {
+            auto set_scene = lib.get_symbol<int32_t (*)(Q_plugin_handle*, const Q_scene_desc*)>("Q_plugin_set_scene");
+            if (!set_scene) {
+                std::fprintf(stderr, "The plugin has no Q_plugin_set_scene: cannot load %s\n", o.scene.c_str());
+                return false;
+            }
+            Q_scene_desc d{};
+            d.struct_size = sizeof d;
+            d.num_vertices = uint32_t(scene.vertices.size() / 3);
+            d.num_triangles = uint32_t(scene.indices.size() / 3);
+            d.vertices = scene.vertices.data();
+            d.indices = scene.indices.data();
+            d.albedo = scene.albedo.data();
+            d.emission = scene.emission.data();
+            if ((*set_scene)(plugin->handle(), &d) != Q_EXT_OK) {
+                std::fprintf(stderr, "Q_plugin_set_scene failed\n");
+                return false;
+            }
+            std::printf("[Host] Scene %s: %u triangles\n", o.scene.c_str(), d.num_triangles);
+        }
+        if (o.spp > 0 || o.depth > 0 || o.rr_start >= 0 || o.seed >= 0 || o.integrator >= 0 || o.rng >= 0 ||
+            o.engine >= 0) {
+            auto get = lib.get_symbol<int32_t (*)(Q_plugin_handle*, Q_render_params*)>("Q_plugin_get_params");
+            auto set = lib.get_symbol<int32_t (*)(Q_plugin_handle*, const Q_render_params*)>("Q_plugin_set_params");
+            if (!get || !set) {
+                std::fprintf(stderr, "The plugin has no render-parameter exports\n");
+                return false;
+            }
+            Q_render_params p{};
+            p.struct_size = sizeof p;
+            if ((*get)(plugin->handle(), &p) != Q_EXT_OK) {
+                std::fprintf(stderr, "Q_plugin_get_params failed\n");
+                return false;
+            }
+            if (o.spp > 0) p.spp_per_render = uint32_t(o.spp);
+            if (o.depth > 0) p.max_depth = uint32_t(o.depth);
+            if (o.rr_start >= 0) p.rr_start = uint32_t(o.rr_start);
+            if (o.seed >= 0) p.seed = uint32_t(o.seed);
+            if (o.integrator >= 0) p.integrator = uint32_t(o.integrator);
+            if (o.rng >= 0) p.rng_mode = uint32_t(o.rng);
+            if (o.engine >= 0) p.engine = uint32_t(o.engine);
+            if ((*set)(plugin->handle(), &p) != Q_EXT_OK) {
+                std::fprintf(stderr, "Q_plugin_set_params rejected the parameters\n");
+                return false;
+            }
+        }
+        camera_dirty = true;   // the accumulation starts again with this plugin
+        const Q_plugin_info info = plugin->info();
+        std::printf("Loaded plugin: %s v%u.%u.%u\n", info.name, info.version.major, info.version.minor,
+                    info.version.patch);
+        std::printf("Description: %s\n", info.description);
+        return true;
+    };
+    bool configured = true;
+    Q::plugin::reload_hooks hooks;
+    hooks.pre_unload = [&] { hip.finish(); };   // nothing of the old plugin may still run
+    hooks.post_load = [&] { configured = configure(); };
+    hooks.on_error = [](const std::string& msg) { std::fprintf(stderr, "[Host] %s\n", msg.c_str()); };
+    Q::plugin::manager mgr(o.plugin, ctx, hooks);
+    mgr_ptr = &mgr;
+    if (auto r = mgr.load(); !r) {
+        std::fprintf(stderr, "Failed to load plugin: %s (%s)\n", Q::plugin::manager::to_string(r.error()),
+                     o.plugin.c_str());
         return EXIT_FAILURE;
     }
-    auto& plugin = *plugin_result;
-    const Q_plugin_info info = plugin.info();
-    std::printf("Loaded plugin: %s v%u.%u.%u\n", info.name, info.version.major, info.version.minor,
-                info.version.patch);
-    std::printf("Description: %s\n", info.description);
-
-    // camera: the reference orbit camera at rest, or the scene's
-    Q_camera cam{{0.0f, 1.0f, 3.5f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, 40.0f};
-    if (!o.scene.empty()) {
-        auto set_scene = lib.get_symbol<int32_t (*)(Q_plugin_handle*, const Q_scene_desc*)>("Q_plugin_set_scene");
-        if (!set_scene) {
-            std::fprintf(stderr, "The plugin has no Q_plugin_set_scene: cannot load %s\n", o.scene.c_str());
-            return EXIT_FAILURE;
-        }
-        auto sc = Q::io::read_scene(o.scene);
-        if (!sc) {
-            std::fprintf(stderr, "Scene %s: %s\n", o.scene.c_str(), Q::io::to_string(sc.error()));
-            return EXIT_FAILURE;
-        }
-        Q_scene_desc d{};
-        d.struct_size = sizeof d;
-        d.num_vertices = uint32_t(sc->vertices.size() / 3);
-        d.num_triangles = uint32_t(sc->indices.size() / 3);
-        d.vertices = sc->vertices.data();
-        d.indices = sc->indices.data();
-        d.albedo = sc->albedo.data();
-        d.emission = sc->emission.data();
-        if ((*set_scene)(plugin.handle(), &d) != Q_EXT_OK) {
-            std::fprintf(stderr, "Q_plugin_set_scene failed\n");
-            return EXIT_FAILURE;
-        }
-        std::memcpy(cam.position, sc->camera, 3 * sizeof(float));
-        std::memcpy(cam.target, sc->camera + 3, 3 * sizeof(float));
-        std::memcpy(cam.up, sc->camera + 6, 3 * sizeof(float));
-        cam.fov = sc->camera[9];
-        std::printf("[Host] Scene %s: %u triangles\n", o.scene.c_str(), d.num_triangles);
-    }
-    if (o.spp > 0 || o.depth > 0 || o.rr_start >= 0 || o.seed >= 0 || o.integrator >= 0 || o.rng >= 0 || o.engine >= 0) {
-        auto get = lib.get_symbol<int32_t (*)(Q_plugin_handle*, Q_render_params*)>("Q_plugin_get_params");
-        auto set = lib.get_symbol<int32_t (*)(Q_plugin_handle*, const Q_render_params*)>("Q_plugin_set_params");
-        if (!get || !set) {
-            std::fprintf(stderr, "The plugin has no render-parameter exports\n");
-            return EXIT_FAILURE;
-        }
-        Q_render_params p{};
-        p.struct_size = sizeof p;
-        if ((*get)(plugin.handle(), &p) != Q_EXT_OK) {
-            std::fprintf(stderr, "Q_plugin_get_params failed\n");
-            return EXIT_FAILURE;
-        }
-        if (o.spp > 0) p.spp_per_render = uint32_t(o.spp);
-        if (o.depth > 0) p.max_depth = uint32_t(o.depth);
-        if (o.rr_start >= 0) p.rr_start = uint32_t(o.rr_start);
-        if (o.seed >= 0) p.seed = uint32_t(o.seed);
-        if (o.integrator >= 0) p.integrator = uint32_t(o.integrator);
-        if (o.rng >= 0) p.rng_mode = uint32_t(o.rng);
-        if (o.engine >= 0) p.engine = uint32_t(o.engine);
-        if ((*set)(plugin.handle(), &p) != Q_EXT_OK) {
-            std::fprintf(stderr, "Q_plugin_set_params rejected the parameters\n");
-            return EXIT_FAILURE;
-        }
-    }
+    if (!configure()) return EXIT_FAILURE;
 
     std::printf("[Host] Batch mode: rendering %d frames then saving EXR\n", o.frames);
     auto last = std::chrono::steady_clock::now();
     const auto t0 = last;
+    int since_load = 0;   // frames accumulated by the current plugin
     for (int i = 0; i < o.frames; ++i) {
+        bool reloaded = false;
+        if (i == o.reload_at) {
+            (void)mgr.reload();
+            reloaded = true;
+        } else if (o.watch) {
+            reloaded = mgr.poll();
+        }
+        if (reloaded) {
+            const auto& st = mgr.stats();
+            std::printf("[Host] Plugin reloaded before frame %d (%llu reloads, %llu ok, %llu failed, %.1f ms)\n", i,
+                        (unsigned long long)st.reload_count, (unsigned long long)st.success_count,
+                        (unsigned long long)st.failure_count, st.last_reload_time * 1000.f);
+            if (!mgr.is_loaded() || !configured) {
+                std::fprintf(stderr, "[Host] No plugin after the reload\n");
+                return EXIT_FAILURE;
+            }
+            since_load = 0;
+        }
         const auto now = std::chrono::steady_clock::now();
-        plugin.update(std::chrono::duration<float>(now - last).count());
+        mgr.update(std::chrono::duration<float>(now - last).count());
         last = now;
         auto fr = hip.begin_frame();
         if (!fr) {
@@ -206,9 +254,11 @@ int main(int argc, char** argv) {
         }
         Q_render_frame& frame = *fr;
         frame.camera = cam;
-        frame.camera_dirty = i == 0 ? 1u : 0u;
-        plugin.render(&frame);
+        frame.camera_dirty = camera_dirty ? 1u : 0u;
+        camera_dirty = false;
+        mgr.render(&frame);
         hip.end_frame(frame);
+        ++since_load;
     }
     hip.finish();
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -216,8 +266,9 @@ int main(int argc, char** argv) {
 
     const auto path = o.out.empty() ? Q::io::make_timestamped_path(o.out_dir) : o.out;
     int rc = EXIT_SUCCESS;
+    Q::plugin::loader& plugin = *mgr.plugin();
     if (plugin.supports_readback_aov() && !o.single_layer) {
-        std::printf("[Host] Saving EXR with AOVs (%d samples)...\n", o.frames);
+        std::printf("[Host] Saving EXR with AOVs (%d samples)...\n", since_load);
         Q_readback_aov_result rb = plugin.readback_aov();
         if (rb.buffers[Q_AOV_BEAUTY].data) {
             auto w = Q::io::write_exr(path, rb, o.compression);
@@ -229,7 +280,7 @@ int main(int argc, char** argv) {
             rc = EXIT_FAILURE;
         }
     } else if (plugin.supports_readback()) {
-        std::printf("[Host] Saving EXR (%d samples)...\n", o.frames);
+        std::printf("[Host] Saving EXR (%d samples)...\n", since_load);
         Q_readback_result rb = plugin.readback();
         if (rb.data) {
             auto w = Q::io::write_exr(path, rb, o.compression);
@@ -243,6 +294,5 @@ int main(int argc, char** argv) {
     } else {
         std::printf("[Host] Plugin does not support HDR readback\n");
     }
-    plugin.destroy();
     return rc;
 }

@@ -64,3 +64,26 @@ def test_host_scene_file_nee(tmp_path, engine):
         r.render(2)
         aov = r.readback_aov()
     check_exr(out, aov)
+
+
+@pytest.mark.parametrize("engine", ["megakernel", "wavefront"])
+def test_host_hot_reload_restarts_identically(tmp_path, engine):
+    """--reload-at K: before frame K the host waits for the GPU, the plugin is
+    destroyed and its library closed, a fresh copy of the library is opened
+    (hot reload, quasi_amd/host/manager.hpp) and handed the scene and
+    parameters again, and a new accumulation starts.  After N frames the image
+    is that of the N - K frames since the reload: bit for bit the EXR of a run
+    of N - K frames without a reload."""
+    w = h = 64
+    mesh = scenes.cornell36(1)
+    sf = tmp_path / "c36.qscene"
+    scenes.save_qscene(mesh, sf)
+    common = [str(ROOT / "quasi_amd" / "libquasi_hip.so"), "--width", str(w), "--height", str(h), "--scene", str(sf),
+              "--spp-per-render", "2", "--depth", "4", "--integrator", "nee", "--rng", "hashed", "--seed", "5",
+              "--engine", engine, "--compression", "none"]
+    plain, reloaded = tmp_path / "plain.exr", tmp_path / "reloaded.exr"
+    run_host(common + ["--render", "3", "--out", str(plain)])
+    log = run_host(common + ["--render", "5", "--reload-at", "2", "--out", str(reloaded)])
+    assert "Plugin reloaded before frame 2 (1 reloads, 1 ok, 0 failed" in log
+    assert log.count("Loaded plugin:") == 2 and "Saving EXR with AOVs (3 samples)" in log
+    assert plain.read_bytes() == reloaded.read_bytes()
