@@ -154,6 +154,14 @@ int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_params* para
  * id_sample for every pixel, row-major top-to-bottom.  Synchronises. */
 int32_t Q_plugin_readback_ids(Q_plugin_handle* handle, uint32_t* out, uint64_t count);
 
+/* Replaces the display pass of plugin.mm:334-350 (tonemap_frag, drawn to the
+ * drawable): the beauty mean, Reinhard c / (c + 1), gamma 1/2.2, as 8-bit RGBA
+ * (unorm: clamped, times 255, rounded to nearest even; alpha 255), row-major
+ * top-to-bottom, 4 * width * height bytes into out.  Synchronises.
+ * Q_EXT_ERR_ARGUMENT when out holds fewer bytes, Q_EXT_ERR_STATE before the
+ * first sample. */
+int32_t Q_plugin_readback_display(Q_plugin_handle* handle, uint8_t* out, uint64_t bytes);
+
 /* Synchronises the plugin's streams, then fills *out. */
 int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out);
 int32_t Q_plugin_reset_stats(Q_plugin_handle* handle);

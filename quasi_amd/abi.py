@@ -165,7 +165,7 @@ REQUIRED_SYMBOLS = (K_SYMBOL_ABI_VERSION, K_SYMBOL_GET_INFO, K_SYMBOL_CREATE, K_
 OPTIONAL_SYMBOLS = (K_SYMBOL_READBACK, K_SYMBOL_READBACK_FREE, K_SYMBOL_READBACK_AOV,
                     K_SYMBOL_READBACK_AOV_FREE)
 EXTENSION_SYMBOLS = ("Q_plugin_set_scene", "Q_plugin_get_params", "Q_plugin_set_params",
-                     "Q_plugin_readback_ids", "Q_plugin_get_stats", "Q_plugin_reset_stats",
+                     "Q_plugin_readback_ids", "Q_plugin_readback_display", "Q_plugin_get_stats", "Q_plugin_reset_stats",
                      "Q_plugin_synchronize", "Q_plugin_set_batch", "Q_plugin_render_batch",
                      "Q_plugin_readback_batch", "Q_plugin_readback_batch_ids", "Q_plugin_autotune",
                      "Q_plugin_adjoint")

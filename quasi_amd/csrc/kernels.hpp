@@ -53,6 +53,8 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
 
 // out[i] = sum[i] / n for npix RGBA pixels (the running mean of plugin.mm:326-327).
 hipError_t launch_resolve(const float4* sum, float4* out, size_t npix, uint32_t n, hipStream_t stream);
+// 8-bit display image (RGBA, 4 bytes per pixel) of the mean of n samples (plugin.mm:334-350).
+hipError_t launch_tonemap(const float4* sum, void* out, size_t npix, uint32_t n, hipStream_t stream);
 
 // Path-replay adjoint over the whole framebuffer with `blocks` blocks of
 // adjoint_block_size() threads (blocks * adjoint_block_size() must not exceed

@@ -917,6 +917,36 @@ Q_EXPORT int32_t Q_plugin_readback_ids(Q_plugin_handle* handle, uint32_t* out, u
     return Q_EXT_OK;
 }
 
+Q_EXPORT int32_t Q_plugin_readback_display(Q_plugin_handle* handle, uint8_t* out, uint64_t bytes) {
+    if (!handle || !out) return Q_EXT_ERR_ARGUMENT;
+    plugin_state* st = as_state(handle);
+    const uint64_t npix = uint64_t(st->width) * st->height;
+    if (bytes < 4 * npix) {
+        log_msg(st, "readback_display: buffer holds %llu bytes, %llu needed", (unsigned long long)bytes,
+                (unsigned long long)(4 * npix));
+        return Q_EXT_ERR_ARGUMENT;
+    }
+    (void)hipSetDevice(st->device);
+    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    if (st->sample_count == 0 || npix == 0) {
+        log_msg(st, "readback_display: nothing rendered yet");
+        return Q_EXT_ERR_STATE;
+    }
+    if (!ensure_buffer(st, st->resolved, 16 * npix, "readback staging")) return Q_EXT_ERR_OUT_OF_MEMORY;
+    {
+        LaunchTimer lt(st, st->stream, K_RESOLVE);
+        if (!hip_ok(st, launch_tonemap(static_cast<const float4*>(st->accum[Q_AOV_BEAUTY].ptr), st->resolved.ptr, npix,
+                                       st->sample_count, st->stream),
+                    "tonemap"))
+            return Q_EXT_ERR_DEVICE;
+    }
+    if (!hip_ok(st, hipMemcpyAsync(out, st->resolved.ptr, 4 * npix, hipMemcpyDeviceToHost, st->stream),
+                "display readback") ||
+        !hip_ok(st, hipStreamSynchronize(st->stream), "display readback"))
+        return Q_EXT_ERR_DEVICE;
+    return Q_EXT_OK;
+}
+
 Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out) {
     if (!handle || !out || out->struct_size < kStatsMinSize) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);

@@ -11,7 +11,8 @@
 //              [--spp-per-render S] [--depth D] [--rr-start R] [--seed X]
 //              [--integrator reference|nee] [--rng reference|hashed]
 //              [--engine megakernel|wavefront] [--compression zip|none]
-//              [--single-layer] [--watch] [--reload-at K]
+//              [--single-layer] [--watch] [--reload-at K] [--ppm file.ppm]
+//              [--azimuth A] [--elevation E] [--distance D] [--orbit-step S]
 //
 // Without a plugin path the library next to this executable's directory
 // (../libquasi_hip.so) is used.  The camera is the reference's orbit camera at
@@ -26,12 +27,25 @@
 // Around a reload the host waits for the GPU, and afterwards hands the new
 // plugin the scene and parameters again and marks the camera dirty, so the
 // accumulation restarts from the reload.
+//
+// The reference's orbit camera (main.cpp:39-112: spherical coordinates around
+// a target, elevation clamped to +-1.5 rad, up +y) places the camera when any
+// of --azimuth / --elevation / --distance / --orbit-step is given (target and
+// fov from the scene file, else (0, 1, 0) and 40 degrees); --orbit-step turns
+// it by S radians of azimuth before every frame after the first, standing in
+// for a mouse drag, and marks the camera dirty, so the accumulation restarts
+// as in the reference (plugin.mm:643-646).  --ppm also writes the display
+// image (Q_plugin_readback_display: the tonemap pass of plugin.mm:334-350) as
+// a binary PPM.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <string_view>
+#include <vector>
 
 #include "dynamic_library.hpp"
 #include "exr_writer.hpp"
@@ -56,7 +70,46 @@ struct Options {
     bool single_layer = false;
     bool watch = false;
     int reload_at = -1;
+    std::filesystem::path ppm;
+    bool orbit = false;
+    float azimuth = 0.f, elevation = 0.f, distance = 3.5f, orbit_step = 0.f;
 };
+
+// The reference host's orbit camera (main.cpp:39-112) without the mouse input.
+struct orbit_camera {
+    float target[3] = {0.0f, 1.0f, 0.0f};
+    float distance = 3.5f;
+    float azimuth = 0.0f;     // radians
+    float elevation = 0.0f;   // radians, clamped to +-1.5
+    float fov = 40.0f;
+
+    Q_camera camera() {
+        elevation = std::min(std::max(elevation, -1.5f), 1.5f);
+        const float ce = std::cos(elevation);
+        Q_camera c{};
+        c.position[0] = target[0] + distance * std::sin(azimuth) * ce;
+        c.position[1] = target[1] + distance * std::sin(elevation);
+        c.position[2] = target[2] + distance * std::cos(azimuth) * ce;
+        for (int k = 0; k < 3; ++k) c.target[k] = target[k];
+        c.up[0] = 0.0f;
+        c.up[1] = 1.0f;
+        c.up[2] = 0.0f;
+        c.fov = fov;
+        return c;
+    }
+};
+
+// Binary PPM (P6) of the RGB channels of an RGBA8 image, top row first.
+bool write_ppm(const std::filesystem::path& path, const std::vector<uint8_t>& rgba, uint32_t w, uint32_t h) {
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    std::fprintf(f, "P6\n%u %u\n255\n", w, h);
+    std::vector<uint8_t> rgb(3 * size_t(w) * h);
+    for (size_t i = 0; i < size_t(w) * h; ++i)
+        for (int k = 0; k < 3; ++k) rgb[3 * i + k] = rgba[4 * i + k];
+    const bool ok = std::fwrite(rgb.data(), 1, rgb.size(), f) == rgb.size();
+    return std::fclose(f) == 0 && ok;
+}
 
 int usage(const char* argv0) {
     std::fprintf(stderr,
@@ -64,7 +117,8 @@ int usage(const char* argv0) {
                  "       [--device K] [--scene file.qscene] [--spp-per-render S] [--depth D] [--rr-start R]\n"
                  "       [--seed X] [--integrator reference|nee] [--rng reference|hashed]\n"
                  "       [--engine megakernel|wavefront] [--compression zip|none] [--single-layer]\n"
-                 "       [--watch] [--reload-at K]\n",
+                 "       [--watch] [--reload-at K] [--ppm file.ppm]\n"
+                 "       [--azimuth A] [--elevation E] [--distance D] [--orbit-step S]\n",
                  argv0);
     return EXIT_FAILURE;
 }
@@ -88,6 +142,11 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--device") o.device = std::atoi(v);
         else if (a == "--scene") o.scene = v;
         else if (a == "--reload-at") o.reload_at = std::atoi(v);
+        else if (a == "--ppm") o.ppm = v;
+        else if (a == "--azimuth") { o.azimuth = std::strtof(v, nullptr); o.orbit = true; }
+        else if (a == "--elevation") { o.elevation = std::strtof(v, nullptr); o.orbit = true; }
+        else if (a == "--distance") { o.distance = std::strtof(v, nullptr); o.orbit = true; }
+        else if (a == "--orbit-step") { o.orbit_step = std::strtof(v, nullptr); o.orbit = true; }
         else if (a == "--spp-per-render") o.spp = std::atol(v);
         else if (a == "--depth") o.depth = std::atol(v);
         else if (a == "--rr-start") o.rr_start = std::atol(v);
@@ -137,6 +196,15 @@ int main(int argc, char** argv) {
         std::memcpy(cam.target, scene.camera + 3, 3 * sizeof(float));
         std::memcpy(cam.up, scene.camera + 6, 3 * sizeof(float));
         cam.fov = scene.camera[9];
+    }
+    orbit_camera orbit;
+    if (o.orbit) {
+        for (int k = 0; k < 3; ++k) orbit.target[k] = cam.target[k];
+        orbit.fov = cam.fov;
+        orbit.distance = o.distance;
+        orbit.azimuth = o.azimuth;
+        orbit.elevation = o.elevation;
+        cam = orbit.camera();
     }
 
     Q_plugin_context ctx{};
@@ -244,6 +312,12 @@ int main(int argc, char** argv) {
             }
             since_load = 0;
         }
+        if (o.orbit && o.orbit_step != 0.f && i > 0) {   // the drag between two frames
+            orbit.azimuth += o.orbit_step;
+            cam = orbit.camera();
+            camera_dirty = true;
+            since_load = 0;
+        }
         const auto now = std::chrono::steady_clock::now();
         mgr.update(std::chrono::duration<float>(now - last).count());
         last = now;
@@ -267,6 +341,18 @@ int main(int argc, char** argv) {
     const auto path = o.out.empty() ? Q::io::make_timestamped_path(o.out_dir) : o.out;
     int rc = EXIT_SUCCESS;
     Q::plugin::loader& plugin = *mgr.plugin();
+    if (!o.ppm.empty()) {
+        auto display = mgr.library().get_symbol<int32_t (*)(Q_plugin_handle*, uint8_t*, uint64_t)>(
+            "Q_plugin_readback_display");
+        std::vector<uint8_t> rgba(4 * size_t(o.width) * o.height);
+        if (!display || (*display)(plugin.handle(), rgba.data(), rgba.size()) != Q_EXT_OK ||
+            !write_ppm(o.ppm, rgba, o.width, o.height)) {
+            std::fprintf(stderr, "[Host] Display image not written: %s\n", o.ppm.c_str());
+            rc = EXIT_FAILURE;
+        } else {
+            std::printf("[Host] Saved display image: %s\n", o.ppm.c_str());
+        }
+    }
     if (plugin.supports_readback_aov() && !o.single_layer) {
         std::printf("[Host] Saving EXR with AOVs (%d samples)...\n", since_load);
         Q_readback_aov_result rb = plugin.readback_aov();

@@ -266,6 +266,7 @@ class HipRenderer:
         self._get_params = g("Q_plugin_get_params", C.c_int32, [_H, P(abi.Q_render_params)])
         self._set_params = g("Q_plugin_set_params", C.c_int32, [_H, P(abi.Q_render_params)])
         self._readback_ids = g("Q_plugin_readback_ids", C.c_int32, [_H, P(C.c_uint32), C.c_uint64])
+        self._readback_display = g("Q_plugin_readback_display", C.c_int32, [_H, P(C.c_uint8), C.c_uint64])
         self._get_stats = g("Q_plugin_get_stats", C.c_int32, [_H, P(abi.Q_render_stats)])
         self._reset_stats = g("Q_plugin_reset_stats", C.c_int32, [_H])
         self._sync = g("Q_plugin_synchronize", C.c_int32, [_H])
@@ -383,6 +384,14 @@ class HipRenderer:
         self._check(self._readback_ids(self.plugin.handle, out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size),
                     "readback_ids")
         return out.reshape(self.height, self.width)
+
+    def readback_display(self) -> np.ndarray:
+        """The 8-bit display image (Reinhard + gamma 1/2.2, plugin.mm:334-350) as
+        (height, width, 4) uint8, top row first."""
+        out = np.empty((self.height, self.width, 4), dtype=np.uint8)
+        self._check(self._readback_display(self.plugin.handle, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size),
+                    "readback_display")
+        return out
 
     # -- reference ABI ---------------------------------------------------
     def frame(self, camera_dirty: Optional[bool] = None, stream: Optional[int] = None) -> abi.Q_render_frame:
