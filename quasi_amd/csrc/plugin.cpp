@@ -358,6 +358,9 @@ int32_t upload_scene(plugin_state* st) {
     int32_t rc = prepare_scene(st, st->verts.data(), nv, st->indices.data(), nt, st->albedo.data(),
                                st->emission.data(), 0, sh, st->params.bvh_builder);
     if (rc != Q_EXT_OK) return rc;
+    // From here on the device arrays change: until every one of them holds
+    // the new scene, nothing may be traced against them.
+    st->scene_ok = false;
     if (sh.builder == Q_BVH_BUILD_GPU) {
         // adopt the device arrays of the GPU build (the stream is idle: the build synchronised it)
         DeviceScene& d = sh.device;
@@ -396,6 +399,7 @@ int32_t upload_scene(plugin_state* st) {
     st->bvh_sah_cost = sh.sah_cost;
     st->bvh_device_ms = sh.device_ms;
     st->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st->scene_ok = true;
     if (nt >= (1u << 20) && sh.builder == Q_BVH_BUILD_GPU) {
         const BvhGpuTimes& g = sh.gpu_times;
         log_msg(st, "scene: %u triangles built on the GPU in %.0f ms: allocate %.0f, mesh upload and corners %.0f, "
@@ -577,6 +581,10 @@ bool clear_single(plugin_state* st, hipStream_t s) { return clear_accumulation(s
 
 bool render_frame(plugin_state* st, const Q_render_frame* frame) {
     if (frame->width == 0 || frame->height == 0) return false;
+    if (!st->scene_ok) {
+        log_msg(st, "render: no scene on the device (the last scene upload failed)");
+        return false;
+    }
     (void)hipSetDevice(st->device);
     hipStream_t s = frame->command_buffer ? static_cast<hipStream_t>(frame->command_buffer) : st->stream;
     if (st->last_stream && st->last_stream != s) (void)hipStreamSynchronize(st->last_stream);
@@ -839,12 +847,25 @@ Q_EXPORT int32_t Q_plugin_set_scene(Q_plugin_handle* handle, const Q_scene_desc*
         (void)hipSetDevice(st->device);
         if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
         const size_t nv = scene->num_vertices, nt = scene->num_triangles;
-        st->verts.assign(scene->vertices, scene->vertices + 3 * nv);
-        st->indices.assign(scene->indices, scene->indices + 3 * nt);
-        st->albedo.assign(scene->albedo, scene->albedo + 3 * nt);
-        st->emission.assign(scene->emission, scene->emission + 3 * nt);
+        std::vector<float> verts(scene->vertices, scene->vertices + 3 * nv);
+        std::vector<uint32_t> indices(scene->indices, scene->indices + 3 * nt);
+        std::vector<float> albedo(scene->albedo, scene->albedo + 3 * nt);
+        std::vector<float> emission(scene->emission, scene->emission + 3 * nt);
+        std::swap(st->verts, verts);
+        std::swap(st->indices, indices);
+        std::swap(st->albedo, albedo);
+        std::swap(st->emission, emission);
         int32_t rc = upload_scene(st);
-        if (rc != Q_EXT_OK) return rc;
+        if (rc != Q_EXT_OK) {
+            if (st->scene_ok) {
+                // rejected before the device arrays changed: the previous scene stays, host copy included
+                std::swap(st->verts, verts);
+                std::swap(st->indices, indices);
+                std::swap(st->albedo, albedo);
+                std::swap(st->emission, emission);
+            }
+            return rc;
+        }
         if (!clear_accumulation(st, st->stream)) return Q_EXT_ERR_DEVICE;
         return Q_EXT_OK;
     } catch (const std::bad_alloc&) {

@@ -33,7 +33,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
         return Q_EXT_ERR_ARGUMENT;
     }
     const Q_render_params& rp = st->params;
-    if (st->num_tris == 0 || !st->have_camera || st->width == 0 || st->height == 0) {
+    if (st->num_tris == 0 || !st->scene_ok || !st->have_camera || st->width == 0 || st->height == 0) {
         log_msg(st, "adjoint: needs a scene and a rendered frame (camera and resolution)");
         return Q_EXT_ERR_STATE;
     }

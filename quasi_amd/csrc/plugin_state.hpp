@@ -130,6 +130,7 @@ struct plugin_state {
     BatchState batch;
     AdjointState adjoint;
     uint64_t scene_generation = 0;      // incremented by every scene upload
+    bool scene_ok = false;              // the device scene is whole (false after a failed upload)
     Q_camera last_camera{};
     bool have_camera = false;
 

@@ -94,3 +94,34 @@ def test_ragged_sample_passes():
     """7 samples in passes of 3 (3 + 3 + 1) on the wavefront engine."""
     render_and_compare(scenes.cornell36(4), 40, 30, 7, abi.Q_ENGINE_WAVEFRONT, "passes", samples_per_pass=3,
                        max_depth=5)
+
+
+def test_rejected_scene_keeps_the_previous_one():
+    """A scene with a vertex index out of range is rejected before anything
+    on the device changes: renders and the adjoint then still use the previous
+    scene (device arrays and host mesh alike).  Renders agree bit for bit; the
+    adjoint's FP32 atomic sums depend on the order waves arrive in, so two
+    calls agree to a few units in the last place (rtol 1e-6)."""
+    good = scenes.cornell36(1)
+    bad = scenes.Mesh(good.vertices.copy(), good.indices.copy(), good.albedo.copy(), good.emission.copy(),
+                      dict(good.camera), "bad")
+    bad.indices[5, 1] = good.vertices.shape[0]
+    w, h = 32, 24
+    target = np.full((h, w, 4), 0.25, dtype=np.float32)
+    with HipRenderer(w, h) as r:
+        r.set_scene(good)
+        r.camera = cam_of(good)
+        r.set_params(spp_per_render=2, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=4)
+        r.render(2)
+        before = r.readback()
+        ga0, gv0, _ = r.adjoint(target, 2, mode=abi.Q_ADJOINT_L2_TARGET)
+        with pytest.raises(RuntimeError):
+            r.set_scene(bad)
+        say(f"[rejected] {r.messages[-1]}")
+        r.set_params(seed=4)                     # restart the accumulation
+        r.render(2)
+        after = r.readback()
+        ga1, gv1, _ = r.adjoint(target, 2, mode=abi.Q_ADJOINT_L2_TARGET)
+    assert np.array_equal(before, after)
+    assert np.allclose(ga0, ga1, rtol=1e-6, atol=1e-9) and np.allclose(gv0, gv1, rtol=1e-6, atol=1e-9)
+    assert np.abs(ga0).max() > 0 and np.abs(gv0).max() > 0
