@@ -110,15 +110,83 @@ __global__ void k_prim_bounds(const float* __restrict__ p0, const float* __restr
     }
 }
 
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
+// ---- cross-lane operations on DPP (data-parallel primitives: VALU operands read
+// from another lane of the same 16-lane row, or broadcast between rows), which
+// keep these reductions and scans off the LDS crossbar.  Every caller runs them
+// with all 64 lanes active.
+constexpr int kDppQuadXor1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppWaveShr1 = 0x138;    // lane i reads lane i - 1 of the wave
+constexpr int kDppRowMirror = 0x140;   // lane i of a row reads lane 15 - i
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowBcast15 = 0x142;  // lane 15 of each row to the next row
+constexpr int kDppRowBcast31 = 0x143;  // lane 31 to rows 2 and 3
+
+// The value of lane (per CTRL) in the lanes the masks enable, `identity` where
+// the source lane lies outside the row or the row is masked off.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_i(int identity, int v) {
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROW_MASK, 0xF, false);
 }
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f(float identity, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(identity), __float_as_int(v), CTRL, ROW_MASK,
+                                                      0xF, false));
+}
+
+struct OpMinI { __device__ static int id() { return int(kIntMax); } __device__ static int f(int a, int b) { return min(a, b); } };
+struct OpMaxI { __device__ static int id() { return -int(kIntMax) - 1; } __device__ static int f(int a, int b) { return max(a, b); } };
+
+// Reduction over all 64 lanes; the result is in lane 63, returned wave-uniform.
+template <class Op>
+__device__ __forceinline__ int wave_reduce_i(int v) {
+    v = Op::f(v, dpp_i<kDppQuadXor1>(Op::id(), v));
+    v = Op::f(v, dpp_i<kDppQuadXor2>(Op::id(), v));
+    v = Op::f(v, dpp_i<kDppRowHalfMirror>(Op::id(), v));
+    v = Op::f(v, dpp_i<kDppRowMirror>(Op::id(), v));          // every lane of a row: the row's result
+    v = Op::f(v, dpp_i<kDppRowBcast15, 0xA>(Op::id(), v));    // rows 1, 3: rows 0-1, 2-3
+    v = Op::f(v, dpp_i<kDppRowBcast31, 0xC>(Op::id(), v));    // rows 2, 3: all rows
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_i<OpMinI>(v); }
+__device__ __forceinline__ int wave_max_i(int v) { return wave_reduce_i<OpMaxI>(v); }
+struct OpAddI { __device__ static int id() { return 0; } __device__ static int f(int a, int b) { return a + b; } };
+__device__ __forceinline__ int wave_reduce_add(int v) { return wave_reduce_i<OpAddI>(v); }
+
+// Inclusive scans within each 32-lane half of the wave (lanes 0-31, 32-63).
+__device__ __forceinline__ uint32_t half_scan_add(uint32_t v) {
+    int x = int(v);
+    x += dpp_i<kDppRowShr1>(0, x);
+    x += dpp_i<kDppRowShr2>(0, x);
+    x += dpp_i<kDppRowShr4>(0, x);
+    x += dpp_i<kDppRowShr8>(0, x);
+    x += dpp_i<kDppRowBcast15, 0xA>(0, x);
+    return uint32_t(x);
+}
+__device__ __forceinline__ float half_scan_min(float x) {
+    x = fminf(x, dpp_f<kDppRowShr1>(INFINITY, x));
+    x = fminf(x, dpp_f<kDppRowShr2>(INFINITY, x));
+    x = fminf(x, dpp_f<kDppRowShr4>(INFINITY, x));
+    x = fminf(x, dpp_f<kDppRowShr8>(INFINITY, x));
+    return fminf(x, dpp_f<kDppRowBcast15, 0xA>(INFINITY, x));
+}
+__device__ __forceinline__ float half_scan_max(float x) {
+    x = fmaxf(x, dpp_f<kDppRowShr1>(-INFINITY, x));
+    x = fmaxf(x, dpp_f<kDppRowShr2>(-INFINITY, x));
+    x = fmaxf(x, dpp_f<kDppRowShr4>(-INFINITY, x));
+    x = fmaxf(x, dpp_f<kDppRowShr8>(-INFINITY, x));
+    return fmaxf(x, dpp_f<kDppRowBcast15, 0xA>(-INFINITY, x));
+}
+
+// Minimum over lanes 0-31, returned wave-uniform.
+__device__ __forceinline__ int half_min_i(int v) {
+    v = min(v, dpp_i<kDppQuadXor1>(int(kIntMax), v));
+    v = min(v, dpp_i<kDppQuadXor2>(int(kIntMax), v));
+    v = min(v, dpp_i<kDppRowHalfMirror>(int(kIntMax), v));
+    v = min(v, dpp_i<kDppRowMirror>(int(kIntMax), v));
+    v = min(v, dpp_i<kDppRowBcast15, 0xA>(int(kIntMax), v));
+    return __builtin_amdgcn_readlane(v, 31);
 }
 
 // ---------------------------------------------------------------- large nodes
@@ -302,6 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_large_partition(LargeTask* tasks, co
                                                             const float4* __restrict__ plo,
                                                             const float4* __restrict__ phi, uint32_t* ids_a,
                                                             uint32_t* ids_b, uint32_t n, uint32_t* err) {
+    static_assert(kChunk <= 32 * kBlock, "one side bit per element in a 32-bit mask");
     __shared__ uint32_t wl[kBlock / 64], wr[kBlock / 64];
     __shared__ uint32_t base_l, base_r;
     if (blockIdx.x >= nitems) return;
@@ -309,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_large_partition(LargeTask* tasks, co
     LargeTask& t = tasks[it.task];
     const uint32_t* in = t.buf ? ids_b : ids_a;
     uint32_t* out = t.buf ? ids_a : ids_b;
-    if (it.end > n || it.start > it.end || t.begin + t.count > n) {
+    if (it.end > n || it.start > it.end || it.end - it.start > kChunk || t.begin + t.count > n) {
         if (threadIdx.x == 0) atomicOr(err, kErrRange);
         return;
     }
@@ -320,96 +389,115 @@ __global__ __launch_bounds__(kBlock) void k_large_partition(LargeTask* tasks, co
         sc = bin_scale(clo, funkey(t.cen_hi[axis]));
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t r0 = it.start; r0 < it.end; r0 += kBlock) {
-        const uint32_t i = r0 + threadIdx.x;
-        const bool valid = i < it.end;
-        uint32_t p = 0;
+    // pass 1: the side of each of this thread's elements (bit j: element
+    // it.start + threadIdx.x + j * kBlock goes left) and the chunk's counts,
+    // then one pair of atomics for the whole chunk
+    uint32_t side = 0, nleft = 0, nright = 0;
+    for (uint32_t j = 0, i = it.start + threadIdx.x; i < it.end; ++j, i += kBlock) {
         bool left = false;
-        if (valid) {
-            p = in[i];
-            if (axis < 0) {
-                left = (i - t.begin) < t.nl;
-            } else if (p < n) {
+        if (axis < 0) {
+            left = (i - t.begin) < t.nl;
+        } else {
+            const uint32_t p = in[i];
+            if (p < n) {
                 const float4 l = plo[p], h = phi[p];
                 const float c = axis == 0 ? 0.5f * (l.x + h.x) : axis == 1 ? 0.5f * (l.y + h.y) : 0.5f * (l.z + h.z);
                 left = bin_of(c, clo, sc) < t.split;
             }
         }
-        const unsigned long long bl = __ballot(valid && left);
+        side |= uint32_t(left) << j;
+        nleft += left;
+        nright += !left;
+    }
+    const uint32_t cl = uint32_t(wave_reduce_add(int(nleft))), cr = uint32_t(wave_reduce_add(int(nright)));
+    if (lane == 0) { wl[wave] = cl; wr[wave] = cr; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sl = 0, sr = 0;
+        for (int w = 0; w < kBlock / 64; ++w) { sl += wl[w]; sr += wr[w]; }
+        base_l = sl ? atomicAdd(&t.lcur, sl) : 0u;
+        base_r = sr ? atomicAdd(&t.rcur, sr) : 0u;
+    }
+    __syncthreads();
+    // pass 2: stable placement inside the chunk's ranges, in element order
+    uint32_t off_l = base_l, off_r = base_r;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t r0 = it.start, j = 0; r0 < it.end; r0 += kBlock, ++j) {
+        const uint32_t i = r0 + threadIdx.x;
+        const bool valid = i < it.end;
+        const bool left = valid && ((side >> j) & 1u);
+        const unsigned long long bl = __ballot(left);
         const unsigned long long br = __ballot(valid && !left);
+        __syncthreads();   // the previous round's wl / wr are read
         if (lane == 0) { wl[wave] = __popcll(bl); wr[wave] = __popcll(br); }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t sl = 0, sr = 0;
-            for (int w = 0; w < kBlock / 64; ++w) { sl += wl[w]; sr += wr[w]; }
-            base_l = atomicAdd(&t.lcur, sl);
-            base_r = atomicAdd(&t.rcur, sr);
-        }
-        __syncthreads();
-        uint32_t off_l = base_l, off_r = base_r;
-        for (int w = 0; w < wave; ++w) { off_l += wl[w]; off_r += wr[w]; }
-        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t ol = off_l, orr = off_r;
+        for (int w = 0; w < wave; ++w) { ol += wl[w]; orr += wr[w]; }
         if (valid) {
-            const uint32_t dst = left ? t.begin + off_l + __popcll(bl & below)
-                                      : t.begin + t.nl + off_r + __popcll(br & below);
+            const uint32_t p = in[i];
+            const uint32_t dst = left ? t.begin + ol + __popcll(bl & below)
+                                      : t.begin + t.nl + orr + __popcll(br & below);
             if (dst < t.begin + t.count) out[dst] = p;
             else atomicOr(err, kErrRange);
         }
-        __syncthreads();
+        for (int w = 0; w < kBlock / 64; ++w) { off_l += wl[w]; off_r += wr[w]; }
     }
 }
 
-// SAH sweep over the 32 LDS bins of axis k, bins on lanes (each half-wave
-// holds the whole axis): prefix and suffix unions of the bin boxes and counts
-// by shuffles, then the first cheapest split plane s (bins < s go left) by a
-// reduction.  Returns split -1 when no plane has triangles on both sides.  The
-// costs are the serial sweep's: 1 + (A_left n_left + A_right n_right) / parent.
-__device__ __forceinline__ void sweep_axis(const Bins& lb, int k, float parent, float& out_cost, int& out_split,
-                                           uint32_t& out_nl) {
-    const int b = threadIdx.x & 31;
-    const uint32_t cnt = lb.cnt[k][b];
-    float pl[3], ph[3], sl[3], sh[3];
+// Per-wave LDS for the sweep: the unions of bins s..31, indexed by s.
+struct SweepScratch {
+    uint32_t cnt[kBins];
+    float lo[3][kBins], hi[3][kBins];
+};
+
+// SAH sweep over the 32 LDS bins of axis k, bin b on lane b (each half-wave
+// holds the whole axis): unions and counts of bins 0..b by DPP prefix scans,
+// of bins b..31 by the same scans over the bins in reverse order (handed to
+// their lanes through LDS), then the first cheapest split plane s (bins < s go
+// left) by a reduction.  Returns split -1 when no plane has triangles on both
+// sides.  The costs are the serial sweep's: 1 + (A_left n_left + A_right
+// n_right) / parent.  Called by all 64 lanes.
+__device__ __forceinline__ void sweep_axis(const Bins& lb, SweepScratch& sx, int k, float parent, float& out_cost,
+                                           int& out_split, uint32_t& out_nl) {
+    const int lane = threadIdx.x & 63;
+    const int b = lane & 31, rb = kBins - 1 - b;
+    const uint32_t cl = lb.cnt[k][b], cr = lb.cnt[k][rb];
+    float pl[3], ph[3], rl[3], rh[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        pl[a] = sl[a] = cnt ? funkey(lb.lo[k][b][a]) : INFINITY;
-        ph[a] = sh[a] = cnt ? funkey(lb.hi[k][b][a]) : -INFINITY;
+        pl[a] = half_scan_min(cl ? funkey(lb.lo[k][b][a]) : INFINITY);
+        ph[a] = half_scan_max(cl ? funkey(lb.hi[k][b][a]) : -INFINITY);
+        rl[a] = half_scan_min(cr ? funkey(lb.lo[k][rb][a]) : INFINITY);
+        rh[a] = half_scan_max(cr ? funkey(lb.hi[k][rb][a]) : -INFINITY);
     }
-    uint32_t pc = cnt, scnt = cnt;   // bins 0..b and b..31
+    const uint32_t pc = half_scan_add(cl), rcnt = half_scan_add(cr);
+    // lane b holds the union of bins rb..31: store it under rb
+    if (lane < 32) {
+        sx.cnt[rb] = rcnt;
 #pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const uint32_t up = __shfl_up(pc, o, 32), dn = __shfl_down(scnt, o, 32);
-        const bool take_up = b >= o, take_dn = b + o < 32;
-        if (take_up) pc += up;
-        if (take_dn) scnt += dn;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float ul = __shfl_up(pl[a], o, 32), uh = __shfl_up(ph[a], o, 32);
-            const float dl = __shfl_down(sl[a], o, 32), dh = __shfl_down(sh[a], o, 32);
-            if (take_up) { pl[a] = fminf(pl[a], ul); ph[a] = fmaxf(ph[a], uh); }
-            if (take_dn) { sl[a] = fminf(sl[a], dl); sh[a] = fmaxf(sh[a], dh); }
-        }
+        for (int a = 0; a < 3; ++a) { sx.lo[a][rb] = rl[a]; sx.hi[a][rb] = rh[a]; }
     }
-    // plane s = b: left = bins 0..b-1 (the prefix one lane down), right = bins b..31
-    const uint32_t lrun = __shfl_up(pc, 1, 32);
+    __syncthreads();
+    const uint32_t scnt = sx.cnt[b];
+    float sl[3], sh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { sl[a] = sx.lo[a][b]; sh[a] = sx.hi[a][b]; }
+    __syncthreads();
+    // plane s = b: left = bins 0..b-1, the prefix one lane down
+    const uint32_t lrun = uint32_t(dpp_i<kDppWaveShr1>(0, int(pc)));
     float ll[3], lh[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { ll[a] = __shfl_up(pl[a], 1, 32); lh[a] = __shfl_up(ph[a], 1, 32); }
+    for (int a = 0; a < 3; ++a) { ll[a] = dpp_f<kDppWaveShr1>(INFINITY, pl[a]); lh[a] = dpp_f<kDppWaveShr1>(-INFINITY, ph[a]); }
     float c = INFINITY;
     if (b >= 1 && lrun != 0 && scnt != 0)
         c = 1.f + (area_of(ll[0], ll[1], ll[2], lh[0], lh[1], lh[2]) * float(lrun) +
                    area_of(sl[0], sl[1], sl[2], sh[0], sh[1], sh[2]) * float(scnt)) / parent;
-    int split = c < INFINITY ? b : 64;
-    uint32_t nl = lrun;
-#pragma unroll
-    for (int o = 16; o >= 1; o >>= 1) {
-        const float c2 = __shfl_xor(c, o, 32);
-        const int s2 = __shfl_xor(split, o, 32);
-        const uint32_t n2 = __shfl_xor(nl, o, 32);
-        if (c2 < c || (c2 == c && s2 < split)) { c = c2; split = s2; nl = n2; }
-    }
-    out_cost = c;
+    // costs are >= 1 or +inf, so their bit patterns order like the values
+    const int best = half_min_i(__float_as_int(c));
+    const int split = half_min_i(c < INFINITY && __float_as_int(c) == best ? b : 64);
+    out_cost = __int_as_float(best);
     out_split = split < 32 ? split : -1;
-    out_nl = nl;
+    out_nl = split < 32 ? uint32_t(__builtin_amdgcn_readlane(int(lrun), split)) : 0u;
 }
 
 // ---------------------------------------------------------------- subtrees (one wave each)
@@ -421,13 +509,27 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                                                 const float4* __restrict__ plo, const float4* __restrict__ phi,
                                                 uint32_t* ids_a, uint32_t* ids_b, uint32_t* order, uint32_t n,
                                                 BinaryNode* nodes, uint32_t node_cap, uint32_t* node_counter,
-                                                uint32_t* err) {
+                                                uint32_t* unused, uint32_t* err) {
     __shared__ Bins lb;
+    __shared__ SweepScratch sx;
     __shared__ SubTask stack[kStack];
+    __shared__ struct { float4 lo[64], hi[64]; uint32_t id[64]; } cache;   // one slot per lane
     const int lane = threadIdx.x;
     for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
         const NodeTask root = tasks[ti];
         if (root.count == 0) continue;
+        // The child slots of the whole subtree, taken with one atomic: a subtree
+        // of c triangles has at most 2c - 1 nodes, its root allocated already.
+        // (One atomic per split on the one counter serialised the kernel.)
+        const uint32_t reserve = 2u * root.count - 2u;
+        uint32_t slot = 0;
+        if (lane == 0 && reserve) slot = atomicAdd(node_counter, reserve);
+        slot = __shfl(slot, 0, 64);
+        const uint32_t slot_end = slot + reserve;
+        if (slot_end > node_cap || slot_end < slot) {
+            if (lane == 0) atomicOr(err, kErrNodes);
+            continue;
+        }
         int sp = 0;
         SubTask cur{root.node, root.begin, root.count, root.buf};
         bool have = true;
@@ -439,6 +541,33 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
             }
             const uint32_t* in = cur.buf ? ids_b : ids_a;
             uint32_t* out = cur.buf ? ids_a : ids_b;
+            // A node of at most 64 triangles keeps triangle `lane` in the lane's own LDS
+            // slot, loaded once for the bounds, the binning and the partition (one
+            // dependent id -> box load chain from memory per node instead of three).
+            const bool cached = cur.count <= 64u;
+            if (cached && uint32_t(lane) < cur.count) {
+                const uint32_t id = in[cur.begin + lane];
+                cache.id[lane] = id;
+                if (id < n) {
+                    cache.lo[lane] = plo[id];
+                    cache.hi[lane] = phi[id];
+                }
+            }
+            // triangle at position i (i = cur.begin + lane for a cached node)
+            auto tri = [&](uint32_t i, uint32_t& p, float4& l, float4& h) -> bool {
+                if (cached) {
+                    p = cache.id[lane];
+                    if (p >= n) return false;
+                    l = cache.lo[lane];
+                    h = cache.hi[lane];
+                    return true;
+                }
+                p = in[i];
+                if (p >= n) return false;
+                l = plo[p];
+                h = phi[p];
+                return true;
+            };
             // bounds
             int v[12];
 #pragma unroll
@@ -446,9 +575,9 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                 v[k] = kIntMax; v[3 + k] = -int(kIntMax) - 1; v[6 + k] = kIntMax; v[9 + k] = -int(kIntMax) - 1;
             }
             for (uint32_t i = cur.begin + lane; i < cur.begin + cur.count; i += 64) {
-                const uint32_t p = in[i];
-                if (p >= n) continue;
-                const float4 l = plo[p], h = phi[p];
+                uint32_t p;
+                float4 l, h;
+                if (!tri(i, p, l, h)) continue;
                 const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -482,9 +611,9 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                 for (int i = lane; i < 3 * kBins * 3; i += 64) { llo[i] = kIntMax; lhi[i] = -int(kIntMax) - 1; }
                 __syncthreads();
                 for (uint32_t i = cur.begin + lane; i < cur.begin + cnt; i += 64) {
-                    const uint32_t p = in[i];
-                    if (p >= n) continue;
-                    const float4 l = plo[p], h = phi[p];
+                    uint32_t p;
+                    float4 l, h;
+                    if (!tri(i, p, l, h)) continue;
                     const float lo[3] = {l.x, l.y, l.z}, hi[3] = {h.x, h.y, h.z};
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
@@ -506,7 +635,7 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     float c;
                     int split;
                     uint32_t nl;
-                    sweep_axis(lb, k, parent, c, split, nl);
+                    sweep_axis(lb, sx, k, parent, c, split, nl);
                     if (split >= 0 && c < best_cost) { best_cost = c; best_axis = k; best_split = split; best_nl = nl; }
                 }
                 __syncthreads();
@@ -522,7 +651,7 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     nd.first = cur.begin;
                     nd.count = cnt;
                 }
-                for (uint32_t i = cur.begin + lane; i < cur.begin + cnt; i += 64) order[i] = in[i];
+                for (uint32_t i = cur.begin + lane; i < cur.begin + cnt; i += 64) order[i] = cached ? cache.id[lane] : in[i];
             } else {
                 const uint32_t nl = best_axis >= 0 ? best_nl : cnt / 2;
                 // stable partition into the other buffer
@@ -537,11 +666,12 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     uint32_t p = 0;
                     bool left = false;
                     if (valid) {
-                        p = in[i];
+                        float4 l, h;
+                        const bool ok = tri(i, p, l, h);
+                        if (!ok) p = cached ? cache.id[lane] : in[i];   // an out-of-range id is carried along unchanged
                         if (best_axis < 0) {
                             left = (i - cur.begin) < nl;
-                        } else if (p < n) {
-                            const float4 l = plo[p], h = phi[p];
+                        } else if (ok) {
                             const float c = best_axis == 0 ? 0.5f * (l.x + h.x)
                                           : best_axis == 1 ? 0.5f * (l.y + h.y) : 0.5f * (l.z + h.z);
                             left = bin_of(c, pclo, psc) < best_split;
@@ -558,10 +688,9 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                     lc += __popcll(bl);
                     rc += __popcll(br);
                 }
-                uint32_t l = 0;
-                if (lane == 0) l = atomicAdd(node_counter, 2u);
-                l = __shfl(l, 0, 64);
-                if (l + 1 >= node_cap) {
+                const uint32_t l = slot;
+                slot += 2u;
+                if (slot > slot_end) {
                     if (lane == 0) atomicOr(err, kErrNodes);
                     break;
                 }
@@ -600,6 +729,8 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                 __syncthreads();
             }
         }
+        // slots the subtree did not need (leaves of several triangles) stay unused
+        if (lane == 0 && slot < slot_end) atomicAdd(unused, slot_end - slot);
     }
 }
 
@@ -632,7 +763,8 @@ namespace {
 struct DeviceTree {
     DevBuf<BinaryNode> nodes;
     DevBuf<uint32_t> prims;   // tree triangle position -> input triangle
-    uint32_t used = 0;        // nodes in use, root 0
+    uint32_t used = 0;        // node slots taken, root 0
+    uint32_t unused = 0;      // of which reserved by a subtree but not needed (never reached)
 };
 
 // Builds the tree of triangles d0/d1/d2 (device arrays, 3 floats per
@@ -642,11 +774,11 @@ bool build_tree(const float* d0, const float* d1, const float* d2, uint32_t n, h
     if (n < 8 || n > (1u << 29)) { *why = "size"; return false; }
     const uint32_t cap = 2 * n;
     DevBuf<float4> plo, phi;
-    DevBuf<uint32_t> ids_a, ids_b, counter, err;
+    DevBuf<uint32_t> ids_a, ids_b, counter, err, unused;
     DevBuf<BinaryNode>& nodes = tree.nodes;
     DevBuf<uint32_t>& order = tree.prims;
     if (!plo.alloc(n) || !phi.alloc(n) || !ids_a.alloc(n) || !ids_b.alloc(n) || !order.alloc(n) ||
-        !counter.alloc(1) || !err.alloc(1) || !nodes.alloc(cap)) {
+        !counter.alloc(1) || !err.alloc(1) || !unused.alloc(1) || !nodes.alloc(cap)) {
         *why = "allocation";
         return false;
     }
@@ -659,6 +791,7 @@ bool build_tree(const float* d0, const float* d1, const float* d2, uint32_t n, h
                        n, plo.p, phi.p, ids_a.p);
     hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, counter.p, size_t(1), 1u);
     hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, err.p, size_t(1), 0u);
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, unused.p, size_t(1), 0u);
     BVH_CHECK(hipGetLastError());
     // large-node levels
     std::vector<NodeTask> medium;
@@ -725,20 +858,22 @@ bool build_tree(const float* d0, const float* d1, const float* d2, uint32_t n, h
         BVH_CHECK(hipMemcpyAsync(dmed.p, medium.data(), medium.size() * sizeof(NodeTask), hipMemcpyHostToDevice, s));
         const uint32_t grid = uint32_t(std::min<size_t>(medium.size(), 65536));
         hipLaunchKernelGGL(k_subtree, dim3(grid), dim3(64), 0, s, dmed.p, uint32_t(medium.size()), plo.p, phi.p,
-                           ids_a.p, ids_b.p, order.p, n, nodes.p, cap, counter.p, err.p);
+                           ids_a.p, ids_b.p, order.p, n, nodes.p, cap, counter.p, unused.p, err.p);
         BVH_CHECK(hipGetLastError());
         BVH_CHECK(hipStreamSynchronize(s));
     }
     BVH_CHECK(hipEventRecord(e1, s));
-    uint32_t used = 0, herr = 0;
+    uint32_t used = 0, herr = 0, spare = 0;
     BVH_CHECK(hipMemcpyAsync(&used, counter.p, 4, hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipMemcpyAsync(&spare, unused.p, 4, hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     if (device_ms) *device_ms = ms;
-    if (herr || used == 0 || used > cap) { *why = "subtree pass"; return false; }
+    if (herr || used == 0 || used > cap || spare >= used) { *why = "subtree pass"; return false; }
     tree.used = used;
+    tree.unused = spare;
     return true;
 }
 
@@ -854,7 +989,7 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
 
     DeviceLayout lay;
     struct LayGuard { DeviceLayout& l; ~LayGuard() { l.release(); } } lg{lay};
-    if (!layout_tree_gpu(tree.nodes.p, tree.used, tree.prims.p, nt, s, lay, why)) return false;
+    if (!layout_tree_gpu(tree.nodes.p, tree.used, tree.unused, tree.prims.p, nt, s, lay, why)) return false;
     tree.nodes.release();
     tree.prims.release();
     DevBuf<float4> tris, mats;
@@ -902,7 +1037,8 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
 }
 
 bool build_tree_gpu_host(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
-                         std::vector<BinaryNode>& tree_out, std::vector<uint32_t>& prims_out, const char** why) {
+                         std::vector<BinaryNode>& tree_out, std::vector<uint32_t>& prims_out, uint32_t* unused,
+                         const char** why) {
     *why = "";
     DevBuf<float> dp[3];
     const float* src[3] = {p0, p1, p2};
@@ -917,6 +1053,7 @@ bool build_tree_gpu_host(const float* p0, const float* p1, const float* p2, uint
     }
     tree_out.resize(tree.used);
     prims_out.resize(ntri);
+    if (unused) *unused = tree.unused;
     BVH_CHECK(hipMemcpyAsync(tree_out.data(), tree.nodes.p, tree.used * sizeof(BinaryNode), hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipMemcpyAsync(prims_out.data(), tree.prims.p, 4 * size_t(ntri), hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipStreamSynchronize(s));

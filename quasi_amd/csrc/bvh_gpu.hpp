@@ -34,11 +34,12 @@ struct DeviceLayout {
 
 // The layouts that convert_binary_tree writes for the same tree, byte for
 // byte (the SAH cost agrees to rounding), computed on the device.  tree:
-// `used` nodes, root 0 internal, every child at a larger index than its
-// parent; prims: n entries.  Every index is checked; false (with a reason)
-// when the tree is malformed or an allocation fails.
-bool layout_tree_gpu(const BinaryNode* tree, uint32_t used, const uint32_t* prims, uint32_t n, hipStream_t s,
-                     DeviceLayout& out, const char** why);
+// `used` node slots, root 0 internal, every child at a larger index than its
+// parent, exactly `unused` slots unreachable from the root (space a builder
+// reserved and did not need); prims: n entries.  Every index is checked;
+// false (with a reason) when the tree is malformed or an allocation fails.
+bool layout_tree_gpu(const BinaryNode* tree, uint32_t used, uint32_t unused, const uint32_t* prims, uint32_t n,
+                     hipStream_t s, DeviceLayout& out, const char** why);
 
 // One scene built on the device: the arrays of common.hpp (triangles,
 // materials, pair nodes, 4-wide nodes), hipMalloc'd and owned by the caller.
@@ -62,9 +63,11 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
                      hipStream_t s, DeviceScene& out, BvhGpuTimes* times, const char** why);
 
 // The binary tree that build_scene_gpu builds for triangles p0/p1/p2 (host
-// arrays, 3 floats per triangle), downloaded: tree nodes and prims, so that
-// tests can lay it out both ways.  False on failure.
+// arrays, 3 floats per triangle), downloaded: tree node slots (with `unused`
+// unreachable ones) and prims, so that tests can lay it out both ways.  False
+// on failure.
 bool build_tree_gpu_host(const float* p0, const float* p1, const float* p2, uint32_t ntri, hipStream_t s,
-                         std::vector<BinaryNode>& tree, std::vector<uint32_t>& prims, const char** why);
+                         std::vector<BinaryNode>& tree, std::vector<uint32_t>& prims, uint32_t* unused,
+                         const char** why);
 
 }  // namespace quasi_hip

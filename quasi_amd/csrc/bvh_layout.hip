@@ -348,8 +348,8 @@ void DeviceLayout::release() {
     order = nullptr;
 }
 
-bool layout_tree_gpu(const BinaryNode* tree, uint32_t used, const uint32_t* prims, uint32_t n, hipStream_t s,
-                     DeviceLayout& out, const char** why) {
+bool layout_tree_gpu(const BinaryNode* tree, uint32_t used, uint32_t unused, const uint32_t* prims, uint32_t n,
+                     hipStream_t s, DeviceLayout& out, const char** why) {
     out.release();
     *why = "";
     if (used < 3 || n == 0 || n > (1u << 29)) { *why = "layout size"; return false; }
@@ -388,7 +388,7 @@ bool layout_tree_gpu(const BinaryNode* tree, uint32_t used, const uint32_t* prim
     while (levels < kMaxLevels && hctl[kCtlCount + levels]) internal += hctl[kCtlCount + levels++];
     if (levels == kMaxLevels && hctl[kCtlCount + kMaxLevels]) { *why = "layout: tree deeper than 64 levels"; return false; }
     const uint32_t leaves = hctl[kCtlLeaves];
-    if (internal + leaves != used) { *why = "layout: unreachable nodes"; return false; }
+    if (internal + leaves + unused != used) { *why = "layout: unreachable nodes"; return false; }
     std::vector<uint32_t> begin(levels);
     for (int L = 0; L < levels; ++L) begin[L] = hctl[kCtlBegin + L];
 

@@ -23,8 +23,10 @@ void say(char* why, unsigned len, const char* msg) {
 }
 
 // info: 0 num_nodes, 1 num_leaves, 2 depth, 3 num_nodes4, 4 depth4 (GPU), then
-// the same five from the host converter at 5..9.
-int layout_both(const BinaryNode* tree, unsigned used, const unsigned* prims, unsigned n, float* gpu_nodes,
+// the same five from the host converter at 5..9; probe_layout_built adds the
+// tree's node slots at 10 and its unused (reserved, unreachable) slots at 11.
+int layout_both(const BinaryNode* tree, unsigned used, unsigned unused, const unsigned* prims, unsigned n,
+                float* gpu_nodes,
                 float* host_nodes, unsigned nodes_cap, float* gpu_nodes4, float* host_nodes4, unsigned nodes4_cap,
                 unsigned* gpu_order, unsigned* host_order, unsigned* info, double* sah, char* why, unsigned why_len) {
     quasi_hip::BvhBuildResult h = quasi_hip::convert_binary_tree(tree, used, prims, n);
@@ -53,7 +55,7 @@ int layout_both(const BinaryNode* tree, unsigned used, const unsigned* prims, un
         hipMemcpy(dprims, prims, n * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
         say(why, why_len, "upload");
         rc = -3;
-    } else if (!quasi_hip::layout_tree_gpu(dtree, used, dprims, n, nullptr, d, &reason)) {
+    } else if (!quasi_hip::layout_tree_gpu(dtree, used, unused, dprims, n, nullptr, d, &reason)) {
         say(why, why_len, reason);
         rc = 1;   // rejected by the layout's checks
     } else {
@@ -86,7 +88,7 @@ PROBE_API int probe_layout_tree(const void* tree, unsigned used, const unsigned*
                                  float* gpu_nodes, float* host_nodes, unsigned nodes_cap, float* gpu_nodes4,
                                  float* host_nodes4, unsigned nodes4_cap, unsigned* gpu_order, unsigned* host_order,
                                  unsigned* info, double* sah, char* why, unsigned why_len) {
-    return layout_both(static_cast<const BinaryNode*>(tree), used, prims, n, gpu_nodes, host_nodes, nodes_cap,
+    return layout_both(static_cast<const BinaryNode*>(tree), used, 0u, prims, n, gpu_nodes, host_nodes, nodes_cap,
                        gpu_nodes4, host_nodes4, nodes4_cap, gpu_order, host_order, info, sah, why, why_len);
 }
 
@@ -105,7 +107,7 @@ PROBE_API int probe_layout_check(const void* tree, unsigned used, const unsigned
     if (hipMemcpy(dtree, tree, used * sizeof(BinaryNode), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dprims, prims, n * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess)
         rc = -3;
-    else if (!quasi_hip::layout_tree_gpu(dtree, used, dprims, n, nullptr, d, &reason)) {
+    else if (!quasi_hip::layout_tree_gpu(dtree, used, 0u, dprims, n, nullptr, d, &reason)) {
         say(why, why_len, reason);
         rc = 1;
     }
@@ -123,11 +125,14 @@ PROBE_API int probe_layout_built(const float* p0, const float* p1, const float* 
     std::vector<BinaryNode> tree;
     std::vector<unsigned> prims;
     const char* reason = "";
-    if (!quasi_hip::build_tree_gpu_host(p0, p1, p2, ntri, nullptr, tree, prims, &reason)) {
+    unsigned unused = 0;
+    if (!quasi_hip::build_tree_gpu_host(p0, p1, p2, ntri, nullptr, tree, prims, &unused, &reason)) {
         say(why, why_len, reason);
         return -5;
     }
     info[10] = unsigned(tree.size());
-    return layout_both(tree.data(), unsigned(tree.size()), prims.data(), ntri, gpu_nodes, host_nodes, nodes_cap,
-                       gpu_nodes4, host_nodes4, nodes4_cap, gpu_order, host_order, info, sah, why, why_len);
+    info[11] = unused;
+    return layout_both(tree.data(), unsigned(tree.size()), unused, prims.data(), ntri, gpu_nodes, host_nodes,
+                       nodes_cap, gpu_nodes4, host_nodes4, nodes4_cap, gpu_order, host_order, info, sah, why,
+                       why_len);
 }

@@ -65,6 +65,9 @@ def test_gpu_build_renders_like_host_build(name, make):
           f"gpu build {st_gpu.bvh_device_ms:.2f} ms (set_scene {st_gpu.bvh_build_ms:.0f} ms vs "
           f"{st_host.bvh_build_ms:.0f} ms)", flush=True)
     assert st_gpu.bvh_sah_cost <= 1.03 * st_host.bvh_sah_cost
+    # both builders make the same binned-SAH decisions on these scenes: the same tree
+    assert st_gpu.bvh_nodes == st_host.bvh_nodes and st_gpu.bvh_depth == st_host.bvh_depth
+    assert abs(st_gpu.bvh_sah_cost - st_host.bvh_sah_cost) <= 1e-9 * st_host.bvh_sah_cost
 
 
 def test_auto_builder_uses_gpu_for_large_scenes():

@@ -135,7 +135,7 @@ def test_trees_of_the_gpu_builder(lib, name, make):
     b, args = buffers(2 * n, n)
     rc = lib.probe_layout_built(*[ptr(x, C.c_float) for x in p], n, *args)
     assert rc == 0, (rc, b["why"].value)
-    compare(b, f"{name}: {n} triangles, {int(b['info'][10])} tree nodes")
+    compare(b, f"{name}: {n} triangles, {int(b['info'][10])} tree node slots ({int(b['info'][11])} unused)")
 
 
 def test_malformed_trees_are_rejected(lib):
