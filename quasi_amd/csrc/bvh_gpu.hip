@@ -238,27 +238,38 @@ __global__ __launch_bounds__(kBlock) void k_large_bounds(LargeTask* tasks, const
     }
 }
 
+// Binning of one chunk of a large node: every lane adds its triangles to one
+// of kBinCopies copies of the bins in LDS (lane & 15; the copies are padded so
+// a word's copies fall in different banks), then the block folds the copies
+// and merges them into the node's bins.  Neighbouring triangles of the order
+// are mostly neighbours in space; with one copy, the 64 lanes of a wave adding
+// to the same bin serialised the LDS atomics.
+constexpr int kBinCopies = 16;
+constexpr int kBinWords = 3 * kBins * 7;            // cnt, then lo, then hi (the Bins layout)
+constexpr int kBinStride = kBinWords + 1;
 __global__ __launch_bounds__(kBlock) void k_large_bin(const LargeTask* tasks, const WorkItem* items, uint32_t nitems,
                                                       Bins* bins, const float4* __restrict__ plo,
                                                       const float4* __restrict__ phi,
                                                       const uint32_t* __restrict__ ids_a,
                                                       const uint32_t* __restrict__ ids_b, uint32_t n) {
-    __shared__ Bins lb;
+    static_assert(sizeof(Bins) == kBinWords * 4, "Bins is cnt[3][32], lo[3][32][3], hi[3][32][3]");
+    constexpr int kCnt = 3 * kBins, kBox = 3 * kBins * 3;
+    __shared__ int lb[kBinCopies * kBinStride];
     if (blockIdx.x >= nitems) return;
     const WorkItem it = items[blockIdx.x];
     const LargeTask& t = tasks[it.task];
     const uint32_t* ids = t.buf ? ids_b : ids_a;
-    uint32_t* lw = reinterpret_cast<uint32_t*>(&lb);
-    for (int i = threadIdx.x; i < 3 * kBins; i += kBlock) lw[i] = 0u;
-    int* llo = &lb.lo[0][0][0];
-    int* lhi = &lb.hi[0][0][0];
-    for (int i = threadIdx.x; i < 3 * kBins * 3; i += kBlock) { llo[i] = kIntMax; lhi[i] = -int(kIntMax) - 1; }
+    for (int i = threadIdx.x; i < kBinCopies * kBinStride; i += kBlock) {
+        const int w = i % kBinStride;
+        lb[i] = w < kCnt ? 0 : w < kCnt + kBox ? int(kIntMax) : -int(kIntMax) - 1;
+    }
     __syncthreads();
     float clo[3], sc[3];
     for (int k = 0; k < 3; ++k) {
         clo[k] = funkey(t.cen_lo[k]);
         sc[k] = bin_scale(clo[k], funkey(t.cen_hi[k]));
     }
+    int* mine = lb + (threadIdx.x & (kBinCopies - 1)) * kBinStride;
     if (it.end <= n && it.start <= it.end) {
         for (uint32_t i = it.start + threadIdx.x; i < it.end; i += kBlock) {
             const uint32_t p = ids[i];
@@ -268,25 +279,28 @@ __global__ __launch_bounds__(kBlock) void k_large_bin(const LargeTask* tasks, co
             for (int k = 0; k < 3; ++k) {
                 if (sc[k] == 0.f) continue;
                 const int b = bin_of(0.5f * (lo[k] + hi[k]), clo[k], sc[k]);
-                atomicAdd(&lb.cnt[k][b], 1u);
+                atomicAdd(&mine[k * kBins + b], 1);
                 for (int a = 0; a < 3; ++a) {
-                    atomicMin(&lb.lo[k][b][a], fkey(lo[a]));
-                    atomicMax(&lb.hi[k][b][a], fkey(hi[a]));
+                    atomicMin(&mine[kCnt + (k * kBins + b) * 3 + a], fkey(lo[a]));
+                    atomicMax(&mine[kCnt + kBox + (k * kBins + b) * 3 + a], fkey(hi[a]));
                 }
             }
         }
     }
     __syncthreads();
     Bins& g = bins[it.task];
-    for (int i = threadIdx.x; i < 3 * kBins; i += kBlock) {
-        const int k = i / kBins, b = i % kBins;
-        const uint32_t c = lb.cnt[k][b];
-        if (c == 0) continue;
-        atomicAdd(&g.cnt[k][b], c);
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&g.lo[k][b][a], lb.lo[k][b][a]);
-            atomicMax(&g.hi[k][b][a], lb.hi[k][b][a]);
+    int* gw = reinterpret_cast<int*>(&g);
+    const bool sole = t.count <= kChunk;   // the node's only block: store instead of merging
+    for (int w = threadIdx.x; w < kBinWords; w += kBlock) {
+        int v = lb[w];
+        for (int c = 1; c < kBinCopies; ++c) {
+            const int x = lb[c * kBinStride + w];
+            v = w < kCnt ? v + x : w < kCnt + kBox ? min(v, x) : max(v, x);
         }
+        if (sole) gw[w] = v;
+        else if (w < kCnt) { if (v) atomicAdd(&gw[w], v); }
+        else if (w < kCnt + kBox) { if (v != int(kIntMax)) atomicMin(&gw[w], v); }
+        else if (v != -int(kIntMax) - 1) atomicMax(&gw[w], v);
     }
 }
 
