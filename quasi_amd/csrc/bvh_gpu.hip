@@ -617,7 +617,31 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
             int best_axis = -1, best_split = -1;
             float best_cost = INFINITY;
             uint32_t best_nl = 0;
-            if (cnt > 1) {
+            if (cnt == 2u && cached && cache.id[0] < n && cache.id[1] < n) {
+                // Two triangles: every plane of an axis whose two bins differ splits
+                // them the same way, so the binned sweep's answer is the first such
+                // axis, at plane min(b0, b1) + 1, with the cost 1 + (A0 + A1) / A of
+                // the two triangles' own boxes (the bins hold exactly those boxes).
+                const float4 l0 = cache.lo[0], h0 = cache.hi[0], l1 = cache.lo[1], h1 = cache.hi[1];
+                const float a0 = area_of(l0.x, l0.y, l0.z, h0.x, h0.y, h0.z);
+                const float a1 = area_of(l1.x, l1.y, l1.z, h1.x, h1.y, h1.z);
+                const float parent = fmaxf(area_of(blo[0], blo[1], blo[2], bhi[0], bhi[1], bhi[2]), 1e-30f);
+                const float lo0[3] = {l0.x, l0.y, l0.z}, hi0[3] = {h0.x, h0.y, h0.z};
+                const float lo1[3] = {l1.x, l1.y, l1.z}, hi1[3] = {h1.x, h1.y, h1.z};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (sc[k] == 0.f || best_axis >= 0) continue;
+                    const int b0 = bin_of(0.5f * (lo0[k] + hi0[k]), clo[k], sc[k]);
+                    const int b1 = bin_of(0.5f * (lo1[k] + hi1[k]), clo[k], sc[k]);
+                    if (b0 == b1) continue;
+                    // the sweep's expression, left side first (the sum does not depend on the order)
+                    const float al = b0 < b1 ? a0 : a1, ar = b0 < b1 ? a1 : a0;
+                    best_cost = 1.f + (al * float(1u) + ar * float(1u)) / parent;
+                    best_axis = k;
+                    best_split = min(b0, b1) + 1;
+                    best_nl = 1u;
+                }
+            } else if (cnt > 1) {
                 uint32_t* lw = &lb.cnt[0][0];
                 for (int i = lane; i < 3 * kBins; i += 64) lw[i] = 0u;
                 int* llo = &lb.lo[0][0][0];
