@@ -107,7 +107,7 @@ typedef struct Q_render_params {
 /* Q_render_params.queue_chunk: the plain (non-persistent) wavefront traversal */
 #define Q_QUEUE_FLAT 0xFFFFFFFFu
 
-#define Q_BVH_BUILD_AUTO 0u   /* on the GPU from 2^20 triangles, else on the host */
+#define Q_BVH_BUILD_AUTO 0u   /* on the GPU from 2^14 triangles, else on the host */
 #define Q_BVH_BUILD_HOST 1u   /* host threads */
 #define Q_BVH_BUILD_GPU  2u   /* GPU kernels (falls back to the host when the GPU build fails) */
 
