@@ -218,44 +218,67 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     free_batch(st);
     auto t0 = std::chrono::steady_clock::now();
 
-    // BVH builds: one scene per host thread
+    // BVH builds: scenes the builder setting sends to the GPU are built on the
+    // device one after another by this thread, the others meanwhile on host
+    // threads, one scene per thread
     std::vector<SceneHost> hosts(n);
+    struct DeviceGuard {   // device arrays of GPU-built scenes are freed once copied (or on failure)
+        std::vector<SceneHost>& h;
+        ~DeviceGuard() { for (SceneHost& x : h) x.device.release(); }
+    } device_guard{hosts};
+    const uint32_t builder = st->params.bvh_builder;
+    auto on_gpu = [&](uint32_t i) {
+        const uint32_t nt = desc->scenes[i].mesh.num_triangles;
+        return builder == Q_BVH_BUILD_GPU || (builder == Q_BVH_BUILD_AUTO && nt >= kGpuBuildMinTriangles);
+    };
+    std::vector<uint32_t> host_list, gpu_list;
+    for (uint32_t i = 0; i < n; ++i) (on_gpu(i) ? gpu_list : host_list).push_back(i);
     std::vector<int32_t> rcs(n, Q_EXT_OK);
     std::atomic<uint32_t> next{0};
-    const unsigned nth = std::max(1u, std::min({16u, std::thread::hardware_concurrency(), n}));
     auto worker = [&] {
         for (;;) {
-            const uint32_t i = next.fetch_add(1);
-            if (i >= n) break;
+            const uint32_t k = next.fetch_add(1);
+            if (k >= host_list.size()) break;
+            const uint32_t i = host_list[k];
             const Q_scene_desc& m = desc->scenes[i].mesh;
             rcs[i] = prepare_scene(st, m.vertices, m.num_vertices, m.indices, m.num_triangles, m.albedo, m.emission,
                                    1, hosts[i]);
         }
     };
+    const unsigned nth = std::max(1u, std::min({16u, std::thread::hardware_concurrency(), uint32_t(host_list.size())}));
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nth; ++t) pool.emplace_back(worker);
-    worker();
+    for (unsigned t = 0; t < nth && !host_list.empty(); ++t) pool.emplace_back(worker);
+    for (uint32_t i : gpu_list) {
+        const Q_scene_desc& m = desc->scenes[i].mesh;
+        rcs[i] = prepare_scene(st, m.vertices, m.num_vertices, m.indices, m.num_triangles, m.albedo, m.emission, 0,
+                               hosts[i], builder);
+    }
     for (auto& t : pool) t.join();
     for (uint32_t i = 0; i < n; ++i)
         if (rcs[i] != Q_EXT_OK) return rcs[i];
+    // floats of each array, for host- and GPU-built scenes alike
+    auto tri_f = [](const SceneHost& h) { return size_t(12) * h.num_tris; };
+    auto mat_f = [](const SceneHost& h) { return size_t(8) * h.num_tris; };
+    auto node_f = [](const SceneHost& h) { return size_t(16) * h.num_nodes; };
+    auto node4_f = [](const SceneHost& h) { return size_t(32) * h.num_nodes4; };
 
     // concatenated arrays
-    size_t tri_f = 0, mat_f = 0, node_f = 0, node4_f = 0, light_n = 0;
+    size_t tri_total = 0, mat_total = 0, node_total = 0, node4_total = 0, light_n = 0;
     uint32_t gentries = 0;
     for (const SceneHost& h : hosts) {
-        tri_f += h.tri4.size();
-        mat_f += h.mat4.size();
-        node_f += h.nodes.size();
-        node4_f += h.nodes4.size();
+        tri_total += tri_f(h);
+        mat_total += mat_f(h);
+        node_total += node_f(h);
+        node4_total += node4_f(h);
         light_n += h.lights.size();
         gentries = std::max(gentries, stack_overflow_entries(h.depth, h.depth4));
     }
     BatchState& b = st->batch;
     const size_t npix = size_t(total_pix);
-    if (!ensure_buffer(st, b.tris, std::max<size_t>(16, tri_f * 4), "batch triangles") ||
-        !ensure_buffer(st, b.mats, std::max<size_t>(16, mat_f * 4), "batch materials") ||
-        !ensure_buffer(st, b.nodes, std::max<size_t>(64, node_f * 4), "batch BVH nodes") ||
-        !ensure_buffer(st, b.nodes4, std::max<size_t>(128, node4_f * 4), "batch BVH4 nodes") ||
+    if (!ensure_buffer(st, b.tris, std::max<size_t>(16, tri_total * 4), "batch triangles") ||
+        !ensure_buffer(st, b.mats, std::max<size_t>(16, mat_total * 4), "batch materials") ||
+        !ensure_buffer(st, b.nodes, std::max<size_t>(64, node_total * 4), "batch BVH nodes") ||
+        !ensure_buffer(st, b.nodes4, std::max<size_t>(128, node4_total * 4), "batch BVH4 nodes") ||
         !ensure_buffer(st, b.lights, std::max<size_t>(16, light_n * sizeof(LightEntry)), "batch lights") ||
         !ensure_buffer(st, b.table, n * sizeof(SceneDesc), "scene table") ||
         !ensure_buffer(st, b.ids, 4 * npix, "batch ids") ||
@@ -277,10 +300,17 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     for (uint32_t i = 0; i < n; ++i) {
         const SceneHost& h = hosts[i];
         const Q_batch_scene& sc = desc->scenes[i];
-        if ((!h.tri4.empty() && !hip_ok(st, hipMemcpy(tb + 4 * to, h.tri4.data(), 4 * h.tri4.size(), hipMemcpyHostToDevice), "batch upload")) ||
-            (!h.mat4.empty() && !hip_ok(st, hipMemcpy(mb + 4 * mo, h.mat4.data(), 4 * h.mat4.size(), hipMemcpyHostToDevice), "batch upload")) ||
-            (!h.nodes.empty() && !hip_ok(st, hipMemcpy(nb + 4 * no, h.nodes.data(), 4 * h.nodes.size(), hipMemcpyHostToDevice), "batch upload")) ||
-            (!h.nodes4.empty() && !hip_ok(st, hipMemcpy(n4b + 4 * n4o, h.nodes4.data(), 4 * h.nodes4.size(), hipMemcpyHostToDevice), "batch upload")) ||
+        // a GPU-built scene's arrays are already on the device; the others come from host memory
+        const bool dev = h.builder == Q_BVH_BUILD_GPU;
+        auto put = [&](char* dst, const void* host_src, const void* dev_src, size_t bytes) {
+            if (bytes == 0) return true;
+            return hip_ok(st, hipMemcpy(dst, dev ? dev_src : host_src, bytes,
+                                        dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice), "batch upload");
+        };
+        if (!put(tb + 4 * to, h.tri4.data(), h.device.tris, 4 * tri_f(h)) ||
+            !put(mb + 4 * mo, h.mat4.data(), h.device.mats, 4 * mat_f(h)) ||
+            !put(nb + 4 * no, h.nodes.data(), h.device.nodes, 4 * node_f(h)) ||
+            !put(n4b + 4 * n4o, h.nodes4.data(), h.device.nodes4, 4 * node4_f(h)) ||
             (!h.lights.empty() && !hip_ok(st, hipMemcpy(lb + lo, h.lights.data(), sizeof(LightEntry) * h.lights.size(), hipMemcpyHostToDevice), "batch upload"))) {
             free_batch(st);
             return Q_EXT_ERR_DEVICE;
@@ -300,19 +330,19 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
         d.pix_begin = pix;
         d.path_begin = pix;
         d.seed = sc.seed;
-        d.num_tris = uint32_t(h.tri4.size() / 3);
-        to += h.tri4.size();
-        mo += h.mat4.size();
-        no += h.nodes.size();
-        n4o += h.nodes4.size();
+        d.num_tris = h.num_tris;
+        to += tri_f(h);
+        mo += mat_f(h);
+        no += node_f(h);
+        n4o += node4_f(h);
         lo += h.lights.size();
         pix += sc.width * sc.height;
     }
     b.num_scenes = n;
     b.total_pixels = uint32_t(total_pix);
     b.total_tris = uint32_t(total_tris);
-    b.total_nodes = uint32_t(node_f / 16);     // 16 floats per BVH2 pair node
-    b.total_nodes4 = uint32_t(node4_f / 32);   // 32 floats per BVH4 node
+    b.total_nodes = uint32_t(node_total / 16);     // 16 floats per BVH2 pair node
+    b.total_nodes4 = uint32_t(node4_total / 32);   // 32 floats per BVH4 node
     b.gstack_entries = std::max(gentries, st->gstack_entries);
     st->gstack_entries = b.gstack_entries;
     b.active = true;

@@ -48,12 +48,16 @@ def plain_render(r, mesh, w, h, seed, spp):
     return r.readback_aov(), r.primary_ids()
 
 
-def test_batch_equals_plain_per_scene():
+@pytest.mark.parametrize("builder", [abi.Q_BVH_BUILD_AUTO, abi.Q_BVH_BUILD_GPU], ids=["auto", "gpu"])
+def test_batch_equals_plain_per_scene(builder):
+    """With the GPU builder the batch's scenes are built on the device and
+    their arrays concatenated there; with the automatic one these small scenes
+    are built on host threads."""
     batch = small_batch()
     spp = 4
-    say(f"[batch==plain] {len(batch)} scenes")
+    say(f"[batch==plain] {len(batch)} scenes, builder {builder}")
     with HipRenderer(16, 16) as r:
-        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        r.set_params(spp_per_render=1, seed=0, bvh_builder=builder, **PARAMS)
         r.set_batch(batch)
         r.render_batch(spp)
         got = [(r.readback_batch(i), r.batch_ids(i)) for i in range(len(batch))]
