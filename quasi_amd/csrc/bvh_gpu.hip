@@ -12,12 +12,17 @@
 // Two phases:
 //  - large nodes (more than kLargeNode triangles) are split one tree level at
 //    a time with many blocks per node: bounds (block reductions + one atomic
-//    per block), binning (LDS bins merged with global atomics), split choice
-//    (one thread per node), partition (block-level compaction, one atomic per
-//    block and side into the other id buffer);
-//  - each remaining subtree is built by one wave: bounds by wave reductions,
-//    bins in LDS, stable partition with ballots, the smaller child first so
-//    the LDS task stack stays within log2 of the subtree size.
+//    per block), binning (sixteen bank-padded LDS copies of the bins per
+//    block, folded and merged with global atomics, or stored when the block
+//    is the node's only one), split choice (one thread per node), partition
+//    (count, then write: one atomic per block and side into the other id
+//    buffer);
+//  - each remaining subtree is built by one wave, which reserves the
+//    subtree's node slots with one atomic: bounds by DPP wave reductions,
+//    bins in LDS, the SAH sweep with one bin per lane, stable partition with
+//    ballots, the smaller child first so the LDS task stack stays within
+//    log2 of the subtree size.  Slots a subtree does not need stay unused
+//    and are counted for the layout's reachability check.
 // Every index a kernel derives is checked against its buffer; a violation
 // sets an error flag and the host falls back to the host build.
 #include <hip/hip_runtime.h>
