@@ -964,6 +964,17 @@ __global__ void k_scene_arrays(const uint32_t* __restrict__ order, uint32_t n, c
     if (bad) atomicOr(err, 2u);
 }
 
+// Emission of the listed triangles (ids checked on the host) into a zeroed array.
+__global__ void k_scatter_emission(const uint32_t* __restrict__ ids, const float* __restrict__ vals, uint32_t m,
+                                   float* __restrict__ emission) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) {
+        const size_t o = 3 * size_t(ids[j]);
+        emission[o] = vals[3 * size_t(j)];
+        emission[o + 1] = vals[3 * size_t(j) + 1];
+        emission[o + 2] = vals[3 * size_t(j) + 2];
+    }
+}
+
 __global__ void k_gather_positions(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ query, uint32_t nq,
                                    uint32_t* __restrict__ pos) {
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) pos[q] = inv[query[q]];
@@ -983,8 +994,9 @@ void DeviceScene::release() {
 }
 
 bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,
-                     const float* emission, const uint32_t* query, uint32_t nquery, uint32_t* query_pos,
-                     hipStream_t s, DeviceScene& out, BvhGpuTimes* times, const char** why) {
+                     const float* emission, const uint32_t* emitters, uint32_t n_emit, const uint32_t* query,
+                     uint32_t nquery, uint32_t* query_pos, hipStream_t s, DeviceScene& out, BvhGpuTimes* times,
+                     const char** why) {
     *why = "";
     out.release();
     if (nt < 8 || nt > (1u << 29) || nv == 0) { *why = "size"; return false; }
@@ -1010,7 +1022,29 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
     BVH_CHECK(hipMemcpyAsync(dverts.p, verts, 12 * size_t(nv), hipMemcpyHostToDevice, s));
     BVH_CHECK(hipMemcpyAsync(didx.p, idx, 4 * n3, hipMemcpyHostToDevice, s));
     BVH_CHECK(hipMemcpyAsync(dalb.p, albedo, 4 * n3, hipMemcpyHostToDevice, s));
-    BVH_CHECK(hipMemcpyAsync(demi.p, emission, 4 * n3, hipMemcpyHostToDevice, s));
+    // emission: the whole array, or zeros and the few nonzero triangles' values
+    const bool sparse = emitters && n_emit <= nt / 8;
+    std::vector<float> evals;
+    DevBuf<uint32_t> deids;
+    DevBuf<float> devals;
+    if (!sparse) {
+        BVH_CHECK(hipMemcpyAsync(demi.p, emission, 4 * n3, hipMemcpyHostToDevice, s));
+    } else {
+        evals.resize(3 * size_t(n_emit));
+        for (uint32_t j = 0; j < n_emit; ++j) {
+            if (emitters[j] >= nt) { *why = "emitter id"; return false; }
+            std::memcpy(&evals[3 * size_t(j)], &emission[3 * size_t(emitters[j])], 12);
+        }
+        if (!deids.alloc(n_emit) || !devals.alloc(3 * size_t(n_emit))) { *why = "allocation"; return false; }
+        hipLaunchKernelGGL(k_fill_u32_bvh, dim3(grid256(n3)), dim3(256), 0, s, reinterpret_cast<uint32_t*>(demi.p),
+                           n3, 0u);
+        if (n_emit) {
+            BVH_CHECK(hipMemcpyAsync(deids.p, emitters, 4 * size_t(n_emit), hipMemcpyHostToDevice, s));
+            BVH_CHECK(hipMemcpyAsync(devals.p, evals.data(), 12 * size_t(n_emit), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_scatter_emission, dim3(grid256(n_emit)), dim3(256), 0, s, deids.p, devals.p, n_emit,
+                               demi.p);
+        }
+    }
     hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, err.p, size_t(1), 0u);
     hipLaunchKernelGGL(k_gather_corners, dim3(grid256(nt)), dim3(256), 0, s, dverts.p, nv, didx.p, nt, dp[0].p,
                        dp[1].p, dp[2].p, err.p);

@@ -54,13 +54,17 @@ struct DeviceScene {
 };
 
 // Builds a mesh on the current device and stream: vertex gather, binned-SAH
-// tree, layouts and the BVH-order triangle / material arrays.  query
-// (nquery input triangle ids, each < nt) receives in query_pos the BVH-order
-// position of each.  False (with a reason) on any failure, including a vertex
-// index out of range; the caller then builds on the host.
+// tree, layouts and the BVH-order triangle / material arrays.  emitters
+// (n_emit input triangle ids, ascending) lists every triangle whose emission
+// has a nonzero bit: only those emissions are uploaded when they are few (all
+// others are zero).  query (nquery input triangle ids, each < nt) receives
+// in query_pos the BVH-order position of each.  False (with a reason) on any
+// failure, including a vertex index out of range; the caller then builds on
+// the host.
 bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,
-                     const float* emission, const uint32_t* query, uint32_t nquery, uint32_t* query_pos,
-                     hipStream_t s, DeviceScene& out, BvhGpuTimes* times, const char** why);
+                     const float* emission, const uint32_t* emitters, uint32_t n_emit, const uint32_t* query,
+                     uint32_t nquery, uint32_t* query_pos, hipStream_t s, DeviceScene& out, BvhGpuTimes* times,
+                     const char** why);
 
 // The binary tree that build_scene_gpu builds for triangles p0/p1/p2 (host
 // arrays, 3 floats per triangle), downloaded: tree node slots (with `unused`

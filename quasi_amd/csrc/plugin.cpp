@@ -133,20 +133,31 @@ void parallel_ranges(size_t n, unsigned threads, F&& f) {
     for (auto& t : pool) t.join();
 }
 
-// Emissive triangles (max emission > 0.1), in input order.
-std::vector<uint32_t> emissive_triangles(const float* emission, uint32_t nt, unsigned threads) {
+// Emissive triangles (max emission > 0.1, the light table's) and triangles
+// whose emission has any nonzero bit (what a sparse upload must carry), in
+// input order.
+struct EmissionScan {
+    std::vector<uint32_t> lights, nonzero;
+};
+EmissionScan scan_emission(const float* emission, uint32_t nt, unsigned threads) {
     const unsigned parts = std::max(1u, std::min(threads, 16u));
-    std::vector<std::vector<uint32_t>> found(parts);
+    std::vector<EmissionScan> found(parts);
     const size_t chunk = (size_t(nt) + parts - 1) / parts;
     parallel_ranges(parts, nt >= (1u << 16) ? parts : 1u, [&](size_t pb, size_t pe) {
         for (size_t part = pb; part < pe; ++part)
             for (size_t t = part * chunk; t < std::min<size_t>(nt, (part + 1) * chunk); ++t) {
                 const float* e = &emission[3 * t];
-                if (std::max(e[0], std::max(e[1], e[2])) > 0.1f) found[part].push_back(uint32_t(t));
+                if (std::max(e[0], std::max(e[1], e[2])) > 0.1f) found[part].lights.push_back(uint32_t(t));
+                uint32_t bits[3];
+                std::memcpy(bits, e, sizeof bits);
+                if (bits[0] | bits[1] | bits[2]) found[part].nonzero.push_back(uint32_t(t));
             }
     });
-    std::vector<uint32_t> all;
-    for (auto& f : found) all.insert(all.end(), f.begin(), f.end());
+    EmissionScan all;
+    for (auto& f : found) {
+        all.lights.insert(all.lights.end(), f.lights.begin(), f.lights.end());
+        all.nonzero.insert(all.nonzero.end(), f.nonzero.begin(), f.nonzero.end());
+    }
     return all;
 }
 
@@ -194,15 +205,17 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
     auto ms_since = [](clock::time_point t) { return std::chrono::duration<double, std::milli>(clock::now() - t).count(); };
     out.builder = Q_BVH_BUILD_HOST;
     out.device_ms = 0;
-    const std::vector<uint32_t> lights = emissive_triangles(emission, nt, host_threads);
+    const EmissionScan scan = scan_emission(emission, nt, host_threads);
+    const std::vector<uint32_t>& lights = scan.lights;
     const bool gpu = threads == 0 && (builder == Q_BVH_BUILD_GPU ||
                                       (builder == Q_BVH_BUILD_AUTO && nt >= kGpuBuildMinTriangles));
     if (gpu) {
         // the whole scene on the device: gather, tree, layouts, BVH-order arrays
         std::vector<uint32_t> lpos(lights.size());
         const char* why = "";
-        if (build_scene_gpu(verts, nv, idx, nt, albedo, emission, lights.data(), uint32_t(lights.size()),
-                            lpos.data(), st->stream, out.device, &out.gpu_times, &why)) {
+        if (build_scene_gpu(verts, nv, idx, nt, albedo, emission, scan.nonzero.data(), uint32_t(scan.nonzero.size()),
+                            lights.data(), uint32_t(lights.size()), lpos.data(), st->stream, out.device,
+                            &out.gpu_times, &why)) {
             const DeviceScene& d = out.device;
             if (d.depth > megakernel_max_depth()) {
                 log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", d.depth, megakernel_max_depth());
