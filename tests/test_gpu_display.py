@@ -1,5 +1,5 @@
-"""The display path (SURVEY next row: the reference's tonemap_frag,
-plugin.mm:334-350, and the host's orbit camera, main.cpp:39-112):
+"""The display path (the reference's tonemap_frag, plugin.mm:334-350, and
+the host's orbit camera, main.cpp:39-112):
 Q_plugin_readback_display against numpy on the plugin's own float mean, the
 host's PPM against the same export, and the orbit camera with its per-frame
 camera_dirty reset (plugin.mm:643-646) against a one-frame render at the final
