@@ -125,3 +125,30 @@ def test_rejected_scene_keeps_the_previous_one():
     assert np.array_equal(before, after)
     assert np.allclose(ga0, ga1, rtol=1e-6, atol=1e-9) and np.allclose(gv0, gv1, rtol=1e-6, atol=1e-9)
     assert np.abs(ga0).max() > 0 and np.abs(gv0).max() > 0
+
+
+def test_rejected_large_scene_through_the_gpu_build():
+    """A 20k-triangle mesh goes to the GPU build: its corner gather finds the
+    index out of range, the host path then rejects the scene with its own
+    check, and the previous scene keeps rendering bit for bit."""
+    good = scenes.soup(20_000, seed=5)
+    bad = scenes.Mesh(good.vertices.copy(), good.indices.copy(), good.albedo.copy(), good.emission.copy(),
+                      dict(good.camera), "bad")
+    bad.indices[12_345, 2] = 10 * good.vertices.shape[0]
+    w, h = 40, 30
+    with HipRenderer(w, h) as r:
+        r.set_scene(good)
+        assert r.stats().bvh_builder == abi.Q_BVH_BUILD_GPU
+        r.camera = cam_of(good)
+        r.set_params(spp_per_render=2, max_depth=4, rr_start=3, rng_mode=1, integrator=1, seed=6)
+        r.render(1)
+        before = r.readback()
+        with pytest.raises(RuntimeError):
+            r.set_scene(bad)
+        log = "\n".join(r.messages[-3:])
+        say(f"[rejected-gpu] {log}")
+        assert "GPU build failed (vertex index out of range)" in log and "out of range" in r.messages[-1]
+        r.set_params(seed=6)
+        r.render(1)
+        after = r.readback()
+    assert np.array_equal(before, after)
