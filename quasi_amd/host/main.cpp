@@ -290,6 +290,7 @@ int main(int argc, char** argv) {
     if (!configure()) return EXIT_FAILURE;
 
     std::printf("[Host] Batch mode: rendering %d frames then saving EXR\n", o.frames);
+    std::fflush(stdout);   // a watcher of this output learns that the frame loop starts
     auto last = std::chrono::steady_clock::now();
     const auto t0 = last;
     int since_load = 0;   // frames accumulated by the current plugin
