@@ -145,20 +145,33 @@ __device__ __forceinline__ bool hit_triangle(const float4* __restrict__ tris, in
 // runs the whole test anyway, and the three loads issue together.  The
 // predicate is the early-exit form's, NaN cases included: a value is rejected
 // only when one of its comparisons is true.
+// Every multiply-add is written out (fmaf) and nothing else is contracted, so
+// the test computes the same bits wherever it is inlined: the traversal
+// variants, the megakernel and the batch kernels all agree on every hit
+// whatever code the compiler places around it.
+__device__ __forceinline__ float dot_fma(const f3& a, const f3& b) {
+#pragma clang fp contract(off)
+    return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
+}
+__device__ __forceinline__ f3 cross_fma(const f3& a, const f3& b) {
+#pragma clang fp contract(off)
+    return mk(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
 __device__ __forceinline__ bool hit_triangle_rec(const float4* __restrict__ tri, const Ray& r, float tmin,
                                                  float tmax, float& t_out, uint32_t& id) {
+#pragma clang fp contract(off)
     const float4 t0 = tri[0];
     const float4 t1 = tri[1];
     const float4 t2 = tri[2];
     const f3 e1 = xyz(t1), e2 = xyz(t2);
-    const f3 pv = cross(r.d, e2);
-    const float det = dot(e1, pv);
+    const f3 pv = cross_fma(r.d, e2);
+    const float det = dot_fma(e1, pv);
     const float idet = 1.0f / det;
-    const f3 s = r.o - xyz(t0);
-    const float u = dot(s, pv) * idet;
-    const f3 q = cross(s, e1);
-    const float vv = dot(r.d, q) * idet;
-    const float t = dot(e2, q) * idet;
+    const f3 s = mk(r.o.x - t0.x, r.o.y - t0.y, r.o.z - t0.z);
+    const float u = dot_fma(s, pv) * idet;
+    const f3 q = cross_fma(s, e1);
+    const float vv = dot_fma(r.d, q) * idet;
+    const float t = dot_fma(e2, q) * idet;
     const bool reject = det == 0.0f || u < 0.0f || u > 1.0f || vv < 0.0f || u + vv > 1.0f || t < tmin || t > tmax;
     if (reject) return false;
     t_out = t;
