@@ -354,9 +354,18 @@ bool clear_accumulation(plugin_state* st, hipStream_t s) {
 
 bool resize_framebuffer(plugin_state* st, uint32_t w, uint32_t h, hipStream_t s) {
     const size_t npix = size_t(w) * h;
-    for (auto& b : st->accum)
-        if (!ensure_buffer(st, b, std::max<size_t>(16, 16 * npix), "accumulation buffer")) return false;
-    if (!ensure_buffer(st, st->ids, std::max<size_t>(4, 4 * npix), "id buffer")) return false;
+    bool ok = true;
+    for (auto& b : st->accum) ok = ok && ensure_buffer(st, b, std::max<size_t>(16, 16 * npix), "accumulation buffer");
+    ok = ok && ensure_buffer(st, st->ids, std::max<size_t>(4, 4 * npix), "id buffer");
+    if (!ok) {
+        // release the framebuffer altogether: as 0x0, the next frame
+        // allocates again whatever its size, and readbacks return nothing
+        for (auto& b : st->accum) free_buffer(b);
+        free_buffer(st->ids);
+        st->width = st->height = 0;
+        st->sample_count = 0;
+        return false;
+    }
     st->width = w;
     st->height = h;
     return clear_accumulation(st, s);
@@ -939,106 +948,132 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
 Q_EXPORT int32_t Q_plugin_readback_ids(Q_plugin_handle* handle, uint32_t* out, uint64_t count) {
     if (!handle || !out) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
-    const uint64_t npix = uint64_t(st->width) * st->height;
-    if (count < npix) {
-        log_msg(st, "readback_ids: buffer holds %llu ids, %llu needed", (unsigned long long)count,
-                (unsigned long long)npix);
-        return Q_EXT_ERR_ARGUMENT;
+    try {
+        const uint64_t npix = uint64_t(st->width) * st->height;
+        if (count < npix) {
+            log_msg(st, "readback_ids: buffer holds %llu ids, %llu needed", (unsigned long long)count,
+                    (unsigned long long)npix);
+            return Q_EXT_ERR_ARGUMENT;
+        }
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        if (npix == 0) return Q_EXT_OK;   // no framebuffer (a resize failed)
+        if (!hip_ok(st, hipMemcpy(out, st->ids.ptr, 4 * npix, hipMemcpyDeviceToHost), "id readback")) return Q_EXT_ERR_DEVICE;
+        return Q_EXT_OK;
+    } catch (...) {
+        log_msg(st, "%s: exception", "readback_ids");
+        return Q_EXT_ERR_STATE;
     }
-    (void)hipSetDevice(st->device);
-    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
-    if (!hip_ok(st, hipMemcpy(out, st->ids.ptr, 4 * npix, hipMemcpyDeviceToHost), "id readback")) return Q_EXT_ERR_DEVICE;
-    return Q_EXT_OK;
 }
 
 Q_EXPORT int32_t Q_plugin_readback_display(Q_plugin_handle* handle, uint8_t* out, uint64_t bytes) {
     if (!handle || !out) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
-    const uint64_t npix = uint64_t(st->width) * st->height;
-    if (bytes < 4 * npix) {
-        log_msg(st, "readback_display: buffer holds %llu bytes, %llu needed", (unsigned long long)bytes,
-                (unsigned long long)(4 * npix));
-        return Q_EXT_ERR_ARGUMENT;
-    }
-    (void)hipSetDevice(st->device);
-    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
-    if (st->sample_count == 0 || npix == 0) {
-        log_msg(st, "readback_display: nothing rendered yet");
+    try {
+        const uint64_t npix = uint64_t(st->width) * st->height;
+        if (bytes < 4 * npix) {
+            log_msg(st, "readback_display: buffer holds %llu bytes, %llu needed", (unsigned long long)bytes,
+                    (unsigned long long)(4 * npix));
+            return Q_EXT_ERR_ARGUMENT;
+        }
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        if (st->sample_count == 0 || npix == 0) {
+            log_msg(st, "readback_display: nothing rendered yet");
+            return Q_EXT_ERR_STATE;
+        }
+        if (!ensure_buffer(st, st->resolved, 16 * npix, "readback staging")) return Q_EXT_ERR_OUT_OF_MEMORY;
+        {
+            LaunchTimer lt(st, st->stream, K_RESOLVE);
+            if (!hip_ok(st, launch_tonemap(static_cast<const float4*>(st->accum[Q_AOV_BEAUTY].ptr), st->resolved.ptr, npix,
+                                           st->sample_count, st->stream),
+                        "tonemap"))
+                return Q_EXT_ERR_DEVICE;
+        }
+        if (!hip_ok(st, hipMemcpyAsync(out, st->resolved.ptr, 4 * npix, hipMemcpyDeviceToHost, st->stream),
+                    "display readback") ||
+            !hip_ok(st, hipStreamSynchronize(st->stream), "display readback"))
+            return Q_EXT_ERR_DEVICE;
+        return Q_EXT_OK;
+    } catch (...) {
+        log_msg(st, "%s: exception", "readback_display");
         return Q_EXT_ERR_STATE;
     }
-    if (!ensure_buffer(st, st->resolved, 16 * npix, "readback staging")) return Q_EXT_ERR_OUT_OF_MEMORY;
-    {
-        LaunchTimer lt(st, st->stream, K_RESOLVE);
-        if (!hip_ok(st, launch_tonemap(static_cast<const float4*>(st->accum[Q_AOV_BEAUTY].ptr), st->resolved.ptr, npix,
-                                       st->sample_count, st->stream),
-                    "tonemap"))
-            return Q_EXT_ERR_DEVICE;
-    }
-    if (!hip_ok(st, hipMemcpyAsync(out, st->resolved.ptr, 4 * npix, hipMemcpyDeviceToHost, st->stream),
-                "display readback") ||
-        !hip_ok(st, hipStreamSynchronize(st->stream), "display readback"))
-        return Q_EXT_ERR_DEVICE;
-    return Q_EXT_OK;
 }
 
 Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out) {
     if (!handle || !out || out->struct_size < kStatsMinSize) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
-    (void)hipSetDevice(st->device);
-    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
-    harvest_events(st);
-    const uint32_t out_size = out->struct_size;
-    Counters c{};
-    if (!hip_ok(st, hipMemcpy(&c, st->counters.ptr, sizeof c, hipMemcpyDeviceToHost), "counter readback"))
-        return Q_EXT_ERR_DEVICE;
-    Q_render_stats s{};
-    s.struct_size = sizeof(Q_render_stats);
-    s.width = st->width;
-    s.height = st->height;
-    s.sample_count = st->sample_count;
-    s.num_triangles = st->num_tris;
-    s.num_lights = st->num_lights;
-    s.bvh_nodes = st->num_nodes;
-    s.bvh_depth = st->bvh_depth;
-    s.rays_camera = c.rays_camera;
-    s.rays_extension = c.rays_extension;
-    s.rays_shadow = c.rays_shadow;
-    s.bvh_build_ms = st->bvh_build_ms;
-    s.bvh_builder = st->bvh_builder;
-    s.bvh_sah_cost = st->bvh_sah_cost;
-    s.bvh_device_ms = st->bvh_device_ms;
-    s.render_ms = 0;
-    for (uint32_t k = 0; k < K_COUNT; ++k)
-        if (kRenderKernel[k] && k != K_BATCH_PASS) s.render_ms += st->kernel_ms[k];
-    s.num_kernels = K_COUNT;
-    for (uint32_t k = 0; k < K_COUNT; ++k) {
-        std::snprintf(s.kernels[k].name, sizeof s.kernels[k].name, "%s", kKernelNames[k]);
-        s.kernels[k].launches = st->launches[k];
-        s.kernels[k].total_ms = st->kernel_ms[k];
+    try {
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        harvest_events(st);
+        const uint32_t out_size = out->struct_size;
+        Counters c{};
+        if (!hip_ok(st, hipMemcpy(&c, st->counters.ptr, sizeof c, hipMemcpyDeviceToHost), "counter readback"))
+            return Q_EXT_ERR_DEVICE;
+        Q_render_stats s{};
+        s.struct_size = sizeof(Q_render_stats);
+        s.width = st->width;
+        s.height = st->height;
+        s.sample_count = st->sample_count;
+        s.num_triangles = st->num_tris;
+        s.num_lights = st->num_lights;
+        s.bvh_nodes = st->num_nodes;
+        s.bvh_depth = st->bvh_depth;
+        s.rays_camera = c.rays_camera;
+        s.rays_extension = c.rays_extension;
+        s.rays_shadow = c.rays_shadow;
+        s.bvh_build_ms = st->bvh_build_ms;
+        s.bvh_builder = st->bvh_builder;
+        s.bvh_sah_cost = st->bvh_sah_cost;
+        s.bvh_device_ms = st->bvh_device_ms;
+        s.render_ms = 0;
+        for (uint32_t k = 0; k < K_COUNT; ++k)
+            if (kRenderKernel[k] && k != K_BATCH_PASS) s.render_ms += st->kernel_ms[k];
+        s.num_kernels = K_COUNT;
+        for (uint32_t k = 0; k < K_COUNT; ++k) {
+            std::snprintf(s.kernels[k].name, sizeof s.kernels[k].name, "%s", kKernelNames[k]);
+            s.kernels[k].launches = st->launches[k];
+            s.kernels[k].total_ms = st->kernel_ms[k];
+        }
+        s.struct_size = out_size;
+        std::memcpy(out, &s, std::min<size_t>(out_size, sizeof s));
+        return Q_EXT_OK;
+    } catch (...) {
+        log_msg(st, "%s: exception", "get_stats");
+        return Q_EXT_ERR_STATE;
     }
-    s.struct_size = out_size;
-    std::memcpy(out, &s, std::min<size_t>(out_size, sizeof s));
-    return Q_EXT_OK;
 }
 
 Q_EXPORT int32_t Q_plugin_reset_stats(Q_plugin_handle* handle) {
     if (!handle) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
-    (void)hipSetDevice(st->device);
-    if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
-    harvest_events(st);
-    for (uint32_t k = 0; k < K_COUNT; ++k) { st->launches[k] = 0; st->kernel_ms[k] = 0; }
-    if (!hip_ok(st, launch_fill_u32(st->counters.ptr, sizeof(Counters) / 4, 0u, st->stream), "counter clear") ||
-        !hip_ok(st, hipStreamSynchronize(st->stream), "counter clear"))
-        return Q_EXT_ERR_DEVICE;
-    return Q_EXT_OK;
+    try {
+        (void)hipSetDevice(st->device);
+        if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+        harvest_events(st);
+        for (uint32_t k = 0; k < K_COUNT; ++k) { st->launches[k] = 0; st->kernel_ms[k] = 0; }
+        if (!hip_ok(st, launch_fill_u32(st->counters.ptr, sizeof(Counters) / 4, 0u, st->stream), "counter clear") ||
+            !hip_ok(st, hipStreamSynchronize(st->stream), "counter clear"))
+            return Q_EXT_ERR_DEVICE;
+        return Q_EXT_OK;
+    } catch (...) {
+        log_msg(st, "%s: exception", "reset_stats");
+        return Q_EXT_ERR_STATE;
+    }
 }
 
 Q_EXPORT int32_t Q_plugin_synchronize(Q_plugin_handle* handle) {
     if (!handle) return Q_EXT_ERR_ARGUMENT;
     plugin_state* st = as_state(handle);
-    (void)hipSetDevice(st->device);
-    return sync_render(st) ? Q_EXT_OK : Q_EXT_ERR_DEVICE;
+    try {
+        (void)hipSetDevice(st->device);
+        return sync_render(st) ? Q_EXT_OK : Q_EXT_ERR_DEVICE;
+    } catch (...) {
+        log_msg(st, "%s: exception", "synchronize");
+        return Q_EXT_ERR_STATE;
+    }
 }
 
 }  // extern "C"
