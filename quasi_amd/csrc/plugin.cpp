@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -33,6 +34,8 @@ const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve",  "fill",    
                                            "wf_shade",   "wf_shadow", "wf_accumulate", "batch_pass"};
 const bool kRenderKernel[K_COUNT] = {true, false, false, true, true, true, true, true, true};
 
+// Scene builds of a batch run on host threads and may log at once: the
+// host's callback is entered by one thread at a time.
 void log_msg(plugin_state* st, const char* fmt, ...) {
     if (!st || !st->context || !st->context->log) return;
     char buf[512];
@@ -40,6 +43,8 @@ void log_msg(plugin_state* st, const char* fmt, ...) {
     va_start(ap, fmt);
     std::vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
+    static std::mutex log_mutex;
+    std::lock_guard<std::mutex> lock(log_mutex);
     st->context->log(st->context->host_data, buf);
 }
 

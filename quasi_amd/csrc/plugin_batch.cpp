@@ -452,6 +452,10 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
         log_msg(st, "autotune: the single-scene target needs the wavefront engine");
         return Q_EXT_ERR_STATE;
     }
+    if (target != Q_AUTOTUNE_BATCH && (!st->have_camera || st->width == 0 || !st->scene_ok)) {
+        log_msg(st, "autotune: the single-scene probes render the last frame's camera: render a frame first");
+        return Q_EXT_ERR_STATE;
+    }
     if (probe_spp == 0) probe_spp = 1;
     if (target == Q_AUTOTUNE_BATCH) {
         const uint32_t pp = batch_samples_per_pass(st, probe_spp);
@@ -459,6 +463,22 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
     }
     (void)hipSetDevice(st->device);
     if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
+    // the probes' rays, launches and kernel times are not the caller's: the
+    // statistics are put back as they were once tuning ends
+    harvest_events(st);
+    Counters counters{};
+    if (!hip_ok(st, hipMemcpy(&counters, st->counters.ptr, sizeof counters, hipMemcpyDeviceToHost), "autotune"))
+        return Q_EXT_ERR_DEVICE;
+    uint64_t launches[K_COUNT];
+    double kernel_ms[K_COUNT];
+    std::copy(std::begin(st->launches), std::end(st->launches), launches);
+    std::copy(std::begin(st->kernel_ms), std::end(st->kernel_ms), kernel_ms);
+    auto restore_stats = [&]() {
+        harvest_events(st);
+        std::copy(launches, launches + K_COUNT, std::begin(st->launches));
+        std::copy(kernel_ms, kernel_ms + K_COUNT, std::begin(st->kernel_ms));
+        return hip_ok(st, hipMemcpy(st->counters.ptr, &counters, sizeof counters, hipMemcpyHostToDevice), "autotune");
+    };
     Q_autotune_result res{};
     res.struct_size = sizeof(Q_autotune_result);
     const std::string key = tune_key(st, target);
@@ -481,6 +501,8 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
             // first run warms code and caches, the second is kept
             if (!time_probe(st, target, probe_spp, warm) || !time_probe(st, target, probe_spp, ms)) {
                 apply_config(st, saved);
+                (void)sync_render(st);
+                (void)restore_stats();
                 return Q_EXT_ERR_DEVICE;
             }
             res.entries[i] = Q_autotune_entry{c.width, c.block, c.chunk, 0, double(ms)};
@@ -495,12 +517,12 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
                 target == Q_AUTOTUNE_BATCH ? "batch" : "single", e.bvh_width, e.block_size, e.queue_chunk, e.ms);
     }
     // leave the pass graph of the chosen configuration captured, the
-    // accumulations empty and the counters as they were before tuning
+    // accumulations empty and the statistics as they were before tuning
     if (target == Q_AUTOTUNE_BATCH && !prepare_batch_pass(st, batch_samples_per_pass(st, probe_spp)))
         return Q_EXT_ERR_DEVICE;
     if (st->batch.active && !clear_batch(st, st->stream)) return Q_EXT_ERR_DEVICE;
     if (!clear_single(st, st->stream)) return Q_EXT_ERR_DEVICE;
-    if (!hip_ok(st, hipStreamSynchronize(st->stream), "autotune")) return Q_EXT_ERR_DEVICE;
+    if (!sync_render(st) || !restore_stats()) return Q_EXT_ERR_DEVICE;
     if (out) *out = res;
     return Q_EXT_OK;
 }

@@ -135,10 +135,19 @@ def test_autotune_batch(tmp_path, monkeypatch):
     batch = small_batch()[:4]
     say("[autotune]")
     with HipRenderer(16, 16) as r:
-        r.set_params(spp_per_render=1, seed=0, **PARAMS)
+        r.set_params(spp_per_render=1, seed=0, **dict(PARAMS, flags=abi.Q_FLAG_RECORD_IDS | abi.Q_FLAG_TIME_KERNELS))
         r.set_batch(batch)
+        r.render_batch(1)
+        before = r.stats()
         res = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=1)
         assert res.num_entries == 20 and res.best < res.num_entries and not res.from_cache
+        # the probes leave the caller's statistics as they were
+        after = r.stats()
+        assert before.rays_camera > 0
+        assert (after.rays_camera, after.rays_extension, after.rays_shadow) == \
+            (before.rays_camera, before.rays_extension, before.rays_shadow)
+        assert [(after.kernels[k].launches, after.kernels[k].total_ms) for k in range(after.num_kernels)] == \
+            [(before.kernels[k].launches, before.kernels[k].total_ms) for k in range(before.num_kernels)]
         best = res.entries[res.best]
         assert all(best.ms <= res.entries[k].ms for k in range(res.num_entries))
         p = r.get_params()
