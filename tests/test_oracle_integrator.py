@@ -1,7 +1,8 @@
 """Pins the oracle's sampling and light transport with closed forms.
 
 - cosine_sample (plugin.mm:111-125): cos(theta) = sqrt(1 - r2) exactly, unit
-  length, upper hemisphere, E[d] = (2/3) n.
+  length, upper hemisphere, E[d] = (2/3) n, and the density cos(theta)/pi
+  (Kolmogorov-Smirnov on cos(theta) and on the azimuth).
 - Reference bounce loop (plugin.mm:223-276): furnace closed form.  In a closed
   box whose walls all have albedo a and emission E <= 0.1 (so no wall counts
   as an emitter), every path sees E * sum_{k<D} a^k exactly.
@@ -44,6 +45,27 @@ def test_cosine_sample_moments(n):
     assert (c ** 2).mean() == pytest.approx(0.5, abs=0.006)   # E[cos^2] = 1/2
     tang = ds - np.outer(c, nn)
     assert np.abs(tang.mean(0)).max() < 0.01                   # no azimuthal bias
+
+
+@pytest.mark.parametrize("n", [(0, 1, 0), (0.95, 0.1, 0.0), (-0.3, -0.2, 0.9)])
+def test_cosine_sample_pdf_ks(n):
+    """Directions follow the density cos(theta)/pi: P(cos theta <= c) = c^2 and the
+    azimuth, measured in a tangent basis built here independently of the
+    sampler's, is uniform (Kolmogorov-Smirnov on both)."""
+    from scipy import stats
+    n = np.array(n, dtype=np.float64)
+    nn = n / np.linalg.norm(n)
+    states = np.random.default_rng(11).integers(0, 2**32, size=20000)
+    ds = np.array([O.cosine_sample(n, int(s)) for s in states])
+    c = np.clip(ds @ nn, 0.0, 1.0)
+    assert stats.kstest(c, lambda x: np.clip(x, 0, 1) ** 2).pvalue > 1e-3
+    t1 = np.cross(nn, [0.0, 0.0, 1.0] if abs(nn[2]) < 0.5 else [1.0, 0.0, 0.0])
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(nn, t1)
+    phi = np.arctan2(ds @ t2, ds @ t1) % (2 * np.pi)
+    assert stats.kstest(phi / (2 * np.pi), "uniform").pvalue > 1e-3
+    # a sampler with the wrong density (uniform over the hemisphere) fails the same test
+    assert stats.kstest(np.sqrt(c), lambda x: np.clip(x, 0, 1) ** 2).pvalue < 1e-6
 
 
 def closed_box(albedo, emission, half=1.0):
