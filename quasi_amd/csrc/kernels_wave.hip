@@ -284,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // uniform over the grid, plus a 32-bit byte offset (in a batch: the lane's
 // scene's offset plus the record's), which the loads take as they are;
 // otherwise each lane holds 64-bit pointers to its scene's arrays.
-constexpr uint32_t kRefillIdle = 40;   // measured best of 8..48 on the 1M soup
+constexpr uint32_t kRefillIdle = 32;   // measured best of 8..48 on the 1M soup (28-32 alike)
 constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32>
