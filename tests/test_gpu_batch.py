@@ -68,6 +68,31 @@ def test_batch_equals_plain_per_scene(builder):
             assert np.array_equal(got[i][1], ids), i
 
 
+def test_batch_with_an_empty_scene():
+    """A scene of no triangles among others: its image is zero and its ids are
+    misses, and the other scenes still equal their plain renders."""
+    z3 = np.zeros((0, 3), dtype=np.float32)
+    room = scenes.cornell36(5)
+    empty = scenes.Mesh(z3, np.zeros((0, 3), dtype=np.uint32), z3, z3, room.camera, "empty")
+    batch = [(room, 24, 16, 11), (empty, 16, 16, 12), (scenes.cornell36(6), 16, 24, 13)]
+    spp = 3
+    say("[batch-empty]")
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, bvh_builder=abi.Q_BVH_BUILD_GPU, **PARAMS)
+        r.set_batch(batch)
+        r.render_batch(spp)
+        got = [(r.readback_batch(i), r.batch_ids(i)) for i in range(len(batch))]
+        for k in ("beauty", "albedo", "normal", "depth"):
+            assert float(np.abs(got[1][0][k][..., :3]).max()) == 0.0, k
+        assert (got[1][1] == 0xFFFFFFFF).all()
+        for i in (0, 2):
+            mesh, w, h, seed = batch[i]
+            aov, ids = plain_render(r, mesh, w, h, seed, spp)
+            for k in ("beauty", "albedo", "normal", "depth"):
+                assert np.array_equal(got[i][0][k], aov[k]), (i, k)
+            assert np.array_equal(got[i][1], ids), i
+
+
 def test_graph_replays_match_eager_with_work_between():
     batch = small_batch()[:4]
     say("[graph-replay]")

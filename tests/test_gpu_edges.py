@@ -1,6 +1,6 @@
 """GPU edge cases against the FP64 oracle, through the C ABI: ragged and tiny
-images, a one-triangle scene without lights, a camera that sees nothing,
-depth 1, and a sample count that does not divide into whole passes.  Same bars
+images, a one-triangle scene without lights, a camera that sees nothing, a
+scene of no triangles, depth 1, and a sample count that does not divide into whole passes.  Same bars
 as test_gpu_parity.py (relative L1 <= 1e-4 on the same samples, ids >= 99.99%
 with only near-tie disagreements, exact camera-ray counts)."""
 import numpy as np
@@ -82,6 +82,36 @@ def test_camera_that_sees_nothing(engine):
     aov, st, ref = render_and_compare(m, 20, 12, 2, engine, "miss")
     assert st.rays_extension == 0 and st.rays_shadow == 0
     assert float(np.abs(aov["beauty"][..., :3]).max()) == 0.0
+
+
+def empty_mesh():
+    m = scenes.cornell36(1)
+    z3 = np.zeros((0, 3), dtype=np.float32)
+    return scenes.Mesh(z3, np.zeros((0, 3), dtype=np.uint32), z3, z3, m.camera, "empty")
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+@pytest.mark.parametrize("integrator", [0, 1], ids=["reference", "nee"])
+def test_empty_scene(engine, integrator):
+    """A scene of no triangles: every camera ray misses, nothing else is
+    traced, the image and the first-hit records are zero and every id is a
+    miss."""
+    w, h, spp = 20, 12, 3
+    say(f"[empty] engine {engine} integrator {integrator}")
+    with HipRenderer(w, h) as r:
+        r.set_scene(empty_mesh())
+        r.camera = cam_of(empty_mesh())
+        r.set_params(spp_per_render=spp, flags=abi.Q_FLAG_RECORD_IDS, id_sample=0, engine=engine, max_depth=4,
+                     rr_start=3, rng_mode=1, integrator=integrator, seed=3)
+        r.render(2)
+        aov = r.readback_aov()
+        ids = r.primary_ids()
+        st = r.stats()
+    assert st.num_triangles == 0 and st.bvh_nodes == 0
+    assert st.rays_camera == 2 * w * h * spp and st.rays_extension == 0 and st.rays_shadow == 0
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert float(np.abs(aov[k][..., :3]).max()) == 0.0, k
+    assert (ids == 0xFFFFFFFF).all()
 
 
 @pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
