@@ -71,13 +71,11 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
         if (e < kMaxDepth * kQueueSegments) S.shadow_counts[e * kCounterStride] = 0u;
         S.fetch[e * kCounterStride] = 0u;
     }
+    // throughput, radiance and the first-hit records are not written here:
+    // bounce 0's shade writes them for every path, hit or miss
     const uint32_t s0 = pass_sample_begin(P);
     float4* ro = static_cast<float4*>(S.ray_o[0]);
     float4* rd = static_cast<float4*>(S.ray_d[0]);
-    float4* thr = static_cast<float4*>(S.thr);
-    float4* rad = static_cast<float4*>(S.radiance);
-    float4* a0 = static_cast<float4*>(S.aov0);
-    float4* a1 = static_cast<float4*>(S.aov1);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < S.paths; p += gridDim.x * blockDim.x) {
         const uint32_t sid = BATCH ? scene_of_path(P, p) : 0u;
         const SceneDesc& sc = scene_at<BATCH>(P, sid);
@@ -91,10 +89,6 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
         const Ray r = camera_ray(sc.cam, sc.width, sc.height, x, y, rng);
         ro[p] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p));
         rd[p] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(sid));
-        thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
-        rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a1[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kMissId));
         S.rng[p] = rng;
     }
 }
@@ -194,8 +188,12 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
             if (P.rec_hit) static_cast<uint2*>(P.rec_hit)[size_t(b) * S.paths + p] = h;
             if (tri >= 0) {
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
-                float4 t4 = thr_a[p], l4 = rad_a[p];
-                f3 thr = xyz(t4), L = xyz(l4);
+                // a camera ray starts with throughput 1 and no radiance
+                f3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
+                if (!FIRST) {
+                    thr = xyz(thr_a[p]);
+                    L = xyz(rad_a[p]);
+                }
                 uint32_t rng = S.rng[p];
                 FirstHit first;
                 shade_hit(P, static_cast<const float4*>(sc.tris), static_cast<const float4*>(sc.mats), sc.lights,
@@ -210,6 +208,11 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PassParams P, WaveState S, uint
                 S.rng[p] = rng;
                 cont = so.cont;
                 shadow = so.shadow;
+            } else if (FIRST) {
+                // a camera ray that missed: no radiance, empty first-hit records
+                rad_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                a0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                a1[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kMissId));
             }
         }
         const uint32_t j = slot0 + wave_append(cont, next_len);
