@@ -1,8 +1,9 @@
 """Pins the oracle's RNG (restating backends/metal/plugin.mm:98-107, :295).
 
-Known answers were computed from the formula by hand-independent means (the
-values listed in the reference-path survey); the float conversion is checked
-against numpy's own u32 -> f32 rounding for a large random set of states.
+Known answers were computed from the formula of plugin.mm:98-102 independently
+of the oracle (plain integer arithmetic with an explicit 2^32 wrap); the float
+conversion is checked against numpy's own u32 -> f32 rounding for a large
+random set of states.
 """
 import numpy as np
 import pytest
