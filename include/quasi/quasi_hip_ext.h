@@ -213,7 +213,10 @@ int32_t Q_plugin_readback_batch_ids(Q_plugin_handle* handle, uint32_t scene, uin
  * on probe_spp samples of the current single scene (Q_AUTOTUNE_SINGLE, with
  * the last camera given to Q_plugin_render) or of the current batch
  * (Q_AUTOTUNE_BATCH), keeps the fastest in the render parameters and clears
- * the accumulation.  Results are cached, one JSON object per line, in
+ * the accumulation.  The probes' rays, launches and kernel times are not
+ * counted: Q_plugin_get_stats reads the same before and after.  The single
+ * target needs a frame rendered first (Q_EXT_ERR_STATE otherwise).  Results
+ * are cached, one JSON object per line, in
  * $QUASI_HIP_TUNE_CACHE (default ~/.cache/quasi_hip/autotune.jsonl), keyed by
  * GPU architecture, target and workload signature. */
 #define Q_AUTOTUNE_SINGLE 0u
