@@ -38,9 +38,12 @@ namespace quasi_hip {
 namespace {
 
 constexpr int kAdjBlock = 128;      // one 16 x 8 pixel tile per block
-constexpr int kRecWords = 12;       // o.xyz, d.xyz, thr.xyz, tri, rng, flags
+constexpr int kRecWords = 11;       // o.xyz, d.xyz, thr.xyz, tri | flags << 31, rng
 constexpr int kAggRounds = 8;       // per-triangle wave aggregation rounds before direct atomics
 constexpr uint32_t kRecNee = 1u;    // the vertex's NEE sample was requested and unoccluded
+// triangle indices are below 2^29, so the flag rides in the top bit of the
+// triangle word: 11 words per vertex let 7 blocks (14 waves) share a CU's LDS
+constexpr uint32_t kRecFlagShift = 31;
 
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -174,13 +177,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
                     REC(b, 6) = __float_as_uint(thr.x);
                     REC(b, 7) = __float_as_uint(thr.y);
                     REC(b, 8) = __float_as_uint(thr.z);
-                    REC(b, 9) = uint32_t(h.tri);
                     REC(b, 10) = rng;
                     ShadeOut out;
                     shade_hit(P, tris, mats, P.lights, P.num_lights, b, ray, h.t, h.tri, thr, L, rng, nullptr, out);
                     uint32_t fl = 0;
                     if (out.shadow && rec_vis[size_t(b) * A.rec_paths + pid]) fl |= kRecNee;
-                    REC(b, 11) = fl;
+                    REC(b, 9) = uint32_t(h.tri) | (fl << kRecFlagShift);
                     K = int(b) + 1;
                     if (!out.cont) break;
                     ray = out.next;
@@ -202,9 +204,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
                     const f3 o = mk(__uint_as_float(REC(j, 0)), __uint_as_float(REC(j, 1)), __uint_as_float(REC(j, 2)));
                     const f3 d = mk(__uint_as_float(REC(j, 3)), __uint_as_float(REC(j, 4)), __uint_as_float(REC(j, 5)));
                     const f3 thr = mk(__uint_as_float(REC(j, 6)), __uint_as_float(REC(j, 7)), __uint_as_float(REC(j, 8)));
-                    const int tri = int(REC(j, 9));
+                    const uint32_t w9 = REC(j, 9);
+                    const int tri = int(w9 & ((1u << kRecFlagShift) - 1u));
                     uint32_t rng = REC(j, 10);
-                    const uint32_t fl = REC(j, 11);
+                    const uint32_t fl = w9 >> kRecFlagShift;
                     const float4 t0 = tris[3 * tri + 0];
                     const float4 t1 = tris[3 * tri + 1];
                     const float4 t2 = tris[3 * tri + 2];
