@@ -28,17 +28,54 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s (+ % HBM roofline) at 1080p, batched vs plain; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
+# VALU lane throughput of MI355X: the published FP32 vector peak, 157.3 TFLOP/s,
+# counts an FMA lane-operation as 2 flops (256 CUs x 128 lanes x 2.4 GHz)
+VALU_PEAK_TLANE = 157.3 / 2
+COUNTERS = ROOT / "profiles" / "counters_soup1m_1080p.json"
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(hardware: int) -> tuple[int, str]:
+    """Threads for the CPU baseline: every hardware thread, unless the
+    machine's job limits (OMP_NUM_THREADS, the CPU affinity mask) give this
+    process fewer; the reason is reported beside the figure."""
+    limits = [hardware]
+    why = "all hardware threads"
+    try:
+        aff = len(os.sched_getaffinity(0))
+        if aff < min(limits):
+            limits.append(aff)
+            why = f"CPU affinity mask of this process ({aff} CPUs)"
+    except (AttributeError, OSError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and 0 < int(omp) < min(limits):
+        limits.append(int(omp))
+        why = f"OMP_NUM_THREADS={omp}: this process's share of the host's cores"
+    return min(limits), why
+
+
 def cpu_baseline(mesh, args, seconds: float):
     """The scalar oracle (FP32, std::thread pool) on this host's cores over a
     bounded, evenly spread subset of the same workload's pixels."""
     from oracle import oracle as O
-    threads = min(16, O.hardware_concurrency())
+    hardware = O.hardware_concurrency()
+    threads, why = cpu_threads(hardware)
+    model = cpu_model()
+    log(f"CPU baseline: {threads} threads of {hardware} ({why}); {model}")
     s = O.Scene(mesh)
     common = dict(max_depth=args.depth, rr_start=3, integrator=1, rng_mode=1, seed=0, precision=32)
     W, H = args.width, args.height
@@ -60,9 +97,45 @@ def cpu_baseline(mesh, args, seconds: float):
         secs += c.seconds
         offset += 4
     return dict(value=rays / secs / 1e6, unit="Mrays/s", cores=threads, kind="port", value_1core=one_core,
+                host_threads=hardware, cpu_model=model, threads_reason=why,
                 sample=(f"oracle FP32, {W}x{H} soup, samples 0..{offset - 1} of every 3rd pixel "
                         f"({paths} paths, {rays} rays, {secs:.1f} s on {threads} threads; 1-core figure from "
                         f"samples 0..3 of every 61st pixel; BVH build excluded)"))
+
+
+def roofline_of(kernel: str, avg_ms: float, launches: int, alg_bytes: float) -> dict:
+    """Roofline of the dominant kernel.  The traversal is bound by VALU issue
+    and SIMD efficiency, so the roof is VALU lane throughput: lane-operations
+    per launch (VALU wave instructions x 64 x the active-lane fraction, read
+    by rocprofv3 on this workload and kept in profiles/) over the launch time
+    measured here.  Beside it: the L2-to-fabric bytes per launch (FETCH_SIZE
+    and WRITE_SIZE scaled by a copy kernel; HBM or Infinity Cache) against
+    the HBM peak, the L2 hit rate, the wave occupancy, and the algorithmic
+    bytes of the frozen traversal model (which count every node visit, so
+    their rate is not bounded by HBM: a cache-amplification figure, not a
+    roof)."""
+    sec = avg_ms * 1e-3
+    out = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_TLANE, "unit": "T lane-op/s", "frac": None,
+           "traffic": None, "kernel": kernel, "avg_launch_ms": round(avg_ms, 3), "launches": launches,
+           "alg_bytes_per_launch": round(alg_bytes), "alg_bytes_rate_gbs": round(alg_bytes / sec / 1e9, 1)}
+    try:
+        c = json.loads(COUNTERS.read_text())["kernels"][kernel]
+    except (OSError, ValueError, KeyError):
+        log(f"no counters for {kernel} in {COUNTERS}: roofline without counter figures")
+        return out
+    lane_ops = c["valu_insts_per_dispatch"] * 64 * c["active_lane_frac"]
+    achieved = lane_ops / sec / 1e12
+    traffic = c["fetch_kib_per_dispatch"] * 1024 * c["fetch_scale"] + c["write_kib_per_dispatch"] * 1024 * c["write_scale"]
+    out.update(achieved=round(achieved, 3), frac=round(achieved / VALU_PEAK_TLANE, 4), traffic=round(traffic),
+               lane_ops_per_launch=round(lane_ops),
+               valu_insts_per_launch=c["valu_insts_per_dispatch"], active_lane_frac=c["active_lane_frac"],
+               valu_issue_frac=round(c["valu_insts_per_dispatch"] * 64 / sec / 1e12 / VALU_PEAK_TLANE, 4),
+               fabric={"achieved_gbs": round(traffic / sec / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
+                       "frac": round(traffic / sec / 1e9 / HBM_PEAK_GBS, 4),
+                       "note": "FETCH_SIZE + WRITE_SIZE: L2-to-fabric bytes (HBM or Infinity Cache)"},
+               l2_hit_rate=c.get("l2_hit_rate"), occupancy_waves_per_simd=c.get("occupancy_waves_per_simd"),
+               counters_source=str(COUNTERS.relative_to(ROOT)))
+    return out
 
 
 def bench_batch(args):
@@ -348,14 +421,7 @@ def main():
         bytes_rays = sum(rays[t] * alg[t]["bytes_per_ray"] for t in ("camera", "extension"))
         bytes_fb = 0.0
     per_launch = (bytes_rays + bytes_fb) / max(1, launches)
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tf = ROOT / "profiles" / "hbm_traffic.json"
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text()).get(f"{dominant}_soup_1m_1080p", {}).get("bytes_per_launch")
-        except (ValueError, AttributeError):
-            traffic = None
+    roofline = roofline_of(dominant, avg_ms, launches, per_launch)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -377,10 +443,7 @@ def main():
         "rays": rays,
         "rays_per_path": round(total_rays / (W * H * args.spp * args.steps), 4),
         "bvh_build_ms": round(st.bvh_build_ms, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": dominant, "avg_launch_ms": round(avg_ms, 3), "launches": launches,
-                     "alg_bytes_per_launch": round(per_launch)},
+        "roofline": roofline,
         "kernel_ms": {n: round(v.total_ms, 3) for n, v in kern.items() if v.launches},
     }
     out["batch"] = None if args.no_batch else bench_batch(args)

@@ -148,6 +148,65 @@ def test_l2_target_mode_and_full_size_subsample():
     assert rel_l1(gs, ra) <= 1e-3 and rel_l1(vs, rv) <= 1e-3
 
 
+def test_adjoint_full_size_cfg4_fd_and_fp64():
+    """The differentiable-rendering workload at its stated size (512 x 512,
+    32 spp, depth 4, dL/dI = 2 (I - T)): the whole image's gradient against
+    the FP64 CPU adjoint, and 64 random parameters (seed 6) against FP64
+    central differences of the frozen estimator (Richardson-extrapolated)."""
+    w = h = 512
+    spp = 32
+    init = scenes.diff_sphere(22, "init")
+    target = scenes.diff_sphere(22, "target")
+    say("[adjoint-full] 512x512 spp=32 depth 4")
+    with HipRenderer(w, h) as r:
+        T = setup(r, target, w, h, spp)
+        M = setup(r, init, w, h, spp)
+        dimg = np.zeros((h, w, 4), dtype=np.float32)
+        dimg[..., :3] = 2.0 * (M[..., :3] - T[..., :3])
+        ga, gv, res = r.adjoint(dimg, spp)
+        say(f"[adjoint-full] GPU adjoint {res.kernel_ms:.2f} ms of kernels")
+    sc = O.Scene(init)
+    kw = dict(max_depth=4, rr_start=3, integrator=1, rng_mode=1, seed=1)
+    ra, rv = sc.adjoint(w, h, spp, dimg, precision=64, **kw)
+    ea, ev = rel_l1(ga, ra), rel_l1(gv, rv)
+    say(f"[adjoint-full] whole image vs FP64 adjoint: rel-L1 albedo {ea:.2e} vertices {ev:.2e}")
+    assert ea <= 1e-3 and ev <= 1e-3
+    tape = sc.tape(w, h, spp, dimg, **kw)
+    nsph_t, nsph_v = 9680, 4842
+    rng = np.random.Generator(np.random.PCG64(6))
+    live_t = np.nonzero(np.abs(ra[:nsph_t]).sum(axis=1) > 0)[0]
+    live_v = np.nonzero(np.abs(rv[:nsph_v]).sum(axis=1) > 0)[0]
+    params = [("albedo", int(t), int(rng.integers(3))) for t in rng.choice(live_t, 32, replace=False)]
+    params += [("vertex", int(v), int(rng.integers(3))) for v in rng.choice(live_v, 32, replace=False)]
+    gpu = np.array([ga[i, c] if k == "albedo" else gv[i, c] for k, i, c in params])
+    fd = np.array([tape.fd(k, i, c, eps=1e-3, richardson=True) for k, i, c in params])
+    tape.close()
+    floor = 1e-2 * np.abs(fd).max()
+    err = np.abs(gpu - fd) / np.maximum(np.abs(fd), floor)
+    worst = int(np.argmax(err))
+    say(f"[adjoint-full] 64 params vs FD: max rel err {err.max():.2e} (param {params[worst]}, gpu {gpu[worst]:.6g}, "
+        f"fd {fd[worst]:.6g}); median {np.median(err):.2e}")
+    assert err.max() <= 1e-3
+
+
+def test_adjoint_deep_paths_use_the_largest_lds_layout():
+    """Depth 12 (the adjoint kernel's 16-segment LDS layout, with the
+    visibility flag packed into the triangle word) against the FP64 oracle."""
+    mesh = scenes.diff_sphere(frequency=8, stage="init")
+    w, h, spp = 48, 40, 4
+    say(f"[adjoint-deep] depth 12 rr 2 {w}x{h} spp={spp}")
+    dimg = weights(w, h, 31)
+    with HipRenderer(w, h) as r:
+        setup(r, mesh, w, h, spp, depth=12, rr_start=2, integrator=1, seed=3)
+        ga, gv, _ = r.adjoint(dimg, spp)
+    sc = O.Scene(mesh)
+    ra, rv = sc.adjoint(w, h, spp, dimg, max_depth=12, rr_start=2, integrator=1, rng_mode=1, seed=3, precision=64)
+    ea, ev = rel_l1(ga, ra), rel_l1(gv, rv)
+    say(f"[adjoint-deep] rel-L1 albedo {ea:.2e} vertices {ev:.2e}")
+    assert np.abs(ra).max() > 0 and np.abs(rv).max() > 0
+    assert ea <= 1e-3 and ev <= 1e-3
+
+
 def test_adjoint_seed_and_sample_range():
     """Samples 4..7 replayed with an explicit seed equal the gradient of an
     oracle render of those samples with that seed."""
