@@ -239,6 +239,33 @@ def bench_batch(args):
             img = r.readback_batch(i)["beauty"]
             ref = plain_imgs[i]
             err = max(err, float(np.abs(img - ref).sum() / max(np.abs(ref).sum(), 1e-30)))
+    # the single-launch form of the batch: one persistent grid per captured
+    # pass runs every path of every scene to its end (Q_ENGINE_PERSISTENT)
+    single_s = 0.0
+    single_rays = 0
+    single_err = 0.0
+    single_best = None
+    if args.batch_single_launch:
+        with HipRenderer(64, 64, device=0) as r:
+            sl = dict(common, engine=abi.Q_ENGINE_PERSISTENT, block_size=256, queue_chunk=1)
+            r.set_params(spp_per_render=1, seed=0, samples_per_pass=args.batch_samples_per_pass, flags=0, **sl)
+            r.set_batch(batch)
+            tune = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=probe, flags=abi.Q_AUTOTUNE_NO_STORE)
+            single_best = tune.entries[tune.best]
+            r.reset_stats()
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.render_batch(spp)
+            r.synchronize()
+            single_s = time.perf_counter() - t0
+            single_rays = rays_of(r.stats())
+            for i in range(len(batch)):
+                img = r.readback_batch(i)["beauty"]
+                ref = plain_imgs[i]
+                single_err = max(single_err, float(np.abs(img - ref).sum() / max(np.abs(ref).sum(), 1e-30)))
+        log(f"batch: single-launch {single_s:.3f} s, {single_rays / single_s / 1e6:.1f} Mrays/s "
+            f"(width {single_best.bvh_width} block {single_best.block_size}, "
+            f"{single_best.samples_per_pass} samples per pass), max rel-L1 vs plain {single_err:.2e}")
     plain = plain_rays / plain_s / 1e6
     batched = batched_rays / batched_s / 1e6
     log(f"batch: batched {batched_s:.3f} s, {batched:.1f} Mrays/s, speedup {batched / plain:.3f}, "
@@ -250,7 +277,13 @@ def bench_batch(args):
             "plain_rays": int(plain_rays), "batched_rays": int(batched_rays),
             "samples_per_pass": args.batch_samples_per_pass,
             "autotuned": {"bvh_width": best.bvh_width, "block_size": best.block_size,
-                          "queue_chunk": best.queue_chunk},
+                          "queue_chunk": best.queue_chunk, "samples_per_pass": best.samples_per_pass},
+            "single_launch_mrays": round(single_rays / single_s / 1e6, 3) if single_s else None,
+            "single_launch_speedup": round(single_rays / single_s / 1e6 / plain, 4) if single_s else None,
+            "single_launch_max_rel_l1_vs_plain": single_err if single_s else None,
+            "single_launch_autotuned": ({"bvh_width": single_best.bvh_width, "block_size": single_best.block_size,
+                                         "samples_per_pass": single_best.samples_per_pass}
+                                        if single_best is not None else None),
             "max_rel_l1_vs_plain": err,
             "plain_progressive_mrays": round(prog_rays / prog_s / 1e6, 3) if prog_s else None,
             "speedup_vs_progressive": round(batched / (prog_rays / prog_s / 1e6), 4) if prog_s else None,
@@ -352,6 +385,10 @@ def main():
                     help="also time the plain non-persistent wavefront tracer per scene (1/0)")
     ap.add_argument("--batch-progressive", type=int, default=1,
                     help="also time the plain engine at one sample per render call (the reference host's usage)")
+    ap.add_argument("--batch-single-launch", type=int, default=1,
+                    help="also time the batch in the single-launch form (one persistent grid per pass) (1/0)")
+    ap.add_argument("--single-launch", type=int, default=1,
+                    help="also time the headline frame with the single-launch engine (1/0)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -406,6 +443,36 @@ def main():
         st = r.stats()
         log(f"timed {args.steps} steps in {elapsed:.3f} s")
 
+    single_launch = None
+    if args.single_launch and args.engine == "wavefront":
+        with HipRenderer(W, H, device=0) as r:
+            r.set_scene(mesh)
+            c = mesh.camera
+            r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+            r.set_params(spp_per_render=args.spp_per_launch, max_depth=args.depth, rr_start=3, rng_mode=1,
+                         integrator=1, seed=0, flags=abi.Q_FLAG_TIME_KERNELS, engine=abi.Q_ENGINE_PERSISTENT,
+                         samples_per_pass=args.samples_per_pass, bvh_width=args.bvh_width,
+                         block_size=args.block if args.block in (128, 256) else 256, queue_chunk=1)
+
+            def step_sl():
+                for i in range(launches_per_step):
+                    r.plugin.render(r.frame(camera_dirty=(i == 0)))
+
+            for _ in range(args.warmup):
+                step_sl()
+                r.synchronize()
+            r.reset_stats()
+            r.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step_sl()
+            r.synchronize()
+            sl_s = time.perf_counter() - t0
+            ss = r.stats()
+            sl_rays = ss.rays_camera + ss.rays_extension + ss.rays_shadow
+            single_launch = {"mrays": round(sl_rays / sl_s / 1e6, 3), "ms_per_step": round(sl_s / args.steps * 1e3, 3),
+                             "rays": int(sl_rays), "vs_wavefront": None}
+        log(f"single-launch engine: {single_launch['mrays']:.1f} Mrays/s, {single_launch['ms_per_step']:.1f} ms/step")
     rays = {"camera": st.rays_camera, "extension": st.rays_extension, "shadow": st.rays_shadow}
     total_rays = sum(rays.values())
     value = total_rays / elapsed / 1e6
@@ -446,6 +513,9 @@ def main():
         "roofline": roofline,
         "kernel_ms": {n: round(v.total_ms, 3) for n, v in kern.items() if v.launches},
     }
+    if single_launch:
+        single_launch["vs_wavefront"] = round(single_launch["mrays"] / value, 4)
+    out["single_launch"] = single_launch
     out["batch"] = None if args.no_batch else bench_batch(args)
     out["adjoint"] = None if args.no_adjoint else bench_adjoint(args)
     if not args.no_cpu:

@@ -55,6 +55,10 @@ extern "C" {
 /* Execution engine for the sample passes. */
 #define Q_ENGINE_MEGAKERNEL 0u  /* one thread per (pixel, sample) path, whole path in one kernel */
 #define Q_ENGINE_WAVEFRONT  1u  /* raygen / extend / shade / shadow kernels over compacted queues */
+#define Q_ENGINE_PERSISTENT 2u  /* single launch per pass: one persistent grid runs every path from its camera
+                                   ray to its end (closest hits, shading, shadow rays), lanes refilled from a
+                                   shared path queue; then the per-pixel accumulation.  Same images as
+                                   Q_ENGINE_WAVEFRONT bit for bit; block_size 128 or 256 */
 
 /* Q_render_params.flags */
 #define Q_FLAG_RECORD_IDS    1u  /* record primary triangle ids of sample id_sample */
@@ -93,8 +97,10 @@ typedef struct Q_render_params {
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
     uint32_t queue_chunk;      /* wavefront: 1 (0 = 1) = each idle lane takes its next ray from the
-                                  queue (persistent waves, refilled when 40 lanes are idle); 64 or
+                                  queue (persistent waves, refilled when 32 lanes are idle); 64 or
                                   256 = each persistent wave takes that many rays at a time;
+                                  Q_QUEUE_LDS_TOP = as 1, with the top three BVH4 levels staged in
+                                  LDS (single scenes, width 4, block 128 or 256);
                                   Q_QUEUE_FLAT = no persistence: one thread per queue slot, the
                                   plain wavefront tracer (block 128 only) */
     uint32_t bvh_builder;      /* Q_BVH_BUILD_*: where Q_plugin_set_scene builds the BVH */
@@ -106,6 +112,7 @@ typedef struct Q_render_params {
  * depend on the tree) */
 /* Q_render_params.queue_chunk: the plain (non-persistent) wavefront traversal */
 #define Q_QUEUE_FLAT 0xFFFFFFFFu
+#define Q_QUEUE_LDS_TOP 2u
 
 #define Q_BVH_BUILD_AUTO 0u   /* on the GPU from 2^14 triangles, else on the host */
 #define Q_BVH_BUILD_HOST 1u   /* host threads */
@@ -212,13 +219,17 @@ int32_t Q_plugin_readback_batch_ids(Q_plugin_handle* handle, uint32_t scene, uin
  * Times every compiled traversal shape (BVH width x block size x queue chunk)
  * on probe_spp samples of the current single scene (Q_AUTOTUNE_SINGLE, with
  * the last camera given to Q_plugin_render) or of the current batch
- * (Q_AUTOTUNE_BATCH), keeps the fastest in the render parameters and clears
- * the accumulation.  The probes' rays, launches and kernel times are not
- * counted: Q_plugin_get_stats reads the same before and after.  The single
- * target needs a frame rendered first (Q_EXT_ERR_STATE otherwise).  Results
- * are cached, one JSON object per line, in
- * $QUASI_HIP_TUNE_CACHE (default ~/.cache/quasi_hip/autotune.jsonl), keyed by
- * GPU architecture, target and workload signature. */
+ * (Q_AUTOTUNE_BATCH), then, with the fastest shape, the pass sizes (samples
+ * per pass, i.e. the ray-queue lengths: powers of two up to about 2^27 paths
+ * and probe_spp, the six largest; a batch's divide probe_spp) when the probe
+ * leaves more than one; keeps the fastest in the render parameters (bvh_width,
+ * block_size, queue_chunk, samples_per_pass) and clears the accumulation.  The probes' rays, launches and kernel times are not
+ * counted: Q_plugin_get_stats reads the same before and after.  Timing the
+ * single target needs a frame rendered first (Q_EXT_ERR_STATE otherwise; a
+ * choice served from the cache does not).  Results are cached, one JSON
+ * object per line, in $QUASI_HIP_TUNE_CACHE (default
+ * ~/.cache/quasi_hip/autotune.jsonl), keyed by GPU architecture, target,
+ * workload signature and a fingerprint of the scene content. */
 #define Q_AUTOTUNE_SINGLE 0u
 #define Q_AUTOTUNE_BATCH  1u
 #define Q_AUTOTUNE_USE_CACHE 1u   /* flags: take a cached choice without timing */
@@ -229,7 +240,7 @@ typedef struct Q_autotune_entry {
     uint32_t bvh_width;
     uint32_t block_size;
     uint32_t queue_chunk;
-    uint32_t reserved;
+    uint32_t samples_per_pass; /* pass size of the probe (samples per pixel in flight: the queue length) */
     double ms;                 /* hipEvent time of the probe */
 } Q_autotune_entry;
 
@@ -266,10 +277,30 @@ int32_t Q_plugin_autotune(Q_plugin_handle* handle, uint32_t target, uint32_t pro
  *   dL/dI = 2 (M - T); L is returned in result->loss.
  * Gradients are written in FP64, in input order: grad_albedo 3 per
  * triangle, grad_vertices 3 per vertex (either may be NULL).
+ *
+ * Albedo parameter map: with albedo_param, triangle t's albedo is parameter
+ * albedo_param[t] (< num_albedo_params; 0xFFFFFFFF: no parameter), shared by
+ * every triangle mapped to it, and grad_albedo holds 3 * num_albedo_params
+ * gradients: each the sum of its triangles' gradients, reduced on the device
+ * in FP64.
+ *
+ * Passes and memory: each pass traces and records at most max_pass_paths
+ * paths (0: Q_ADJOINT_PASS_PATHS): samples of the whole image, or, when one
+ * sample of the image holds more pixels, one sample of a band of full rows
+ * (per-tile path recomputation: every band is traced, recorded and
+ * replayed on its own).  The working memory beyond the scene is then
+ * max_pass_paths * (9 * max_depth + 188) bytes of path records and path state,
+ * 4 * 8 * 3 * (num_triangles + num_vertices) bytes of FP32 gradient
+ * replicas, 8 * 3 * (num_triangles + num_vertices + num_albedo_params) bytes
+ * of FP64 gradients, 3 * 16 * width * height bytes of images and 16 bytes per
+ * triangle of vertex indices and map (result->working_bytes reports it for
+ * the call; buffers kept from earlier, larger calls are not released).
  */
 #define Q_ADJOINT_IMAGE_GRADIENT 0u
 #define Q_ADJOINT_L2_TARGET      1u
 #define Q_ADJOINT_SEED           1u   /* flags: use desc->seed instead of the render seed */
+#define Q_ADJOINT_PASS_PATHS     (1u << 24)
+#define Q_ADJOINT_NO_PARAM       0xFFFFFFFFu
 
 typedef struct Q_adjoint_desc {
     uint32_t struct_size;
@@ -279,8 +310,12 @@ typedef struct Q_adjoint_desc {
     uint32_t seed;             /* with Q_ADJOINT_SEED */
     uint32_t flags;
     const float* image;        /* width * height * 4 floats (host memory) */
-    double* grad_albedo;       /* 3 * num_triangles, or NULL */
+    double* grad_albedo;       /* 3 * num_triangles (3 * num_albedo_params with a map), or NULL */
     double* grad_vertices;     /* 3 * num_vertices, or NULL */
+    /* fields below are 0 for callers whose struct_size ends before them */
+    uint32_t max_pass_paths;   /* paths per recorded pass (0: Q_ADJOINT_PASS_PATHS) */
+    uint32_t num_albedo_params;
+    const uint32_t* albedo_param;   /* num_triangles entries (host memory), or NULL: per-triangle albedo */
 } Q_adjoint_desc;
 
 typedef struct Q_adjoint_result {
@@ -289,9 +324,14 @@ typedef struct Q_adjoint_result {
     uint64_t paths;            /* paths replayed */
     double loss;               /* Q_ADJOINT_L2_TARGET: L; otherwise 0 */
     double kernel_ms;          /* hipEvent time of the adjoint and reduction kernels */
+    /* fields below are written only when struct_size covers them */
+    uint64_t working_bytes;    /* device memory the call needs beyond the scene and framebuffer */
+    uint32_t passes;           /* recorded passes (samples x row bands) */
+    uint32_t bands;            /* row bands per sample block (1: whole image) */
 } Q_adjoint_result;
 
 int32_t Q_plugin_adjoint(Q_plugin_handle* handle, const Q_adjoint_desc* desc, Q_adjoint_result* result);
+
 
 #ifdef __cplusplus
 }  /* extern "C" */

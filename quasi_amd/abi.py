@@ -24,10 +24,12 @@ Q_EXT_ERR_STATE = -2
 Q_EXT_ERR_DEVICE = -3
 Q_EXT_ERR_OUT_OF_MEMORY = -4
 Q_EXT_ERR_UNSUPPORTED = -5
+Q_ADJOINT_PASS_PATHS = 1 << 24
+Q_ADJOINT_NO_PARAM = 0xFFFFFFFF
 
 Q_RNG_REFERENCE, Q_RNG_HASHED = 0, 1
 Q_INTEGRATOR_REFERENCE, Q_INTEGRATOR_NEE = 0, 1
-Q_ENGINE_MEGAKERNEL, Q_ENGINE_WAVEFRONT = 0, 1
+Q_ENGINE_MEGAKERNEL, Q_ENGINE_WAVEFRONT, Q_ENGINE_PERSISTENT = 0, 1, 2
 Q_FLAG_RECORD_IDS = 1
 Q_FLAG_TIME_KERNELS = 2
 Q_STATS_MAX_KERNELS = 16
@@ -92,6 +94,7 @@ class Q_render_params(C.Structure):
 
 Q_BVH_BUILD_AUTO, Q_BVH_BUILD_HOST, Q_BVH_BUILD_GPU = 0, 1, 2
 Q_QUEUE_FLAT = 0xFFFFFFFF   # queue_chunk: the plain (non-persistent) wavefront traversal
+Q_QUEUE_LDS_TOP = 2         # queue_chunk: per-lane fetch, top BVH4 levels staged in LDS
 
 
 class Q_kernel_stat(C.Structure):
@@ -124,7 +127,7 @@ Q_AUTOTUNE_MAX_ENTRIES = 32
 
 class Q_autotune_entry(C.Structure):
     _fields_ = [("bvh_width", C.c_uint32), ("block_size", C.c_uint32), ("queue_chunk", C.c_uint32),
-                ("reserved", C.c_uint32), ("ms", C.c_double)]
+                ("samples_per_pass", C.c_uint32), ("ms", C.c_double)]
 
 
 class Q_autotune_result(C.Structure):
@@ -140,12 +143,14 @@ class Q_adjoint_desc(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("mode", C.c_uint32), ("spp", C.c_uint32),
                 ("sample_begin", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32),
                 ("image", C.POINTER(C.c_float)), ("grad_albedo", C.POINTER(C.c_double)),
-                ("grad_vertices", C.POINTER(C.c_double))]
+                ("grad_vertices", C.POINTER(C.c_double)), ("max_pass_paths", C.c_uint32),
+                ("num_albedo_params", C.c_uint32), ("albedo_param", C.POINTER(C.c_uint32))]
 
 
 class Q_adjoint_result(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("reserved", C.c_uint32), ("paths", C.c_uint64),
-                ("loss", C.c_double), ("kernel_ms", C.c_double)]
+                ("loss", C.c_double), ("kernel_ms", C.c_double), ("working_bytes", C.c_uint64),
+                ("passes", C.c_uint32), ("bands", C.c_uint32)]
 
 
 # Symbol names, plugin_interface.hpp:170-179

@@ -35,6 +35,7 @@ HOSTONLY = ["-x", "c++", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__", "-I/op
 PLUGIN_SOURCES = [
     ("kernels_mega.hip", DEVICE),
     ("kernels_wave.hip", DEVICE),
+    ("kernels_paths.hip", DEVICE),
     ("kernels_util.hip", DEVICE),
     ("kernels_adjoint.hip", DEVICE),
     ("bvh_gpu.hip", DEVICE),

@@ -106,6 +106,10 @@ struct WaveState {
 // Scene i owns paths [path_begin, path_begin + width*height*samples) of the
 // pass and pixels [pix_begin, pix_begin + width*height) of the framebuffer
 // arrays; path p_local = s*width*height + y*width + x.
+// A pass may cover a band of rows [row_begin, row_begin + rows) of the image
+// only (rows = 0: every row): its paths are then p_local = s*width*rows +
+// (y - row_begin)*width + x, while the camera and the RNG streams stay those of
+// the whole width x height image.
 struct SceneDesc {
     const void* tris;
     const void* mats;
@@ -118,6 +122,7 @@ struct SceneDesc {
     uint32_t path_begin, pix_begin;
     uint32_t seed;
     uint32_t num_tris;
+    uint32_t row_begin, rows;
 };
 
 // Everything one sample pass needs (passed by value as a kernel argument).
@@ -175,6 +180,7 @@ struct AdjointParams {
     float* grad_vertices;         // replicas x 3 x num_vertices
     uint32_t num_tris, num_vertices;
     uint32_t rec_paths;           // paths of the recorded pass (pass.rec_hit / rec_vis stride)
+    uint32_t row_begin, rows;     // the band of image rows replayed (the recorded pass's band)
 };
 
 // Launch shape of the persistent traversal kernels (autotuned).
@@ -185,6 +191,7 @@ struct TraceConfig {
                       // kChunkFlat = no persistence (one thread per queue slot)
 };
 constexpr uint32_t kChunkFlat = 0xFFFFFFFFu;
+constexpr uint32_t kChunkLdsTop = 2;  // per-lane fetch, top BVH4 levels staged in LDS (single scenes, width 4)
 constexpr int kFlatStack = 48;        // LDS stack entries per thread of the non-persistent traversal
 
 constexpr int kShortStack = 16;       // LDS entries per lane in the persistent kernels

@@ -51,6 +51,21 @@ uint32_t persistent_blocks(int device, const TraceConfig& c);
 hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const TraceConfig& cfg,
                                  uint32_t trace_blocks, hipStream_t stream, WaveTimers* timers);
 
+// The end of a pass: per pixel, the pass's per-path radiance and first-hit
+// records summed in sample order into the accumulation (and, for a captured
+// pass, the device-side sample index moved on).
+hipError_t launch_pass_accumulate(const PassParams& p, const WaveState& s, hipStream_t stream);
+
+// The single-launch form of a pass (Q_ENGINE_PERSISTENT, and batches rendered
+// with it): one persistent grid traces every path of the pass from its camera
+// ray to its end -- closest hit, shading, the shadow ray, the next segment --
+// then launch_pass_accumulate.  Same results as launch_wavefront_pass, bit for
+// bit.  cfg.chunk is not used; `blocks` = paths_pass_blocks(device, cfg).
+bool paths_config_supported(const TraceConfig& c);
+uint32_t paths_pass_blocks(int device, const TraceConfig& c);
+hipError_t launch_paths_pass(const PassParams& p, const WaveState& s, const TraceConfig& cfg, uint32_t blocks,
+                             hipStream_t stream);
+
 // out[i] = sum[i] / n for npix RGBA pixels (the running mean of plugin.mm:326-327).
 hipError_t launch_resolve(const float4* sum, float4* out, size_t npix, uint32_t n, hipStream_t stream);
 // 8-bit display image (RGBA, 4 bytes per pixel) of the mean of n samples (plugin.mm:334-350).
@@ -63,6 +78,11 @@ hipError_t launch_tonemap(const float4* sum, void* out, size_t npix, uint32_t n,
 uint32_t adjoint_block_size();
 uint32_t adjoint_max_depth();
 hipError_t launch_adjoint(const AdjointParams& p, uint32_t blocks, hipStream_t stream);
+
+// out[param[t]] += per_tri[t] (3 doubles per triangle and parameter) for the
+// nt triangles whose param[t] < np; out must be cleared first.
+hipError_t launch_param_scatter(const double* per_tri, const uint32_t* param, size_t nt, uint32_t np, double* out,
+                               hipStream_t stream);
 
 // out[i] = sum over r < replicas of (double) rep[r * n + i].
 hipError_t launch_replica_reduce(const float* rep, size_t n, uint32_t replicas, double* out, hipStream_t stream);

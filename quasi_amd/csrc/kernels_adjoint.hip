@@ -137,8 +137,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
     const uint8_t* __restrict__ rec_vis = P.rec_vis;
     const float4* __restrict__ dimg = static_cast<const float4*>(A.dimg);
     const uint32_t W = P.width, H = P.height;
+    // the band of rows [row_begin, row_end) that the recorded pass traced
+    const uint32_t row0 = A.row_begin, rows = A.rows ? A.rows : H, row_end = row0 + rows;
     const uint32_t tiles_x = (W + 15u) / 16u;
-    const uint32_t tiles = tiles_x * ((H + 7u) / 8u);
+    const uint32_t tiles = tiles_x * ((rows + 7u) / 8u);
     const uint32_t rep = blockIdx.x % A.replicas;
     float* ga = A.grad_albedo + size_t(rep) * 3 * A.num_tris;
     float* gv = A.grad_vertices + size_t(rep) * 3 * A.num_vertices;
@@ -147,9 +149,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
 
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t x = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
-        const uint32_t y = (tile / tiles_x) * 8u + (threadIdx.x >> 4);
+        const uint32_t y = row0 + (tile / tiles_x) * 8u + (threadIdx.x >> 4);
         f3 delta = mk(0.f, 0.f, 0.f);
-        if (x < W && y < H) {
+        if (x < W && y < row_end) {
             const float4 g = dimg[size_t(y) * W + x];
             delta = mk(g.x, g.y, g.z) * A.scale;
         }
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
                 Ray ray = camera_ray(P.cam, W, H, x, y, rng);
                 f3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
                 // the path's id in the recorded pass (sample-major, as the wavefront engine numbers paths)
-                const size_t pid = (size_t(s) * H + y) * W + x;
+                const size_t pid = (size_t(s) * rows + (y - row0)) * W + x;
                 for (uint32_t b = 0; b < P.max_depth; ++b) {
                     // the recorded closest hit, distance and triangle as traced
                     const uint2 rh = rec_hit[size_t(b) * A.rec_paths + pid];
@@ -312,6 +314,22 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
 #undef REC
 }
 
+// out[param[t]] += per_tri[t] for every triangle t with a parameter (3
+// channels), in FP64: the reduction of per-triangle gradients onto a shared
+// parameter map.
+__global__ void k_param_scatter(const double* __restrict__ per_tri, const uint32_t* __restrict__ param, size_t nt,
+                                uint32_t np, double* __restrict__ out) {
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < nt; t += size_t(gridDim.x) * blockDim.x) {
+        const uint32_t k = param[t];
+        if (k >= np) continue;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double v = per_tri[3 * t + c];
+            if (v != 0.0) atomicAdd(&out[3 * size_t(k) + c], v);
+        }
+    }
+}
+
 __global__ void k_replica_reduce(const float* __restrict__ rep, size_t n, uint32_t replicas, double* __restrict__ out) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
         double s = 0.0;
@@ -348,9 +366,10 @@ uint32_t adjoint_block_size() { return kAdjBlock; }
 uint32_t adjoint_max_depth() { return kMaxDepth; }
 
 hipError_t launch_adjoint(const AdjointParams& p, uint32_t blocks, hipStream_t stream) {
+    const uint32_t rows = p.rows ? p.rows : p.pass.height;
     if (blocks == 0 || p.pass.max_depth == 0 || p.pass.max_depth > kMaxDepth || p.replicas == 0 ||
-        !p.pass.rec_hit || !p.pass.rec_vis ||
-        uint64_t(p.pass.sample_count) * p.pass.width * p.pass.height > p.rec_paths)
+        !p.pass.rec_hit || !p.pass.rec_vis || uint64_t(p.row_begin) + rows > p.pass.height ||
+        uint64_t(p.pass.sample_count) * p.pass.width * rows > p.rec_paths)
         return hipErrorInvalidValue;
     if (p.pass.max_depth <= 4)
         hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(kAdjBlock), 0, stream, p);
@@ -358,6 +377,15 @@ hipError_t launch_adjoint(const AdjointParams& p, uint32_t blocks, hipStream_t s
         hipLaunchKernelGGL(k_adjoint<8>, dim3(blocks), dim3(kAdjBlock), 0, stream, p);
     else
         hipLaunchKernelGGL(k_adjoint<16>, dim3(blocks), dim3(kAdjBlock), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_param_scatter(const double* per_tri, const uint32_t* param, size_t nt, uint32_t np, double* out,
+                               hipStream_t stream) {
+    if (nt == 0 || np == 0) return hipSuccess;
+    const size_t want = (nt + 255) / 256;
+    const uint32_t blocks = uint32_t(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(k_param_scatter, dim3(blocks), dim3(256), 0, stream, per_tri, param, nt, np, out);
     return hipGetLastError();
 }
 

@@ -17,43 +17,13 @@
 // BATCH = true: several scenes of different sizes share the queues of one
 // pass; each ray carries its scene index and every kernel reads the scene's
 // geometry, camera and framebuffer offsets from the scene table.
-#include "device.hpp"
 #include "kernels.hpp"
+#include "wave_common.hpp"
 
 namespace quasi_hip {
 
 namespace {
 
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-template <bool BATCH>
-__device__ __forceinline__ const SceneDesc& scene_at(const PassParams& P, uint32_t sid) {
-    if (BATCH) return P.scenes[sid];
-    return P.scene0;
-}
-
-// Scene owning path id p (BATCH): the last scene whose path_begin <= p.
-__device__ __forceinline__ uint32_t scene_of_path(const PassParams& P, uint32_t p) {
-    uint32_t lo = 0, hi = P.num_scenes;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (P.scenes[mid].path_begin <= p) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t scene_of_pixel(const PassParams& P, uint32_t pix) {
-    uint32_t lo = 0, hi = P.num_scenes;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (P.scenes[mid].pix_begin <= pix) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t pass_sample_begin(const PassParams& P) {
-    return P.dyn_sample_begin ? *P.dyn_sample_begin : P.sample_begin;
-}
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
@@ -79,12 +49,14 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < S.paths; p += gridDim.x * blockDim.x) {
         const uint32_t sid = BATCH ? scene_of_path(P, p) : 0u;
         const SceneDesc& sc = scene_at<BATCH>(P, sid);
-        const uint32_t npix = sc.width * sc.height;
+        const uint32_t rows = sc.rows ? sc.rows : sc.height;
+        const uint32_t npix = sc.width * rows;
         const uint32_t pl = p - sc.path_begin;
         const uint32_t s = pl / npix;
         const uint32_t pix = pl - s * npix;
-        const uint32_t y = pix / sc.width;
-        const uint32_t x = pix - y * sc.width;
+        const uint32_t yl = pix / sc.width;
+        const uint32_t x = pix - yl * sc.width;
+        const uint32_t y = sc.row_begin + yl;
         uint32_t rng = path_seed(P.rng_mode, x, y, sc.width, sc.height, s0 + s, sc.seed);
         const Ray r = camera_ray(sc.cam, sc.width, sc.height, x, y, rng);
         ro[p] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p));
@@ -93,23 +65,7 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
     }
 }
 
-__device__ __forceinline__ uint32_t wave_global() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
 
-// Adds the length of a segmented queue to a ray counter (first wave of the
-// grid; one segment per lane).
-static_assert(kQueueSegments == 64, "one segment per lane of a wave");
-__device__ __forceinline__ void count_rays(const uint32_t* lengths, unsigned long long* counter) {
-    if (blockIdx.x == 0 && threadIdx.x < 64u) {
-        const uint32_t total = wave_sum(lengths[threadIdx.x * kCounterStride]);
-        if (threadIdx.x == 0u && total) atomicAdd(counter, (unsigned long long)total);
-    }
-}
-
-template <int WIDTH, bool BATCH>
-__device__ __forceinline__ const float4* scene_nodes(const PassParams& P, uint32_t sid) {
-    const SceneDesc& sc = scene_at<BATCH>(P, sid);
-    return static_cast<const float4*>(WIDTH == 4 ? sc.nodes4 : sc.nodes);
-}
 
 // Persistent closest-hit traversal: each wave reserves CHUNK rays of the
 // bounce's queue at a time until the queue is drained.
@@ -232,12 +188,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
     }
 }
 
-__device__ __forceinline__ void add_shadow(float4* rad, const float4* sc_a, uint32_t k, uint32_t p) {
-    const float4 c = sc_a[k];
-    float4 l = rad[p];
-    l.x += c.x; l.y += c.y; l.z += c.z;
-    rad[p] = l;
-}
 
 // Persistent any-hit traversal of the bounce's shadow queue; an unoccluded
 // ray adds its contribution to the path's radiance slot (one shadow ray per
@@ -290,13 +240,61 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // uniform over the grid, plus a 32-bit byte offset (in a batch: the lane's
 // scene's offset plus the record's), which the loads take as they are;
 // otherwise each lane holds 64-bit pointers to its scene's arrays.
-constexpr uint32_t kRefillIdle = 32;   // measured best of 8..48 on the 1M soup (28-32 alike)
-constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
-template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32>
-__global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
+// TOP (single-scene BVH4 kernels): the top three levels of the tree (the
+// root, its internal children and theirs: at most 21 nodes, 2.7 KB) are
+// copied into LDS when the block starts, their child references to staged
+// nodes rewritten as kLdsRef + slot, and node steps on them read LDS instead
+// of global memory.  The kernel is held to the register budget of a full CU
+// (kMaxThreadsPerCU threads) so that the extra LDS path costs no occupancy.
+constexpr int kTopNodes = 21;
+constexpr int kLdsRef = 0x40000000;   // node references >= this are LDS slots (fewer than 2^30 nodes)
+
+template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32, bool TOP = false>
+__global__ __launch_bounds__(BLOCK, (TOP && WIDTH == 4 && !BATCH) ? kMaxThreadsPerCU / BLOCK : 1)
+void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
     constexpr uint32_t kNodeShift = WIDTH == 4 ? 7u : 6u;   // 128-B BVH4 / 64-B BVH2 nodes
+    constexpr bool kTop = TOP && WIDTH == 4 && !BATCH;
     __shared__ int s_stack[kShortStack * BLOCK];
+    __shared__ float4 s_top[kTop ? kTopNodes * 8 : 1];
+    __shared__ int s_top_map[kTop ? kTopNodes : 1];
+    __shared__ int s_top_n;
+    if (kTop) {
+        const float4* gn = static_cast<const float4*>(P.nodes4);
+        if (threadIdx.x == 0) {
+            // breadth first over three levels: slot -> global node index
+            int n = 0;
+            if (P.scene0.num_nodes != 0u) {
+                s_top_map[n++] = 0;
+                int lo = 0, hi = 1;
+                for (int level = 1; level < 3; ++level) {
+                    for (int k = lo; k < hi; ++k) {
+                        const float4 chf = gn[size_t(s_top_map[k]) * 8 + 6];
+                        const int c[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                                          __float_as_int(chf.w)};
+                        for (int j = 0; j < 4; ++j)
+                            if (c[j] >= 0 && c[j] != kEmptyChild && n < kTopNodes) s_top_map[n++] = c[j];
+                    }
+                    lo = hi;
+                    hi = n;
+                }
+            }
+            s_top_n = n;
+        }
+        __syncthreads();
+        const int n = s_top_n;
+        for (int i = int(threadIdx.x); i < n * 8; i += BLOCK) s_top[i] = gn[size_t(s_top_map[i >> 3]) * 8 + (i & 7)];
+        __syncthreads();
+        // child references to staged nodes point at their slots
+        for (int i = int(threadIdx.x); i < n * 4; i += BLOCK) {
+            int* ref = reinterpret_cast<int*>(&s_top[(i >> 2) * 8 + 6]) + (i & 3);
+            const int c = *ref;
+            if (c < 0 || c == kEmptyChild) continue;
+            for (int m = 1; m < n; ++m)
+                if (s_top_map[m] == c) { *ref = kLdsRef + m; break; }
+        }
+        __syncthreads();
+    }
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
                                       gridDim.x * BLOCK, 0};
     const uint32_t* lengths = SHADOW ? S.shadow_counts + queue_counter(b, 0) : S.counts + queue_counter(b, 0);
@@ -367,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 best_tri = -1;
                 best_id = 0u;
                 st.sp = 0;
-                node = sc.num_nodes ? 0 : kDone;
+                node = sc.num_nodes ? (kTop ? kLdsRef : 0) : kDone;
                 leaf = 0;
                 busy = true;
             }
@@ -392,10 +390,19 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             if (WIDTH == 4) {
                 // entry and exit planes chosen by the direction's signs: the same
                 // values slab() takes the min and max of, without the min/max
-                const float4 ex = ld(sel_x), fx = ld(sel_x ^ 16u);
-                const float4 ey = ld(sel_y), fy = ld(sel_y ^ 16u);
-                const float4 ez = ld(sel_z), fz = ld(sel_z ^ 16u);
-                const float4 chf = ld(96u);
+                float4 ex, fx, ey, fy, ez, fz, chf;
+                if (kTop && node >= kLdsRef) {
+                    const float4* q = s_top + (node - kLdsRef) * 8;
+                    ex = q[sel_x >> 4]; fx = q[(sel_x ^ 16u) >> 4];
+                    ey = q[sel_y >> 4]; fy = q[(sel_y ^ 16u) >> 4];
+                    ez = q[sel_z >> 4]; fz = q[(sel_z ^ 16u) >> 4];
+                    chf = q[6];
+                } else {
+                    ex = ld(sel_x); fx = ld(sel_x ^ 16u);
+                    ey = ld(sel_y); fy = ld(sel_y ^ 16u);
+                    ez = ld(sel_z); fz = ld(sel_z ^ 16u);
+                    chf = ld(96u);
+                }
                 const int4 ch = make_int4(__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
                                           __float_as_int(chf.w));
                 float t0n, t0f, t1n, t1f, t2n, t2f, t3n, t3f;
@@ -599,10 +606,13 @@ int occupancy_of(K kernel, int block) {
 }
 
 // CHUNK 64 / 256: static chunks per wave; CHUNK 1: per-lane dynamic fetch;
-// CHUNK 0 (kChunkFlat in a TraceConfig): the non-persistent traversal.
+// CHUNK 2 (kChunkLdsTop): per-lane dynamic fetch with the top BVH4 levels in
+// LDS; CHUNK 0 (kChunkFlat in a TraceConfig): the non-persistent traversal.
 template <int WIDTH, int BLOCK, int CHUNK>
 struct TraceVariant {
-    static constexpr int kStatic = CHUNK <= 1 ? 64 : CHUNK;
+    static constexpr bool kDynamic = CHUNK == 1 || CHUNK == int(kChunkLdsTop);
+    static constexpr bool kTop = CHUNK == int(kChunkLdsTop);
+    static constexpr int kStatic = kDynamic || CHUNK == 0 ? 64 : CHUNK;
     // one thread per queue slot; the stack must hold the deepest walk
     template <bool SHADOW, bool BATCH>
     static hipError_t flat(const PassParams& p, const WaveState& s, uint32_t b, hipStream_t st) {
@@ -619,14 +629,14 @@ struct TraceVariant {
     template <bool SHADOW, bool BATCH>
     static void dynamic(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
         if (off32(p))
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, true, kTop>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         else
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, false, kTop>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
     }
     template <bool BATCH>
     static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        if (CHUNK == 0) return flat<false, BATCH>(p, s, b, st);
-        if (CHUNK == 1)
+        if constexpr (CHUNK == 0) return flat<false, BATCH>(p, s, b, st);
+        if constexpr (kDynamic)
             dynamic<false, BATCH>(p, s, b, blocks, st);
         else
             hipLaunchKernelGGL((k_extend<WIDTH, BLOCK, kStatic, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
@@ -634,8 +644,8 @@ struct TraceVariant {
     }
     template <bool BATCH>
     static hipError_t shadow(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        if (CHUNK == 0) return flat<true, BATCH>(p, s, b, st);
-        if (CHUNK == 1)
+        if constexpr (CHUNK == 0) return flat<true, BATCH>(p, s, b, st);
+        if constexpr (kDynamic)
             dynamic<true, BATCH>(p, s, b, blocks, st);
         else
             hipLaunchKernelGGL((k_shadow<WIDTH, BLOCK, kStatic, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
@@ -646,12 +656,12 @@ struct TraceVariant {
     // the first ones finish, which is harmless: no block waits for another.
     static int occupancy() {
         int a, c;
-        if (CHUNK == 0) {
+        if constexpr (CHUNK == 0) {
             a = occupancy_of(k_trace_flat<WIDTH, BLOCK, false, false>, BLOCK);
             c = occupancy_of(k_trace_flat<WIDTH, BLOCK, true, false>, BLOCK);
-        } else if (CHUNK == 1) {
-            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, false, true>, BLOCK);
-            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, false, true>, BLOCK);
+        } else if constexpr (kDynamic) {
+            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, false, true, kTop>, BLOCK);
+            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, false, true, kTop>, BLOCK);
         } else {
             a = occupancy_of(k_extend<WIDTH, BLOCK, kStatic, false>, BLOCK);
             c = occupancy_of(k_shadow<WIDTH, BLOCK, kStatic, false>, BLOCK);
@@ -678,6 +688,7 @@ const VariantEntry kVariants[] = {
     QH_VARIANT(2, 64, 1),   QH_VARIANT(2, 128, 1),   QH_VARIANT(2, 256, 1),
     QH_VARIANT(4, 64, 1),   QH_VARIANT(4, 128, 1),   QH_VARIANT(4, 256, 1),
     QH_VARIANT(2, 128, 0),  QH_VARIANT(4, 128, 0),
+    QH_VARIANT(4, 128, 2),  QH_VARIANT(4, 256, 2),
 };
 #undef QH_VARIANT
 
@@ -753,10 +764,15 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
     }
     if (p.rec_hit) return hipGetLastError();   // a recorded pass for the adjoint: no accumulation
     before(WAVE_K_ACCUMULATE);
-    const uint32_t acc_blocks = grid_for(p.total_pixels, 256, 16384);
-    if (batch) hipLaunchKernelGGL(k_accumulate<true>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
-    else hipLaunchKernelGGL(k_accumulate<false>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
+    const hipError_t e = launch_pass_accumulate(p, s, stream);
     after(WAVE_K_ACCUMULATE);
+    return e;
+}
+
+hipError_t launch_pass_accumulate(const PassParams& p, const WaveState& s, hipStream_t stream) {
+    const uint32_t acc_blocks = grid_for(p.total_pixels, 256, 16384);
+    if (p.num_scenes > 0) hipLaunchKernelGGL(k_accumulate<true>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
+    else hipLaunchKernelGGL(k_accumulate<false>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
     if (p.dyn_sample_begin)
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, const_cast<uint32_t*>(p.dyn_sample_begin),
                            p.sample_count);

@@ -31,8 +31,8 @@ namespace quasi_hip {
 namespace host {
 
 const char* const kKernelNames[K_COUNT] = {"megakernel", "resolve",  "fill",         "wf_raygen", "wf_extend",
-                                           "wf_shade",   "wf_shadow", "wf_accumulate", "batch_pass"};
-const bool kRenderKernel[K_COUNT] = {true, false, false, true, true, true, true, true, true};
+                                           "wf_shade",   "wf_shadow", "wf_accumulate", "batch_pass", "paths_pass"};
+const bool kRenderKernel[K_COUNT] = {true, false, false, true, true, true, true, true, true, true};
 
 // Scene builds of a batch run on host threads and may log at once: the
 // host's callback is entered by one thread at a time.
@@ -655,7 +655,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
     p.num_nodes4 = st->num_nodes4;
     p.gstack = static_cast<int*>(st->gstack.ptr);
     p.gstack_entries = st->gstack_entries;
-    if (rp.engine == Q_ENGINE_WAVEFRONT) {
+    if (rp.engine == Q_ENGINE_WAVEFRONT || rp.engine == Q_ENGINE_PERSISTENT) {
         SceneDesc& sd = p.scene0;
         sd.tris = p.tris;
         sd.mats = p.mats;
@@ -682,9 +682,14 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
             p.sample_begin = st->sample_count + done;
             p.sample_count = n;
             const WaveState ws = wave_state(st->wave, n * npix);
-            if (!hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, &tm),
-                        "wavefront launch"))
+            if (rp.engine == Q_ENGINE_PERSISTENT) {
+                LaunchTimer lt(st, s, K_PATHS);
+                if (!hip_ok(st, launch_paths_pass(p, ws, st->trace_cfg, st->paths_blocks, s), "single-launch pass"))
+                    return false;
+            } else if (!hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, &tm),
+                               "wavefront launch")) {
                 return false;
+            }
         }
     } else {
         LaunchTimer lt(st, s, K_MEGAKERNEL);
@@ -745,6 +750,7 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
         std::snprintf(st->arch, sizeof st->arch, "%s", have_prop ? prop.gcnArchName : "unknown");
         st->trace_cfg = TraceConfig{4, 256, 1};
         st->trace_blocks = persistent_blocks(dev, st->trace_cfg);
+        st->paths_blocks = paths_pass_blocks(dev, st->trace_cfg);
         if (ctx->gpu->queue) {
             st->stream = static_cast<hipStream_t>(ctx->gpu->queue);
         } else {
@@ -926,7 +932,7 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
         std::memcpy(&p, in, std::min<size_t>(in->struct_size, sizeof p));
         p.struct_size = sizeof(Q_render_params);
         if (p.spp_per_render == 0 || p.spp_per_render > 4096 || p.max_depth == 0 || p.max_depth > kMaxDepth ||
-            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_WAVEFRONT ||
+            p.rng_mode > Q_RNG_HASHED || p.integrator > Q_INTEGRATOR_NEE || p.engine > Q_ENGINE_PERSISTENT ||
             p.bvh_builder > Q_BVH_BUILD_GPU) {
             log_msg(st, "set_params: unsupported parameter combination");
             return Q_EXT_ERR_UNSUPPORTED;
@@ -938,11 +944,17 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
                     cfg.chunk);
             return Q_EXT_ERR_UNSUPPORTED;
         }
+        if (p.engine == Q_ENGINE_PERSISTENT && !paths_config_supported(cfg)) {
+            log_msg(st, "set_params: the single-launch engine has no kernel for width %u, block %u (blocks 128, 256)",
+                    cfg.width, cfg.block);
+            return Q_EXT_ERR_UNSUPPORTED;
+        }
         (void)hipSetDevice(st->device);
         if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
         st->params = p;
         st->trace_cfg = cfg;
         st->trace_blocks = persistent_blocks(st->device, cfg);
+        st->paths_blocks = paths_pass_blocks(st->device, cfg);
         if (!clear_accumulation(st, st->stream)) return Q_EXT_ERR_DEVICE;
         return Q_EXT_OK;
     } catch (...) {

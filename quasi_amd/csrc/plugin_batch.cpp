@@ -127,6 +127,8 @@ bool prepare_batch_pass(plugin_state* st, uint32_t spp_pass) {
     k.add(spp_pass);
     k.add(st->trace_cfg.width); k.add(st->trace_cfg.block); k.add(st->trace_cfg.chunk); k.add(st->trace_blocks);
     const Q_render_params& rp = st->params;
+    const bool single_launch = rp.engine == Q_ENGINE_PERSISTENT;
+    k.add(single_launch); k.add(st->paths_blocks);
     k.add(rp.max_depth); k.add(rp.rr_start); k.add(rp.rng_mode); k.add(rp.integrator);
     k.add(rp.flags & Q_FLAG_RECORD_IDS); k.add(rp.id_sample);
     k.add(reinterpret_cast<uintptr_t>(st->wave.ray_o[0].ptr)); k.add(st->wave.capacity);
@@ -148,7 +150,11 @@ bool prepare_batch_pass(plugin_state* st, uint32_t spp_pass) {
     const WaveState ws = wave_state(st->wave, uint32_t(paths));
     if (!hip_ok(st, hipStreamBeginCapture(st->capture_stream, hipStreamCaptureModeThreadLocal), "begin capture"))
         return false;
-    const hipError_t le = launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, st->capture_stream, nullptr);
+    // the batch renders with the single-launch pass under Q_ENGINE_PERSISTENT,
+    // else with the wavefront launch sequence
+    const hipError_t le =
+        single_launch ? launch_paths_pass(p, ws, st->trace_cfg, st->paths_blocks, st->capture_stream)
+                      : launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, st->capture_stream, nullptr);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st->capture_stream, &g);
     if (!hip_ok(st, le, "captured pass launch") || !hip_ok(st, ce, "end capture")) {
@@ -364,37 +370,73 @@ std::string tune_cache_path() {
     return std::string(home ? home : "/tmp") + "/.cache/quasi_hip/autotune.jsonl";
 }
 
+// A fingerprint of the scene's content: FNV-1a over up to 4096 evenly spaced
+// vertex coordinates and indices of the single scene, or over every batch
+// scene's triangle count, resolution and seed.
+uint64_t scene_fingerprint(const plugin_state* st, uint32_t target) {
+    KeyHash k;
+    if (target == Q_AUTOTUNE_BATCH) {
+        for (const SceneDesc& d : st->batch.descs) {
+            k.add(d.num_tris);
+            k.add(uint64_t(d.width) << 32 | d.height);
+            k.add(d.seed);
+        }
+        return k.h;
+    }
+    auto sample = [&](const auto& v) {
+        const size_t n = v.size(), step = std::max<size_t>(1, n / 4096);
+        k.add(n);
+        for (size_t i = 0; i < n; i += step) {
+            uint32_t bits;
+            std::memcpy(&bits, &v[i], 4);
+            k.add(bits);
+        }
+    };
+    sample(st->verts);
+    sample(st->indices);
+    return k.h;
+}
+
 std::string tune_key(const plugin_state* st, uint32_t target) {
-    char buf[256];
+    char buf[320];
+    const unsigned long long fp = (unsigned long long)scene_fingerprint(st, target);
     if (target == Q_AUTOTUNE_BATCH)
-        std::snprintf(buf, sizeof buf, "%s|batch|%u|%u|%u|d%u|i%u", st->arch, st->batch.num_scenes,
-                      st->batch.total_tris, st->batch.total_pixels, st->params.max_depth, st->params.integrator);
+        std::snprintf(buf, sizeof buf, "%s|batch|%u|%u|%u|d%u|i%u|%016llx", st->arch, st->batch.num_scenes,
+                      st->batch.total_tris, st->batch.total_pixels, st->params.max_depth, st->params.integrator, fp);
     else
-        std::snprintf(buf, sizeof buf, "%s|single|1|%u|%u|d%u|i%u", st->arch, st->num_tris, st->width * st->height,
-                      st->params.max_depth, st->params.integrator);
+        std::snprintf(buf, sizeof buf, "%s|single|1|%u|%u|d%u|i%u|%016llx", st->arch, st->num_tris,
+                      st->width * st->height, st->params.max_depth, st->params.integrator, fp);
     return buf;
 }
 
-bool tune_cache_lookup(const std::string& key, TraceConfig& cfg, double& ms) {
+// A cached choice: traversal shape and samples per pass.
+struct TuneChoice {
+    TraceConfig cfg;
+    uint32_t samples_per_pass;
+};
+
+bool tune_cache_lookup(const std::string& key, TuneChoice& choice, double& ms) {
     std::ifstream in(tune_cache_path());
     std::string line;
     bool found = false;
     while (std::getline(in, line)) {
-        char k[256] = {0};
-        unsigned w = 0, bl = 0, ch = 0;
+        char k[320] = {0};
+        unsigned w = 0, bl = 0, ch = 0, spp = 0;
         double t = 0;
-        if (std::sscanf(line.c_str(), "{\"key\": \"%255[^\"]\", \"bvh_width\": %u, \"block_size\": %u, \"queue_chunk\": %u, \"ms\": %lf}",
-                        k, &w, &bl, &ch, &t) == 5 &&
+        if (std::sscanf(line.c_str(),
+                        "{\"key\": \"%319[^\"]\", \"bvh_width\": %u, \"block_size\": %u, \"queue_chunk\": %u, "
+                        "\"samples_per_pass\": %u, \"ms\": %lf}",
+                        k, &w, &bl, &ch, &spp, &t) == 6 &&
             key == k) {
-            cfg = TraceConfig{w, bl, ch};
+            choice = TuneChoice{TraceConfig{w, bl, ch}, spp};
             ms = t;
-            found = trace_config_supported(cfg);   // the last matching line wins
+            found = trace_config_supported(choice.cfg);   // the last matching line wins
         }
     }
     return found;
 }
 
-void tune_cache_store(plugin_state* st, const std::string& key, const TraceConfig& c, double ms) {
+void tune_cache_store(plugin_state* st, const std::string& key, const TraceConfig& c, uint32_t spp_pass, double ms) {
     const std::string path = tune_cache_path();
     std::error_code ec;
     std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
@@ -403,15 +445,18 @@ void tune_cache_store(plugin_state* st, const std::string& key, const TraceConfi
         log_msg(st, "autotune: cannot write %s", path.c_str());
         return;
     }
-    char buf[512];
-    std::snprintf(buf, sizeof buf, "{\"key\": \"%s\", \"bvh_width\": %u, \"block_size\": %u, \"queue_chunk\": %u, \"ms\": %.4f}\n",
-                  key.c_str(), c.width, c.block, c.chunk, ms);
+    char buf[640];
+    std::snprintf(buf, sizeof buf,
+                  "{\"key\": \"%s\", \"bvh_width\": %u, \"block_size\": %u, \"queue_chunk\": %u, "
+                  "\"samples_per_pass\": %u, \"ms\": %.4f}\n",
+                  key.c_str(), c.width, c.block, c.chunk, spp_pass, ms);
     out << buf;
 }
 
 void apply_config(plugin_state* st, const TraceConfig& c) {
     st->trace_cfg = c;
     st->trace_blocks = persistent_blocks(st->device, c);
+    st->paths_blocks = paths_pass_blocks(st->device, c);
     st->params.bvh_width = c.width;
     st->params.block_size = c.block;
     st->params.queue_chunk = c.chunk;
@@ -443,23 +488,52 @@ bool time_probe(plugin_state* st, uint32_t target, uint32_t spp, float& ms) {
     return ok;
 }
 
+// Samples per pass the probes of `probe_spp` samples use now.
+uint32_t probe_pass_size(const plugin_state* st, uint32_t target, uint32_t probe_spp) {
+    if (target == Q_AUTOTUNE_BATCH) return batch_samples_per_pass(st, probe_spp);
+    const uint32_t npix = std::max<uint32_t>(1u, st->width * st->height);
+    uint32_t s = st->params.samples_per_pass ? st->params.samples_per_pass
+                                             : std::max<uint32_t>(1u, kAutoPassPaths / npix);
+    return std::max(1u, std::min(s, probe_spp));
+}
+
+// Candidate pass sizes (samples per pass, hence queue lengths) for probes of
+// probe_spp samples: powers of two up to the automatic size of about 2^27
+// paths, the six largest; a batch's must divide probe_spp.
+std::vector<uint32_t> pass_candidates(const plugin_state* st, uint32_t target, uint32_t probe_spp) {
+    const uint32_t npix = std::max<uint32_t>(1u, target == Q_AUTOTUNE_BATCH ? st->batch.total_pixels
+                                                                             : st->width * st->height);
+    const uint32_t cap = std::min<uint32_t>(probe_spp, std::max<uint32_t>(1u, kAutoPassPaths / npix));
+    std::vector<uint32_t> out;
+    for (uint32_t s = 1; s <= cap; s *= 2)
+        if (target != Q_AUTOTUNE_BATCH || probe_spp % s == 0) out.push_back(s);
+    if (out.size() > 6) out.erase(out.begin(), out.end() - 6);
+    return out;
+}
+
 int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t flags, Q_autotune_result* out) {
     if (target == Q_AUTOTUNE_BATCH && !st->batch.active) {
         log_msg(st, "autotune: no batch set");
         return Q_EXT_ERR_STATE;
     }
-    if (target != Q_AUTOTUNE_BATCH && st->params.engine != Q_ENGINE_WAVEFRONT) {
-        log_msg(st, "autotune: the single-scene target needs the wavefront engine");
-        return Q_EXT_ERR_STATE;
-    }
-    if (target != Q_AUTOTUNE_BATCH && (!st->have_camera || st->width == 0 || !st->scene_ok)) {
-        log_msg(st, "autotune: the single-scene probes render the last frame's camera: render a frame first");
+    if (target != Q_AUTOTUNE_BATCH && st->params.engine == Q_ENGINE_MEGAKERNEL) {
+        log_msg(st, "autotune: the single-scene target needs the wavefront or single-launch engine");
         return Q_EXT_ERR_STATE;
     }
     if (probe_spp == 0) probe_spp = 1;
     if (target == Q_AUTOTUNE_BATCH) {
         const uint32_t pp = batch_samples_per_pass(st, probe_spp);
         probe_spp = std::max(pp, probe_spp - probe_spp % pp);
+    }
+    const std::string key = tune_key(st, target);
+    TuneChoice cached{};
+    double cached_ms = 0;
+    const bool from_cache = (flags & Q_AUTOTUNE_USE_CACHE) && tune_cache_lookup(key, cached, cached_ms);
+    // probes render the last frame's camera; a choice served from the cache
+    // renders nothing
+    if (!from_cache && target != Q_AUTOTUNE_BATCH && (!st->have_camera || st->width == 0 || !st->scene_ok)) {
+        log_msg(st, "autotune: the single-scene probes render the last frame's camera: render a frame first");
+        return Q_EXT_ERR_STATE;
     }
     (void)hipSetDevice(st->device);
     if (!sync_render(st)) return Q_EXT_ERR_DEVICE;
@@ -481,40 +555,71 @@ int32_t autotune(plugin_state* st, uint32_t target, uint32_t probe_spp, uint32_t
     };
     Q_autotune_result res{};
     res.struct_size = sizeof(Q_autotune_result);
-    const std::string key = tune_key(st, target);
-    TraceConfig cached{};
-    double cached_ms = 0;
-    if ((flags & Q_AUTOTUNE_USE_CACHE) && tune_cache_lookup(key, cached, cached_ms)) {
-        apply_config(st, cached);
+    if (from_cache) {
+        apply_config(st, cached.cfg);
+        st->params.samples_per_pass = cached.samples_per_pass;
         res.num_entries = 1;
-        res.entries[0] = Q_autotune_entry{cached.width, cached.block, cached.chunk, 0, cached_ms};
+        res.entries[0] = Q_autotune_entry{cached.cfg.width, cached.cfg.block, cached.cfg.chunk,
+                                          cached.samples_per_pass, cached_ms};
         res.best = 0;
         res.from_cache = 1;
     } else {
         const TraceConfig saved = st->trace_cfg;
-        const uint32_t n = std::min<uint32_t>(trace_config_count(), Q_AUTOTUNE_MAX_ENTRIES);
+        const uint32_t saved_pass = st->params.samples_per_pass;
+        auto fail = [&]() {
+            apply_config(st, saved);
+            st->params.samples_per_pass = saved_pass;
+            (void)sync_render(st);
+            (void)restore_stats();
+            return Q_EXT_ERR_DEVICE;
+        };
+        // first run warms code and caches, the second is kept
+        auto probe = [&](float& ms) {
+            float warm = 0.f;
+            return time_probe(st, target, probe_spp, warm) && time_probe(st, target, probe_spp, ms);
+        };
+        // stage 1: every compiled traversal shape at the current pass size
+        // (the single-launch engine: each BVH width and block size it has once)
+        const bool single_launch = st->params.engine == Q_ENGINE_PERSISTENT;
+        const uint32_t pass0 = probe_pass_size(st, target, probe_spp);
         double best = 1e300;
-        for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t i = 0; i < trace_config_count() && res.num_entries < Q_AUTOTUNE_MAX_ENTRIES; ++i) {
             const TraceConfig c = trace_config_at(i);
+            if (single_launch && (c.chunk != 1u || !paths_config_supported(c))) continue;
             apply_config(st, c);
-            float ms = 0.f, warm = 0.f;
-            // first run warms code and caches, the second is kept
-            if (!time_probe(st, target, probe_spp, warm) || !time_probe(st, target, probe_spp, ms)) {
-                apply_config(st, saved);
-                (void)sync_render(st);
-                (void)restore_stats();
-                return Q_EXT_ERR_DEVICE;
-            }
-            res.entries[i] = Q_autotune_entry{c.width, c.block, c.chunk, 0, double(ms)};
-            if (ms < best) { best = ms; res.best = i; }
+            float ms = 0.f;
+            if (!probe(ms)) return fail();
+            const uint32_t k = res.num_entries++;
+            res.entries[k] = Q_autotune_entry{c.width, c.block, c.chunk, pass0, double(ms)};
+            if (ms < best) { best = ms; res.best = k; }
         }
-        res.num_entries = n;
+        {
+            const Q_autotune_entry& e = res.entries[res.best];
+            apply_config(st, TraceConfig{e.bvh_width, e.block_size, e.queue_chunk});
+        }
+        // stage 2: pass sizes (queue lengths) with the chosen shape, when the
+        // probe leaves a choice
+        const std::vector<uint32_t> passes = pass_candidates(st, target, probe_spp);
+        if (passes.size() > 1) {
+            const Q_autotune_entry shape = res.entries[res.best];
+            for (uint32_t sp : passes) {
+                if (res.num_entries >= Q_AUTOTUNE_MAX_ENTRIES) break;
+                st->params.samples_per_pass = sp;
+                float ms = 0.f;
+                if (!probe(ms)) return fail();
+                const uint32_t i = res.num_entries++;
+                res.entries[i] = Q_autotune_entry{shape.bvh_width, shape.block_size, shape.queue_chunk, sp, double(ms)};
+                if (ms < best) { best = ms; res.best = i; }
+            }
+        }
         const Q_autotune_entry& e = res.entries[res.best];
-        apply_config(st, TraceConfig{e.bvh_width, e.block_size, e.queue_chunk});
+        st->params.samples_per_pass = passes.size() > 1 ? e.samples_per_pass : saved_pass;
         if (!(flags & Q_AUTOTUNE_NO_STORE))
-            tune_cache_store(st, key, TraceConfig{e.bvh_width, e.block_size, e.queue_chunk}, e.ms);
-        log_msg(st, "autotune (%s): best width %u, block %u, chunk %u: %.3f ms",
-                target == Q_AUTOTUNE_BATCH ? "batch" : "single", e.bvh_width, e.block_size, e.queue_chunk, e.ms);
+            tune_cache_store(st, key, TraceConfig{e.bvh_width, e.block_size, e.queue_chunk},
+                             st->params.samples_per_pass, e.ms);
+        log_msg(st, "autotune (%s): best width %u, block %u, chunk %u, %u samples per pass: %.3f ms",
+                target == Q_AUTOTUNE_BATCH ? "batch" : "single", e.bvh_width, e.block_size, e.queue_chunk,
+                e.samples_per_pass, e.ms);
     }
     // leave the pass graph of the chosen configuration captured, the
     // accumulations empty and the statistics as they were before tuning

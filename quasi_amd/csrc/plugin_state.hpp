@@ -26,6 +26,7 @@ enum KernelSlot : uint32_t {
     K_MEGAKERNEL = 0, K_RESOLVE, K_FILL,
     K_RAYGEN, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUMULATE,  // wavefront, in WaveKernel order
     K_BATCH_PASS,                                          // one replay of a captured batch pass
+    K_PATHS,                                               // one single-launch pass (Q_ENGINE_PERSISTENT)
     K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
@@ -85,6 +86,8 @@ struct AdjointState {
     DeviceBuffer rec_hit;        // path records of the replayed pass: hit (t, triangle) per bounce and path
     DeviceBuffer rec_vis;        // ... and shadow-ray visibility (bytes)
     DeviceBuffer counters;       // ray counters of the recorded passes (kept out of the render statistics)
+    DeviceBuffer param_map;      // albedo parameter of each triangle (input order)
+    DeviceBuffer param_out;      // FP64 gradients of the albedo parameters
     uint64_t indices_generation = 0;
 };
 
@@ -124,6 +127,7 @@ struct plugin_state {
     WaveBuffers wave;
     TraceConfig trace_cfg{4, 256, 1};
     uint32_t trace_blocks = 0;
+    uint32_t paths_blocks = 0;          // grid of the single-launch pass kernel for trace_cfg (0: unsupported)
     hipEvent_t pending_start = nullptr;
     int num_cus = 256;
 

@@ -1,0 +1,70 @@
+// wave_common.hpp -- device helpers shared by the wavefront kernels
+// (kernels_wave.hip) and the single-launch path kernel (kernels_paths.hip).
+#pragma once
+
+#include "device.hpp"
+
+namespace quasi_hip {
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+template <bool BATCH>
+__device__ __forceinline__ const SceneDesc& scene_at(const PassParams& P, uint32_t sid) {
+    if (BATCH) return P.scenes[sid];
+    return P.scene0;
+}
+
+// Scene owning path id p (BATCH): the last scene whose path_begin <= p.
+__device__ __forceinline__ uint32_t scene_of_path(const PassParams& P, uint32_t p) {
+    uint32_t lo = 0, hi = P.num_scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.scenes[mid].path_begin <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scene_of_pixel(const PassParams& P, uint32_t pix) {
+    uint32_t lo = 0, hi = P.num_scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.scenes[mid].pix_begin <= pix) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t pass_sample_begin(const PassParams& P) {
+    return P.dyn_sample_begin ? *P.dyn_sample_begin : P.sample_begin;
+}
+
+__device__ __forceinline__ uint32_t wave_global() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+
+// Adds the length of a segmented queue to a ray counter (first wave of the
+// grid; one segment per lane).
+static_assert(kQueueSegments == 64, "one segment per lane of a wave");
+__device__ __forceinline__ void count_rays(const uint32_t* lengths, unsigned long long* counter) {
+    if (blockIdx.x == 0 && threadIdx.x < 64u) {
+        const uint32_t total = wave_sum(lengths[threadIdx.x * kCounterStride]);
+        if (threadIdx.x == 0u && total) atomicAdd(counter, (unsigned long long)total);
+    }
+}
+
+template <int WIDTH, bool BATCH>
+__device__ __forceinline__ const float4* scene_nodes(const PassParams& P, uint32_t sid) {
+    const SceneDesc& sc = scene_at<BATCH>(P, sid);
+    return static_cast<const float4*>(WIDTH == 4 ? sc.nodes4 : sc.nodes);
+}
+
+__device__ __forceinline__ void add_shadow(float4* rad, const float4* sc_a, uint32_t k, uint32_t p) {
+    const float4 c = sc_a[k];
+    float4 l = rad[p];
+    l.x += c.x; l.y += c.y; l.z += c.z;
+    rad[p] = l;
+}
+
+constexpr uint32_t kRefillIdle = 32;   // measured best of 8..48 on the 1M soup (28-32 alike)
+constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
+
+}  // namespace
+}  // namespace quasi_hip

@@ -336,19 +336,28 @@ class HipRenderer:
         return r
 
     def adjoint(self, image, spp: int, sample_begin: int = 0, mode: int = abi.Q_ADJOINT_IMAGE_GRADIENT,
-                seed: Optional[int] = None, num_vertices: Optional[int] = None):
+                seed: Optional[int] = None, num_vertices: Optional[int] = None, max_pass_paths: int = 0,
+                albedo_param=None, num_albedo_params: int = 0):
         """Q_plugin_adjoint for the current scene, camera and resolution.
         image: (H, W, 4) float32 dL/dI (or the L2 target).  Returns
-        (grad_albedo (T, 3) float64, grad_vertices (V, 3) float64, result)."""
+        (grad_albedo (T, 3) float64 -- (num_albedo_params, 3) with an albedo
+        parameter map --, grad_vertices (V, 3) float64, result)."""
         img = np.ascontiguousarray(image, dtype=np.float32)
         nt = self.stats().num_triangles
         nv = self._num_vertices if num_vertices is None else num_vertices
-        ga = np.zeros((nt, 3), dtype=np.float64)
+        pmap = None
+        if albedo_param is not None:
+            pmap = np.ascontiguousarray(albedo_param, dtype=np.uint32)
+            ga = np.zeros((num_albedo_params, 3), dtype=np.float64)
+        else:
+            ga = np.zeros((nt, 3), dtype=np.float64)
         gv = np.zeros((nv, 3), dtype=np.float64)
         d = abi.Q_adjoint_desc(C.sizeof(abi.Q_adjoint_desc), mode, spp, sample_begin, 0 if seed is None else seed,
                                0 if seed is None else abi.Q_ADJOINT_SEED,
                                img.ctypes.data_as(C.POINTER(C.c_float)),
-                               ga.ctypes.data_as(C.POINTER(C.c_double)), gv.ctypes.data_as(C.POINTER(C.c_double)))
+                               ga.ctypes.data_as(C.POINTER(C.c_double)), gv.ctypes.data_as(C.POINTER(C.c_double)),
+                               max_pass_paths, num_albedo_params if pmap is not None else 0,
+                               pmap.ctypes.data_as(C.POINTER(C.c_uint32)) if pmap is not None else None)
         res = abi.Q_adjoint_result()
         res.struct_size = C.sizeof(abi.Q_adjoint_result)
         self._check(self._adjoint(self.plugin.handle, C.byref(d), C.byref(res)), "adjoint")

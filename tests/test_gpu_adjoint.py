@@ -207,6 +207,60 @@ def test_adjoint_deep_paths_use_the_largest_lds_layout():
     assert ea <= 1e-3 and ev <= 1e-3
 
 
+def test_adjoint_row_bands_and_sample_blocks_equal_one_pass():
+    """Passes bounded by max_pass_paths: blocks of 3 samples, and single
+    samples of 7-row bands (the last band 3 rows), give the one-pass gradient
+    to FP32 atomic-order rounding, and both match the FP64 oracle."""
+    mesh = scenes.diff_sphere(frequency=8, stage="init")
+    w, h, spp = 96, 80, 8
+    dimg = weights(w, h, 41)
+    say(f"[adjoint-bands] {w}x{h} spp={spp}")
+    with HipRenderer(w, h) as r:
+        setup(r, mesh, w, h, spp, seed=2)
+        ga, gv, res = r.adjoint(dimg, spp)
+        assert (res.passes, res.bands) == (1, 1)
+        gb, vb, rb = r.adjoint(dimg, spp, max_pass_paths=w * h * 3)
+        assert (rb.passes, rb.bands) == (3, 1)
+        gc, vc, rc = r.adjoint(dimg, spp, max_pass_paths=w * 7)
+        assert (rc.passes, rc.bands) == (spp * 12, 12)
+        assert rc.working_bytes < res.working_bytes
+        say(f"[adjoint-bands] working bytes: one pass {res.working_bytes}, 7-row bands {rc.working_bytes}")
+    for a, b in ((gb, ga), (gc, ga)):
+        assert rel_l1(a, b) <= 1e-6
+    for a, b in ((vb, gv), (vc, gv)):
+        assert rel_l1(a, b) <= 1e-6
+    sc = O.Scene(mesh)
+    ra, rv = sc.adjoint(w, h, spp, dimg, max_depth=4, rr_start=3, integrator=1, rng_mode=1, seed=2, precision=64)
+    assert rel_l1(gc, ra) <= 1e-3 and rel_l1(vc, rv) <= 1e-3
+
+
+def test_adjoint_albedo_parameter_map():
+    """Triangles sharing albedo parameters: the device's FP64 reduction onto
+    the map equals the per-triangle gradients summed on the host; unmapped
+    triangles contribute nothing."""
+    mesh = scenes.diff_sphere(frequency=8, stage="init")
+    w, h, spp = 64, 48, 4
+    nt = mesh.num_triangles
+    pmap = (np.arange(nt, dtype=np.uint32) // 5).astype(np.uint32)
+    pmap[::7] = abi.Q_ADJOINT_NO_PARAM
+    npar = int(pmap[pmap != abi.Q_ADJOINT_NO_PARAM].max()) + 1
+    dimg = weights(w, h, 43)
+    with HipRenderer(w, h) as r:
+        setup(r, mesh, w, h, spp, seed=4)
+        ga, gv, _ = r.adjoint(dimg, spp)
+        gp, gvp, _ = r.adjoint(dimg, spp, albedo_param=pmap, num_albedo_params=npar)
+        with pytest.raises(RuntimeError):
+            bad = pmap.copy()
+            bad[3] = npar
+            r.adjoint(dimg, spp, albedo_param=bad, num_albedo_params=npar)
+    keep = pmap != abi.Q_ADJOINT_NO_PARAM
+    host = np.stack([np.bincount(pmap[keep], weights=ga[keep, c], minlength=npar) for c in range(3)], axis=1)
+    say(f"[adjoint-map] {npar} parameters: rel-L1 vs host sums {rel_l1(gp, host):.2e}")
+    assert gp.shape == (npar, 3)
+    assert rel_l1(gp, host) <= 1e-6
+    assert rel_l1(gvp, gv) <= 1e-6
+
+
 def test_adjoint_seed_and_sample_range():
     """Samples 4..7 replayed with an explicit seed equal the gradient of an
     oracle render of those samples with that seed."""

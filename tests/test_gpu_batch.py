@@ -68,6 +68,29 @@ def test_batch_equals_plain_per_scene(builder):
             assert np.array_equal(got[i][1], ids), i
 
 
+def test_single_launch_batch_equals_wavefront_batch():
+    """A batch rendered with one persistent launch per pass (Q_ENGINE_PERSISTENT,
+    captured in the pass graph) equals the wavefront launch sequence's batch
+    bit for bit, scene by scene, and each scene its single-scene render."""
+    batch = small_batch()
+    spp = 4
+    say(f"[batch single-launch] {len(batch)} scenes")
+    got = {}
+    for engine in (abi.Q_ENGINE_WAVEFRONT, abi.Q_ENGINE_PERSISTENT):
+        with HipRenderer(16, 16) as r:
+            r.set_params(spp_per_render=1, seed=0, **dict(PARAMS, engine=engine, block_size=128))
+            r.set_batch(batch)
+            r.render_batch(spp)
+            got[engine] = [(r.readback_batch(i), r.batch_ids(i)) for i in range(len(batch))]
+            st = r.stats()
+            say(f"[batch single-launch] engine {engine}: rays {st.rays_camera}/{st.rays_extension}/{st.rays_shadow}")
+    for i in range(len(batch)):
+        (aw, iw), (ap, ip) = got[abi.Q_ENGINE_WAVEFRONT][i], got[abi.Q_ENGINE_PERSISTENT][i]
+        for k in ("beauty", "albedo", "normal", "depth"):
+            assert np.array_equal(aw[k], ap[k]), (i, k)
+        assert np.array_equal(iw, ip), i
+
+
 def test_batch_with_an_empty_scene():
     """A scene of no triangles among others: its image is zero and its ids are
     misses, and the other scenes still equal their plain renders."""
@@ -165,7 +188,9 @@ def test_autotune_batch(tmp_path, monkeypatch):
         r.render_batch(1)
         before = r.stats()
         res = r.autotune(abi.Q_AUTOTUNE_BATCH, probe_spp=1)
-        assert res.num_entries == 20 and res.best < res.num_entries and not res.from_cache
+        # every traversal shape once; a 1-sample probe leaves no choice of pass size
+        assert res.num_entries == 22 and res.best < res.num_entries and not res.from_cache
+        assert all(res.entries[k].samples_per_pass == 1 for k in range(res.num_entries))
         # the probes leave the caller's statistics as they were
         after = r.stats()
         assert before.rays_camera > 0

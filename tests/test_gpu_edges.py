@@ -14,8 +14,8 @@ from tests.parity import check_ids, rel_l1
 pytestmark = pytest.mark.gpu
 
 REL_BAR = 1e-4
-ENGINES = [abi.Q_ENGINE_MEGAKERNEL, abi.Q_ENGINE_WAVEFRONT]
-ENGINE_IDS = ["megakernel", "wavefront"]
+ENGINES = [abi.Q_ENGINE_MEGAKERNEL, abi.Q_ENGINE_WAVEFRONT, abi.Q_ENGINE_PERSISTENT]
+ENGINE_IDS = ["megakernel", "wavefront", "single-launch"]
 
 
 def say(*a):

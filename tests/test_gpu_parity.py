@@ -51,8 +51,8 @@ def cam_of(mesh):
     return make_camera(c["position"], c["target"], c["up"], c["fov"])
 
 
-ENGINES = [abi.Q_ENGINE_MEGAKERNEL, abi.Q_ENGINE_WAVEFRONT]
-ENGINE_IDS = ["megakernel", "wavefront"]
+ENGINES = [abi.Q_ENGINE_MEGAKERNEL, abi.Q_ENGINE_WAVEFRONT, abi.Q_ENGINE_PERSISTENT]
+ENGINE_IDS = ["megakernel", "wavefront", "single-launch"]
 
 
 @pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
@@ -176,7 +176,7 @@ def test_wavefront_matches_megakernel(integrator):
     w, h, spp = 128, 72, 6
     say(f"[wave-vs-mega] integrator={integrator}")
     out = {}
-    for engine in ENGINES:
+    for engine in ENGINES[:2]:
         with HipRenderer(w, h) as r:
             r.set_scene(mesh)
             r.camera = cam_of(mesh)
@@ -191,6 +191,33 @@ def test_wavefront_matches_megakernel(integrator):
     assert sm.rays_camera == sw.rays_camera == w * h * spp
     assert abs(int(sm.rays_extension) - int(sw.rays_extension)) <= 1e-4 * sm.rays_extension
     assert abs(int(sm.rays_shadow) - int(sw.rays_shadow)) <= 1e-4 * max(1, sm.rays_shadow)
+
+
+@pytest.mark.parametrize("integrator,width,block", [(1, 4, 256), (1, 2, 128), (0, 4, 128)])
+def test_single_launch_equals_wavefront(integrator, width, block):
+    """Q_ENGINE_PERSISTENT traces every path as the wavefront kernels do and
+    accumulates in the same order: every AOV, the primary ids and the ray
+    counts equal the wavefront engine's bit for bit."""
+    mesh = scenes.soup(150_000, seed=8)
+    w, h, spp = 160, 96, 8
+    say(f"[single-launch] integrator={integrator} width={width} block={block}")
+    out = {}
+    for engine in (abi.Q_ENGINE_WAVEFRONT, abi.Q_ENGINE_PERSISTENT):
+        with HipRenderer(w, h) as r:
+            r.set_scene(mesh)
+            r.camera = cam_of(mesh)
+            r.set_params(spp_per_render=spp, max_depth=8, rr_start=3, rng_mode=1, integrator=integrator, seed=5,
+                         flags=abi.Q_FLAG_RECORD_IDS, id_sample=2, engine=engine, bvh_width=width, block_size=block,
+                         samples_per_pass=4)
+            r.render(1)
+            out[engine] = (r.readback_aov(), r.primary_ids(), r.stats())
+    (aw, iw, sw), (ap, ip, sp) = out[abi.Q_ENGINE_WAVEFRONT], out[abi.Q_ENGINE_PERSISTENT]
+    for k in ("beauty", "albedo", "normal", "depth"):
+        assert np.array_equal(aw[k], ap[k]), k
+    assert np.array_equal(iw, ip)
+    assert (sw.rays_camera, sw.rays_extension, sw.rays_shadow) == (sp.rays_camera, sp.rays_extension, sp.rays_shadow)
+    names = {sp.kernels[k].name.decode(): sp.kernels[k].launches for k in range(sp.num_kernels)}
+    assert names["paths_pass"] == 2 and names["wf_extend"] == 0
 
 
 def test_wavefront_pass_split_and_determinism():
@@ -214,12 +241,14 @@ def test_wavefront_pass_split_and_determinism():
 
 
 TRACE_CONFIGS = ([(w, b, c) for w in (2, 4) for b in (64, 128, 256) for c in (1, 64, 256)] +
-                 [(w, 128, abi.Q_QUEUE_FLAT) for w in (2, 4)])
+                 [(w, 128, abi.Q_QUEUE_FLAT) for w in (2, 4)] +
+                 [(4, b, abi.Q_QUEUE_LDS_TOP) for b in (128, 256)])
 
 
 def test_all_trace_configs_render_identically():
     """Every compiled traversal variant (BVH width 2/4, block 64/128/256, queue
-    chunk: per-lane fetch, 64, 256, and the non-persistent form) yields the same
+    chunk: per-lane fetch, 64, 256, the non-persistent form, and per-lane fetch
+    with the top BVH4 levels staged in LDS) yields the same
     image bit for bit: the closest hit is unique under the tie rule and
     occlusion does not depend on which blocker is found."""
     mesh = scenes.soup(150_000, seed=8)

@@ -9,7 +9,12 @@ Size-independent properties checked at full size:
   l_k the loss of GPU forward renders of depth k (each NEE term found at
   bounce b carries b+1 albedo factors), to 1e-4 relative;
 - the 1M-parameter albedo grid (1024 x 1024 blocks of 4 x 4 cells, 32
-  triangles each) collects every terrain triangle's gradient once.
+  triangles each) collects every terrain triangle's gradient once;
+- with block-constant albedo (the grid as the scene's parameters), the
+  device's FP64 reduction onto the grid satisfies Euler's identity on the grid
+  itself, and passes bounded to 2^20 paths (single samples of 512-row bands:
+  per-tile path recomputation) give the same grid gradient as the default
+  passes.
 """
 import time
 
@@ -100,3 +105,58 @@ def test_adjoint_euler_identity_full_size(terrain):
     gg = np.stack([np.bincount(group, weights=ga[:NT_TERRAIN, c], minlength=1024 * 1024) for c in range(3)], axis=1)
     assert gg.shape == (1024 * 1024, 3)
     assert np.allclose(gg.sum(axis=0), ga[:NT_TERRAIN].sum(axis=0), rtol=1e-9, atol=1e-12)
+
+
+def grid_map():
+    """Albedo parameter of every triangle: the 1024 x 1024 grid of 4 x 4-cell
+    blocks (the last block of a row or column also takes the 3 leftover
+    cells); the sun and sky triangles have none."""
+    cells = np.arange(NT_TERRAIN, dtype=np.int64) // 2
+    row, col = cells // (GRID - 1), cells % (GRID - 1)
+    return (np.minimum(row // 4, 1023) * 1024 + np.minimum(col // 4, 1023)).astype(np.uint32)
+
+
+def test_adjoint_parameter_grid_full_size(terrain):
+    w = h = 2048
+    spp, D = 16, 6
+    npar = 1024 * 1024
+    group = grid_map()
+    # block-constant albedo: the block mean of the noise albedo
+    alb = terrain.albedo.astype(np.float64)
+    theta = np.stack([np.bincount(group, weights=alb[:NT_TERRAIN, c], minlength=npar) for c in range(3)], axis=1)
+    theta /= np.bincount(group, minlength=npar)[:, None]
+    mesh = scenes.Mesh(terrain.vertices, terrain.indices, terrain.albedo.copy(), terrain.emission, terrain.camera,
+                       "terrain-grid")
+    mesh.albedo[:NT_TERRAIN] = theta[group].astype(np.float32)
+    theta32 = theta.astype(np.float32).astype(np.float64)
+    pmap = np.full(mesh.num_triangles, abi.Q_ADJOINT_NO_PARAM, dtype=np.uint32)
+    pmap[:NT_TERRAIN] = group
+    g = np.random.Generator(np.random.PCG64(3)).normal(size=(h, w, 4)).astype(np.float32)
+    g[..., 3] = 0.0
+    with HipRenderer(w, h) as r:
+        r.set_scene(mesh)
+        cam(r, mesh)
+        losses = []
+        for k in range(1, D + 1):
+            r.set_params(spp_per_render=spp, max_depth=k, rr_start=D, rng_mode=1, integrator=1, seed=5,
+                         engine=abi.Q_ENGINE_WAVEFRONT, bvh_width=4)
+            r.render(1)
+            img = r.readback().astype(np.float64)
+            losses.append(float((img[..., :3] * g[..., :3]).sum()))
+        t0 = time.time()
+        gp, gv, res = r.adjoint(g, spp, albedo_param=pmap, num_albedo_params=npar)
+        print(f"[terrain-grid] adjoint onto {npar} parameters: {res.kernel_ms:.1f} ms device, "
+              f"{time.time() - t0:.2f} s wall, {res.passes} passes, {res.working_bytes / 2**30:.2f} GiB", flush=True)
+        t0 = time.time()
+        gb, vb, rb = r.adjoint(g, spp, albedo_param=pmap, num_albedo_params=npar, max_pass_paths=1 << 20)
+        print(f"[terrain-grid] 2^20-path passes: {rb.passes} passes in {rb.bands} bands, {rb.kernel_ms:.1f} ms "
+              f"device, {rb.working_bytes / 2**30:.2f} GiB", flush=True)
+    assert (rb.bands, rb.passes) == (4, 4 * spp)
+    assert rb.working_bytes < res.working_bytes
+    lhs = float((theta32 * gp).sum())
+    rhs = sum((k - 1) * (losses[k - 1] - losses[k - 2]) for k in range(2, D + 1))
+    print(f"[terrain-grid] Euler on the grid: sum(theta * grad) {lhs:.6f}, from forward renders {rhs:.6f}", flush=True)
+    assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
+    err = float(np.abs(gb - gp).sum() / np.abs(gp).sum())
+    print(f"[terrain-grid] banded vs default passes: rel-L1 {err:.2e}", flush=True)
+    assert err <= 1e-5 and float(np.abs(vb - gv).sum() / np.abs(gv).sum()) <= 1e-5
