@@ -165,6 +165,7 @@ def bench_batch(args):
     plain_s = 0.0
     plain_rays = 0
     plain_imgs = []
+    per_scene = []   # (triangles, pixels, seconds, rays) of the persistent engine, scene by scene
     prog_s = 0.0
     prog_rays = 0
     flat_s = 0.0
@@ -197,8 +198,11 @@ def bench_batch(args):
             t0 = time.perf_counter()
             r.render(1)
             r.synchronize()
-            plain_s += time.perf_counter() - t0
-            plain_rays += rays_of(r.stats())
+            dt = time.perf_counter() - t0
+            plain_s += dt
+            nr = rays_of(r.stats())
+            plain_rays += nr
+            per_scene.append((mesh.num_triangles, w * h, dt, nr))
             plain_imgs.append(r.readback_aov()["beauty"])
             if args.batch_progressive:
                 # the reference host's usage: one sample per Q_plugin_render call
@@ -268,11 +272,23 @@ def bench_batch(args):
             f"{single_best.samples_per_pass} samples per pass), max rel-L1 vs plain {single_err:.2e}")
     plain = plain_rays / plain_s / 1e6
     batched = batched_rays / batched_s / 1e6
+    # where the plain time goes, by scene size: triangle decades and resolutions
+    by_class = {}
+    for nt, npx, dt, nr in per_scene:
+        key = f"{'1k-10k' if nt < 10_000 else '10k-100k' if nt < 100_000 else '100k-500k'} tris, {int(npx ** 0.5)}^2"
+        c = by_class.setdefault(key, {"scenes": 0, "seconds": 0.0, "rays": 0})
+        c["scenes"] += 1
+        c["seconds"] += dt
+        c["rays"] += nr
+    for c in by_class.values():
+        c["mrays"] = round(c["rays"] / c["seconds"] / 1e6, 1)
+        c["seconds"] = round(c["seconds"], 4)
     log(f"batch: batched {batched_s:.3f} s, {batched:.1f} Mrays/s, speedup {batched / plain:.3f}, "
         f"max rel-L1 vs plain {err:.2e}")
     return {"workload": f"{len(batch)} scenes (room + soup, {ntri} triangles, {npix} pixels), {spp} spp, "
                         f"depth {depth}, RR from 3, NEE",
             "plain_mrays": round(plain, 3), "batched_mrays": round(batched, 3),
+            "plain_by_class": dict(sorted(by_class.items())),
             "speedup": round(batched / plain, 4), "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4),
             "plain_rays": int(plain_rays), "batched_rays": int(batched_rays),
             "samples_per_pass": args.batch_samples_per_pass,

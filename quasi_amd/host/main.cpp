@@ -10,8 +10,9 @@
 //              [--width W] [--height H] [--device K] [--scene file.qscene]
 //              [--spp-per-render S] [--depth D] [--rr-start R] [--seed X]
 //              [--integrator reference|nee] [--rng reference|hashed]
-//              [--engine megakernel|wavefront] [--compression zip|none]
-//              [--single-layer] [--watch] [--reload-at K] [--ppm file.ppm]
+//              [--engine megakernel|wavefront|single-launch] [--compression zip|none]
+//              [--single-layer] [--watch] [--reload-at K] [--wait-at K --wait-file PATH]
+//              [--ppm file.ppm]
 //              [--azimuth A] [--elevation E] [--distance D] [--orbit-step S]
 //
 // Without a plugin path the library next to this executable's directory
@@ -23,7 +24,9 @@
 // The plugin is loaded through the hot-reload manager (manager.hpp, after the
 // reference's plugin/manager.hpp): from a private copy, so the library may be
 // rebuilt while the host runs.  --watch polls the library between frames and
-// reloads it when it changes; --reload-at K reloads it once before frame K.
+// reloads it when it changes; --reload-at K reloads it once before frame K;
+// --wait-at K --wait-file PATH holds frame K until PATH exists (a fixed point
+// for a driver of the host, such as a test that rebuilds the library).
 // Around a reload the host waits for the GPU, and afterwards hands the new
 // plugin the scene and parameters again and marks the camera dirty, so the
 // accumulation restarts from the reload.
@@ -45,6 +48,7 @@
 #include <cstring>
 #include <string>
 #include <string_view>
+#include <thread>
 #include <vector>
 
 #include "dynamic_library.hpp"
@@ -70,6 +74,8 @@ struct Options {
     bool single_layer = false;
     bool watch = false;
     int reload_at = -1;
+    int wait_at = -1;                      // before this frame, wait until wait_file exists
+    std::filesystem::path wait_file;
     std::filesystem::path ppm;
     bool orbit = false;
     float azimuth = 0.f, elevation = 0.f, distance = 3.5f, orbit_step = 0.f;
@@ -116,8 +122,8 @@ int usage(const char* argv0) {
                  "usage: %s [plugin.so] --render N [--out file.exr | --out-dir DIR] [--width W] [--height H]\n"
                  "       [--device K] [--scene file.qscene] [--spp-per-render S] [--depth D] [--rr-start R]\n"
                  "       [--seed X] [--integrator reference|nee] [--rng reference|hashed]\n"
-                 "       [--engine megakernel|wavefront] [--compression zip|none] [--single-layer]\n"
-                 "       [--watch] [--reload-at K] [--ppm file.ppm]\n"
+                 "       [--engine megakernel|wavefront|single-launch] [--compression zip|none] [--single-layer]\n"
+                 "       [--watch] [--reload-at K] [--wait-at K --wait-file PATH] [--ppm file.ppm]\n"
                  "       [--azimuth A] [--elevation E] [--distance D] [--orbit-step S]\n",
                  argv0);
     return EXIT_FAILURE;
@@ -142,6 +148,8 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--device") o.device = std::atoi(v);
         else if (a == "--scene") o.scene = v;
         else if (a == "--reload-at") o.reload_at = std::atoi(v);
+        else if (a == "--wait-at") o.wait_at = std::atoi(v);
+        else if (a == "--wait-file") o.wait_file = v;
         else if (a == "--ppm") o.ppm = v;
         else if (a == "--azimuth") { o.azimuth = std::strtof(v, nullptr); o.orbit = true; }
         else if (a == "--elevation") { o.elevation = std::strtof(v, nullptr); o.orbit = true; }
@@ -153,7 +161,10 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "--seed") o.seed = std::atol(v);
         else if (a == "--integrator") o.integrator = std::string_view(v) == "nee" ? Q_INTEGRATOR_NEE : Q_INTEGRATOR_REFERENCE;
         else if (a == "--rng") o.rng = std::string_view(v) == "hashed" ? Q_RNG_HASHED : Q_RNG_REFERENCE;
-        else if (a == "--engine") o.engine = std::string_view(v) == "wavefront" ? Q_ENGINE_WAVEFRONT : Q_ENGINE_MEGAKERNEL;
+        else if (a == "--engine")
+            o.engine = std::string_view(v) == "wavefront"       ? Q_ENGINE_WAVEFRONT
+                       : std::string_view(v) == "single-launch" ? Q_ENGINE_PERSISTENT
+                                                                : Q_ENGINE_MEGAKERNEL;
         else if (a == "--compression") o.compression = std::string_view(v) == "none" ? Q::io::exr_compression::none : Q::io::exr_compression::zip;
         else if (a[0] != '-') o.plugin = std::string(a);
         else return false;
@@ -295,6 +306,21 @@ int main(int argc, char** argv) {
     const auto t0 = last;
     int since_load = 0;   // frames accumulated by the current plugin
     for (int i = 0; i < o.frames; ++i) {
+        if (i == o.wait_at && !o.wait_file.empty()) {
+            // a fixed point for whoever drives the host (a test changing the
+            // watched library): frame i starts once the file exists
+            std::printf("[Host] Waiting before frame %d for %s\n", i, o.wait_file.c_str());
+            std::fflush(stdout);
+            const auto until = std::chrono::steady_clock::now() + std::chrono::seconds(120);
+            std::error_code ec;
+            while (!std::filesystem::exists(o.wait_file, ec)) {
+                if (std::chrono::steady_clock::now() > until) {
+                    std::fprintf(stderr, "[Host] %s did not appear within 120 s\n", o.wait_file.c_str());
+                    return EXIT_FAILURE;
+                }
+                std::this_thread::sleep_for(std::chrono::milliseconds(5));
+            }
+        }
         bool reloaded = false;
         if (i == o.reload_at) {
             (void)mgr.reload();

@@ -152,7 +152,9 @@ def test_adjoint_full_size_cfg4_fd_and_fp64():
     """The differentiable-rendering workload at its stated size (512 x 512,
     32 spp, depth 4, dL/dI = 2 (I - T)): the whole image's gradient against
     the FP64 CPU adjoint, and 64 random parameters (seed 6) against FP64
-    central differences of the frozen estimator (Richardson-extrapolated)."""
+    central differences of the frozen estimator (Richardson-extrapolated),
+    at eps 1e-3 where that difference has converged (it agrees with eps 1e-4
+    to 1e-4) and at eps 1e-4 for all 64."""
     w = h = 512
     spp = 32
     init = scenes.diff_sphere(22, "init")
@@ -179,14 +181,26 @@ def test_adjoint_full_size_cfg4_fd_and_fp64():
     params = [("albedo", int(t), int(rng.integers(3))) for t in rng.choice(live_t, 32, replace=False)]
     params += [("vertex", int(v), int(rng.integers(3))) for v in rng.choice(live_v, 32, replace=False)]
     gpu = np.array([ga[i, c] if k == "albedo" else gv[i, c] for k, i, c in params])
-    fd = np.array([tape.fd(k, i, c, eps=1e-3, richardson=True) for k, i, c in params])
+    # eps 1e-3 (Richardson-extrapolated) and, to see whether that difference
+    # has converged, eps 1e-4: at full size a few vertices near the jittered
+    # sphere's silhouette have gradients of 100+ that are far from linear over
+    # 1e-3 (vertex 77: 131.12 at eps 1e-3, 133.887 at eps 1e-4, where the FP64
+    # and the FP32 CPU adjoints also are)
+    fd3 = np.array([tape.fd(k, i, c, eps=1e-3, richardson=True) for k, i, c in params])
+    fd4 = np.array([tape.fd(k, i, c, eps=1e-4, richardson=True) for k, i, c in params])
     tape.close()
-    floor = 1e-2 * np.abs(fd).max()
-    err = np.abs(gpu - fd) / np.maximum(np.abs(fd), floor)
+    floor = 1e-2 * np.abs(fd4).max()
+    err = np.abs(gpu - fd4) / np.maximum(np.abs(fd4), floor)
+    conv = np.abs(fd3 - fd4) / np.maximum(np.abs(fd4), floor) <= 1e-4
+    err3 = np.abs(gpu - fd3) / np.maximum(np.abs(fd3), floor)
     worst = int(np.argmax(err))
-    say(f"[adjoint-full] 64 params vs FD: max rel err {err.max():.2e} (param {params[worst]}, gpu {gpu[worst]:.6g}, "
-        f"fd {fd[worst]:.6g}); median {np.median(err):.2e}")
+    say(f"[adjoint-full] 64 params vs FD (eps 1e-4): max rel err {err.max():.2e} (param {params[worst]}, "
+        f"gpu {gpu[worst]:.6g}, fd {fd4[worst]:.6g}); median {np.median(err):.2e}")
+    say(f"[adjoint-full] eps 1e-3 converged on {int(conv.sum())} of 64 params; there the max rel err is "
+        f"{err3[conv].max():.2e}; unconverged: "
+        f"{[(params[i], round(float(fd3[i]), 4), round(float(fd4[i]), 4)) for i in np.nonzero(~conv)[0]]}")
     assert err.max() <= 1e-3
+    assert conv.sum() >= 48 and err3[conv].max() <= 1e-3
 
 
 def test_adjoint_deep_paths_use_the_largest_lds_layout():

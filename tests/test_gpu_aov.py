@@ -8,7 +8,10 @@ at the top, 0.001-uv jitter from the pixel's PCG stream).
 
 Bar: every pixel within 2e-6 absolute for albedo / normal (the FP32 mean of
 a few equal values) and 1e-5 for depth (FP32 Moeller-Trumbore t on t <= 8);
-the primary ids equal the closed form's face on every pixel.
+the primary ids equal the closed form's face on every pixel.  Pixels whose
+ray passes within 1e-4 of a face edge in some sample (where FP32 and the
+closed form may legitimately pick different faces) are left out; they must be
+under 1% of the image.
 """
 import numpy as np
 import pytest
@@ -50,17 +53,20 @@ def test_gpu_aovs_match_closed_form(engine, width, chunk, integrator, rng_mode, 
           flush=True)
     aov, ids = render_box(w, h, spp, engine, width, chunk, integrator, rng_mode, seed)
     ex = aov_box.expected_sums(w, h, spp, rng_mode=rng_mode, seed=seed)
-    assert ex["margin"].min() > 1e-5
+    clear = ex["margin"] > 1e-4
+    print(f"[aov-box] {int((~clear).sum())} of {w * h} pixels graze an edge", flush=True)
+    assert clear.mean() > 0.99
     for k, tol in (("albedo", 2e-6), ("normal", 2e-6), ("depth", 1e-5)):
         got = aov[k][..., :3].astype(np.float64)
-        err = np.abs(got - ex[k] / spp).max()
+        err = np.abs(got - ex[k] / spp)[clear].max()
         print(f"[aov-box] {k}: max abs error {err:.3e}", flush=True)
         assert err <= tol, (k, err)
         assert np.all(aov[k][..., 3] == 1.0)
     e0 = aov_box.expected_sample(w, h, 0, rng_mode, seed)
     quad = np.where(ids == 0xFFFFFFFF, -1, ids.astype(np.int64) // 2)
     names = {0: "back", 1: "left", 2: "floor", 3: "ceil", 4: "near", -1: "right"}
-    assert np.array_equal(np.vectorize(names.get)(quad), e0["face"])
+    sel = e0["margin"] > 1e-4
+    assert np.array_equal(np.vectorize(names.get)(quad)[sel], e0["face"][sel])
 
 
 @pytest.mark.parametrize("engine,width,chunk", VARIANTS, ids=VARIANT_IDS)

@@ -93,29 +93,32 @@ def test_host_watch_reloads_a_rebuilt_library(tmp_path):
     """--watch: while the host renders, the library file it was started from
     changes (its modification time moves, as after a rebuild); the host
     reloads it between two frames and restarts the accumulation, so its image
-    equals a fresh run of the frames rendered after the reload."""
+    equals a fresh run of the frames rendered after the reload.  The host
+    holds frame 3 until a trigger file appears (--wait-at / --wait-file), and
+    the test moves the library's time before creating it, so the reload
+    happens before frame 3 however fast either side runs."""
     import os
-    import re
     import shutil
-    import time
     lib = tmp_path / "libquasi_hip.so"
     shutil.copyfile(ROOT / "quasi_amd" / "libquasi_hip.so", lib)
     common = ["--width", "96", "--height", "64", "--spp-per-render", "2", "--depth", "4", "--integrator", "nee",
               "--rng", "hashed", "--seed", "9", "--compression", "none"]
-    frames = 4000
+    frames, at = 8, 3
+    trigger = tmp_path / "go"
     watched = tmp_path / "watched.exr"
-    args = [str(HOST), str(lib), "--watch", "--render", str(frames), "--out", str(watched)] + common
+    args = [str(HOST), str(lib), "--watch", "--render", str(frames), "--out", str(watched),
+            "--wait-at", str(at), "--wait-file", str(trigger)] + common
     print("[host]", " ".join(args), flush=True)
     proc = subprocess.Popen(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     try:
         head = []
-        deadline = time.time() + 60
-        for line in proc.stdout:             # the host flushes this line as its frame loop starts
+        for line in proc.stdout:             # the host flushes this line when it holds frame `at`
             head.append(line)
-            if "Batch mode" in line or time.time() > deadline:
+            if f"Waiting before frame {at}" in line:
                 break
         st = os.stat(lib)
         os.utime(lib, ns=(st.st_atime_ns, st.st_mtime_ns + 10_000_000_000))
+        trigger.write_text("go")
         rest, err = proc.communicate(timeout=120)
         out = "".join(head) + rest
     finally:
@@ -124,9 +127,8 @@ def test_host_watch_reloads_a_rebuilt_library(tmp_path):
             proc.wait()
     print(out[-1500:], err[-1500:], flush=True)
     assert proc.returncode == 0
-    m = re.search(r"Plugin reloaded before frame (\d+) \(1 reloads, 1 ok, 0 failed", out)
-    assert m, "no reload seen (the run may have ended before the file changed)"
-    after = frames - int(m.group(1))
+    assert f"Plugin reloaded before frame {at} (1 reloads, 1 ok, 0 failed" in out
+    after = frames - at
     assert f"Saving EXR with AOVs ({after} samples)" in out
     fresh = tmp_path / "fresh.exr"
     run_host([str(lib), "--render", str(after), "--out", str(fresh)] + common)
