@@ -388,7 +388,8 @@ def main():
     ap.add_argument("--samples-per-pass", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=4)
     ap.add_argument("--block", type=int, choices=[64, 128, 256], default=256)
-    ap.add_argument("--chunk", type=int, choices=[1, 64, 256], default=1)
+    ap.add_argument("--chunk", type=int, choices=[1, 2, 64, 256], default=2,
+                    help="queue chunk: 1 per-lane fetch, 2 the same with the top BVH4 levels in LDS, 64 or 256")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")

@@ -96,11 +96,13 @@ typedef struct Q_render_params {
     uint32_t block_size;       /* wavefront traversal threads per block: 64, 128, 256 (0 = 256) */
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
-    uint32_t queue_chunk;      /* wavefront: 1 (0 = 1) = each idle lane takes its next ray from the
-                                  queue (persistent waves, refilled when 32 lanes are idle); 64 or
-                                  256 = each persistent wave takes that many rays at a time;
-                                  Q_QUEUE_LDS_TOP = as 1, with the top three BVH4 levels staged in
-                                  LDS (single scenes, width 4, block 128 or 256);
+    uint32_t queue_chunk;      /* wavefront: 1 = each idle lane takes its next ray from the queue
+                                  (persistent waves, refilled when 32 lanes are idle); 64 or 256 =
+                                  each persistent wave takes that many rays at a time;
+                                  Q_QUEUE_LDS_TOP = as 1, with the top BVH4 levels staged in LDS
+                                  (single scenes, width 4: four levels at block 256, three at 128;
+                                  batches fetch as 1); 0 = Q_QUEUE_LDS_TOP where a kernel has it,
+                                  else 1;
                                   Q_QUEUE_FLAT = no persistence: one thread per queue slot, the
                                   plain wavefront tracer (block 128 only) */
     uint32_t bvh_builder;      /* Q_BVH_BUILD_*: where Q_plugin_set_scene builds the BVH */

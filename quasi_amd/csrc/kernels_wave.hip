@@ -241,33 +241,44 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // scene's offset plus the record's), which the loads take as they are;
 // otherwise each lane holds 64-bit pointers to its scene's arrays.
 
-// TOP (single-scene BVH4 kernels): the top three levels of the tree (the
-// root, its internal children and theirs: at most 21 nodes, 2.7 KB) are
-// copied into LDS when the block starts, their child references to staged
-// nodes rewritten as kLdsRef + slot, and node steps on them read LDS instead
-// of global memory.  The kernel is held to the register budget of a full CU
-// (kMaxThreadsPerCU threads) so that the extra LDS path costs no occupancy.
-constexpr int kTopNodes = 21;
+// TOP (single-scene BVH4 kernels): the top levels of the tree (the root, its
+// internal children, theirs, ...: four levels, at most 85 nodes, 9.5 KB, in
+// 256-thread blocks; three levels, 21 nodes, in smaller ones) are copied into
+// LDS when the block starts, without the nodes' unused last float4, their
+// child references to staged nodes rewritten as kLdsRef + slot, and node steps
+// on them read LDS instead of global memory.  Every ray starts at the root, so
+// these few nodes take a share of all node requests that would otherwise
+// concentrate on a handful of L2 lines; the LDS copy takes that load off the
+// L2.  On the 1M soup at 1080p, extend + shadow time per 64-spp frame: 150.8 ms
+// without staging, 142.3 ms with three levels (block 128), 139.4 ms with four
+// (block 256; 5 resident blocks per CU, limited by LDS), 147.7 ms with four
+// levels and an 8-entry short stack (8 blocks per CU, more stack spills).
 constexpr int kLdsRef = 0x40000000;   // node references >= this are LDS slots (fewer than 2^30 nodes)
+constexpr int kTopFloat4 = 7;         // float4s of a BVH4 node that the traversal reads ([7] is unused)
+
+template <int BLOCK>
+constexpr int top_levels() { return BLOCK >= 256 ? 4 : 3; }
+template <int BLOCK>
+constexpr int top_nodes() { return top_levels<BLOCK>() == 4 ? 1 + 4 + 16 + 64 : 1 + 4 + 16; }
 
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32, bool TOP = false>
-__global__ __launch_bounds__(BLOCK, (TOP && WIDTH == 4 && !BATCH) ? kMaxThreadsPerCU / BLOCK : 1)
-void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
+__global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
     constexpr uint32_t kNodeShift = WIDTH == 4 ? 7u : 6u;   // 128-B BVH4 / 64-B BVH2 nodes
     constexpr bool kTop = TOP && WIDTH == 4 && !BATCH;
+    constexpr int kTopNodes = top_nodes<BLOCK>();
     __shared__ int s_stack[kShortStack * BLOCK];
-    __shared__ float4 s_top[kTop ? kTopNodes * 8 : 1];
+    __shared__ float4 s_top[kTop ? kTopNodes * kTopFloat4 : 1];
     __shared__ int s_top_map[kTop ? kTopNodes : 1];
     __shared__ int s_top_n;
     if (kTop) {
         const float4* gn = static_cast<const float4*>(P.nodes4);
         if (threadIdx.x == 0) {
-            // breadth first over three levels: slot -> global node index
+            // breadth first over the top levels: slot -> global node index
             int n = 0;
             if (P.scene0.num_nodes != 0u) {
                 s_top_map[n++] = 0;
                 int lo = 0, hi = 1;
-                for (int level = 1; level < 3; ++level) {
+                for (int level = 1; level < top_levels<BLOCK>(); ++level) {
                     for (int k = lo; k < hi; ++k) {
                         const float4 chf = gn[size_t(s_top_map[k]) * 8 + 6];
                         const int c[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
@@ -283,11 +294,12 @@ void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
         }
         __syncthreads();
         const int n = s_top_n;
-        for (int i = int(threadIdx.x); i < n * 8; i += BLOCK) s_top[i] = gn[size_t(s_top_map[i >> 3]) * 8 + (i & 7)];
+        for (int i = int(threadIdx.x); i < n * kTopFloat4; i += BLOCK)
+            s_top[i] = gn[size_t(s_top_map[i / kTopFloat4]) * 8 + (i % kTopFloat4)];
         __syncthreads();
         // child references to staged nodes point at their slots
         for (int i = int(threadIdx.x); i < n * 4; i += BLOCK) {
-            int* ref = reinterpret_cast<int*>(&s_top[(i >> 2) * 8 + 6]) + (i & 3);
+            int* ref = reinterpret_cast<int*>(&s_top[(i >> 2) * kTopFloat4 + 6]) + (i & 3);
             const int c = *ref;
             if (c < 0 || c == kEmptyChild) continue;
             for (int m = 1; m < n; ++m)
@@ -392,7 +404,7 @@ void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
                 // values slab() takes the min and max of, without the min/max
                 float4 ex, fx, ey, fy, ez, fz, chf;
                 if (kTop && node >= kLdsRef) {
-                    const float4* q = s_top + (node - kLdsRef) * 8;
+                    const float4* q = s_top + (node - kLdsRef) * kTopFloat4;
                     ex = q[sel_x >> 4]; fx = q[(sel_x ^ 16u) >> 4];
                     ey = q[sel_y >> 4]; fy = q[(sel_y ^ 16u) >> 4];
                     ez = q[sel_z >> 4]; fz = q[(sel_z ^ 16u) >> 4];

@@ -748,7 +748,7 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
         hipDeviceProp_t prop{};
         const bool have_prop = hipGetDeviceProperties(&prop, dev) == hipSuccess;
         std::snprintf(st->arch, sizeof st->arch, "%s", have_prop ? prop.gcnArchName : "unknown");
-        st->trace_cfg = TraceConfig{4, 256, 1};
+        st->trace_cfg = TraceConfig{4, 256, kChunkLdsTop};
         st->trace_blocks = persistent_blocks(dev, st->trace_cfg);
         st->paths_blocks = paths_pass_blocks(dev, st->trace_cfg);
         if (ctx->gpu->queue) {
@@ -938,7 +938,11 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
             return Q_EXT_ERR_UNSUPPORTED;
         }
         if (p.bvh_width == 0) p.bvh_width = 4;
-        const TraceConfig cfg{p.bvh_width, p.block_size ? p.block_size : 256u, p.queue_chunk ? p.queue_chunk : 1u};
+        const uint32_t block = p.block_size ? p.block_size : 256u;
+        // the default fetch: per-lane, with the top BVH4 levels in LDS where a
+        // kernel has them (measured fastest on the 1M soup), else per-lane
+        TraceConfig cfg{p.bvh_width, block, p.queue_chunk ? p.queue_chunk : kChunkLdsTop};
+        if (!p.queue_chunk && !trace_config_supported(cfg)) cfg.chunk = 1u;
         if (!trace_config_supported(cfg)) {
             log_msg(st, "set_params: no traversal kernel for width %u, block %u, chunk %u", cfg.width, cfg.block,
                     cfg.chunk);
