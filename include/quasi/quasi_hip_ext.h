@@ -334,6 +334,35 @@ typedef struct Q_adjoint_result {
 
 int32_t Q_plugin_adjoint(Q_plugin_handle* handle, const Q_adjoint_desc* desc, Q_adjoint_result* result);
 
+/* ---------------------------------------------------------------- no-alloc render
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the wavefront render of Q_plugin_render with every piece of
+ * working memory (path state, ray queues, queue counters, traversal-stack
+ * overflow) in one fixed device array of Q_NOALLOC_SCRATCH_BYTES per device,
+ * set up by Q_plugin_create.  The call allocates nothing.
+ *
+ * Adds samples sample_begin .. sample_begin + spp - 1 of the current scene,
+ * seen from *camera at width x height (render parameters: those of
+ * Q_plugin_set_params; the engine is always the wavefront one), to the
+ * caller's device buffer `sums`: 4 planes of width * height RGBA32F sums,
+ * beauty, albedo, normal, depth (the Q_aov_type order), row-major
+ * top-to-bottom, each sample adding 1 to alpha -- what Q_plugin_render
+ * accumulates, before Q_plugin_readback divides by the sample count.
+ * Asynchronous on `stream` (a hipStream_t; NULL: the plugin's stream).
+ * Primary ids are not recorded and the render statistics are not changed.
+ *
+ * A pass holds samples_per_pass samples (0: as many as fit) of every pixel;
+ * Q_EXT_ERR_ARGUMENT when one pass of the image does not fit the array
+ * (2.1M to 2.8M paths, depending on the traversal-stack overflow the
+ * scene's BVH needs: 1920 x 1080 fits at 1 sample per pass).  With the same
+ * samples_per_pass the sums equal Q_plugin_render's bit for bit.  Every call
+ * on one device shares that device's array: calls that overlap on one device
+ * (two streams, two plugin instances) race. */
+#define Q_NOALLOC_SCRATCH_BYTES (512ull << 20)
+
+int32_t Q_plugin_render_noalloc(Q_plugin_handle* handle, const Q_camera* camera, uint32_t width, uint32_t height,
+                                uint32_t sample_begin, uint32_t spp, float* sums, void* stream);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -24,6 +24,7 @@ Q_EXT_ERR_STATE = -2
 Q_EXT_ERR_DEVICE = -3
 Q_EXT_ERR_OUT_OF_MEMORY = -4
 Q_EXT_ERR_UNSUPPORTED = -5
+Q_NOALLOC_SCRATCH_BYTES = 512 << 20
 Q_ADJOINT_PASS_PATHS = 1 << 24
 Q_ADJOINT_NO_PARAM = 0xFFFFFFFF
 

@@ -40,6 +40,7 @@ PLUGIN_SOURCES = [
     ("kernels_adjoint.hip", DEVICE),
     ("bvh_gpu.hip", DEVICE),
     ("bvh_layout.hip", DEVICE),
+    ("plugin_noalloc.hip", DEVICE),
     ("plugin.cpp", HOSTONLY),
     ("plugin_batch.cpp", HOSTONLY),
     ("plugin_adjoint.cpp", HOSTONLY),

@@ -760,6 +760,7 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
             }
             st->own_stream = true;
         }
+        if (!noalloc_init(st)) log_msg(st, "Q_plugin_render_noalloc is unavailable on device %d", dev);
         if (!ensure_buffer(st, st->counters, sizeof(Counters), "counters") ||
             !hip_ok(st, launch_fill_u32(st->counters.ptr, sizeof(Counters) / 4, 0u, st->stream), "counter clear")) {
             destroy_state(st);

@@ -137,6 +137,7 @@ struct plugin_state {
     bool scene_ok = false;              // the device scene is whole (false after a failed upload)
     Q_camera last_camera{};
     bool have_camera = false;
+    void* noalloc_scratch = nullptr;    // this device's working area of Q_plugin_render_noalloc
 
     // stats
     std::vector<hipEvent_t> event_pool;
@@ -188,6 +189,10 @@ CameraBasis make_camera(const Q_camera& c, uint32_t w, uint32_t h);
 
 // Mean of npix RGBA sums (divided by n samples) into a malloc'd array.
 Q_aov_buffer read_mean_of(plugin_state* st, const void* sums, uint32_t width, uint32_t height, uint32_t n);
+
+// Takes the address of this device's Q_plugin_render_noalloc working area
+// (st->device must be current); false when the runtime refuses.
+bool noalloc_init(plugin_state* st);
 
 void free_batch(plugin_state* st);
 void free_adjoint(plugin_state* st);

@@ -278,6 +278,9 @@ class HipRenderer:
         self._autotune = g("Q_plugin_autotune", C.c_int32, [_H, C.c_uint32, C.c_uint32, C.c_uint32,
                                                             P(abi.Q_autotune_result)])
         self._adjoint = g("Q_plugin_adjoint", C.c_int32, [_H, P(abi.Q_adjoint_desc), P(abi.Q_adjoint_result)])
+        self._render_noalloc = g("Q_plugin_render_noalloc", C.c_int32,
+                                 [_H, P(abi.Q_camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                  C.c_void_p])
 
     def _check(self, rc: int, what: str) -> None:
         if rc != abi.Q_EXT_OK:
@@ -362,6 +365,14 @@ class HipRenderer:
         res.struct_size = C.sizeof(abi.Q_adjoint_result)
         self._check(self._adjoint(self.plugin.handle, C.byref(d), C.byref(res)), "adjoint")
         return ga, gv, res
+
+    def render_noalloc(self, sums_ptr: int, width: int, height: int, spp: int, sample_begin: int = 0,
+                       camera: Optional[abi.Q_camera] = None, stream: Optional[int] = None) -> int:
+        """Q_plugin_render_noalloc into the device buffer at sums_ptr (4 planes of
+        width*height RGBA32F sums); returns the Q_EXT_* code."""
+        cam = self.camera if camera is None else camera
+        return int(self._render_noalloc(self.plugin.handle, C.byref(cam), width, height, sample_begin, spp,
+                                        C.c_void_p(sums_ptr), C.c_void_p(stream) if stream else None))
 
     def get_params(self) -> abi.Q_render_params:
         p = abi.Q_render_params()
