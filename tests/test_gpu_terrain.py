@@ -151,6 +151,13 @@ def test_adjoint_parameter_grid_full_size(terrain):
         gb, vb, rb = r.adjoint(g, spp, albedo_param=pmap, num_albedo_params=npar, max_pass_paths=1 << 20)
         print(f"[terrain-grid] 2^20-path passes: {rb.passes} passes in {rb.bands} bands, {rb.kernel_ms:.1f} ms "
               f"device, {rb.working_bytes / 2**30:.2f} GiB", flush=True)
+        ga, _, _ = r.adjoint(g, spp)
+    host = np.stack([np.bincount(group, weights=ga[:NT_TERRAIN, c], minlength=npar) for c in range(3)], axis=1)
+    del ga
+    err_map = float(np.abs(gp - host).sum() / np.abs(host).sum())
+    print(f"[terrain-grid] device reduction onto the grid vs per-triangle gradients summed on the host: "
+          f"rel-L1 {err_map:.2e}", flush=True)
+    assert err_map <= 1e-6
     assert (rb.bands, rb.passes) == (4, 4 * spp)
     assert rb.working_bytes < res.working_bytes
     lhs = float((theta32 * gp).sum())
