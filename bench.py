@@ -104,16 +104,17 @@ def cpu_baseline(mesh, args, seconds: float):
 
 
 def roofline_of(kernel: str, avg_ms: float, launches: int, alg_bytes: float) -> dict:
-    """Roofline of the dominant kernel.  The traversal is bound by VALU issue
-    and SIMD efficiency, so the roof is VALU lane throughput: lane-operations
-    per launch (VALU wave instructions x 64 x the active-lane fraction, read
-    by rocprofv3 on this workload and kept in profiles/) over the launch time
-    measured here.  Beside it: the L2-to-fabric bytes per launch (FETCH_SIZE
-    and WRITE_SIZE scaled by a copy kernel; HBM or Infinity Cache) against
-    the HBM peak, the L2 hit rate, the wave occupancy, and the algorithmic
-    bytes of the frozen traversal model (which count every node visit, so
-    their rate is not bounded by HBM: a cache-amplification figure, not a
-    roof)."""
+    """Roofline of the dominant kernel.  The roof is VALU lane throughput:
+    lane-operations per launch (VALU wave instructions x 64 x the active-lane
+    fraction, read by rocprofv3 on this workload and kept in profiles/) over
+    the launch time measured here.  Beside it: the texture addresser's busy
+    fraction and vector-memory instructions per launch (the node loads: the
+    unit that bound the 128-byte-node traversal), the L2-to-fabric bytes per
+    launch (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel; HBM or
+    Infinity Cache) against the HBM peak, the L2 hit rate, the wave
+    occupancy, and the algorithmic bytes of the frozen traversal model (which
+    count every node visit, so their rate is not bounded by HBM: a
+    cache-amplification figure, not a roof)."""
     sec = avg_ms * 1e-3
     out = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_TLANE, "unit": "T lane-op/s", "frac": None,
            "traffic": None, "kernel": kernel, "avg_launch_ms": round(avg_ms, 3), "launches": launches,
@@ -134,6 +135,7 @@ def roofline_of(kernel: str, avg_ms: float, launches: int, alg_bytes: float) -> 
                        "frac": round(traffic / sec / 1e9 / HBM_PEAK_GBS, 4),
                        "note": "FETCH_SIZE + WRITE_SIZE: L2-to-fabric bytes (HBM or Infinity Cache)"},
                l2_hit_rate=c.get("l2_hit_rate"), occupancy_waves_per_simd=c.get("occupancy_waves_per_simd"),
+               ta_busy_frac=c.get("ta_busy_frac"), vmem_insts_per_launch=c.get("vmem_insts_per_dispatch"),
                counters_source=str(COUNTERS.relative_to(ROOT)))
     return out
 
@@ -387,9 +389,10 @@ def main():
     ap.add_argument("--engine", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--samples-per-pass", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, choices=[2, 4], default=4)
-    ap.add_argument("--block", type=int, choices=[64, 128, 256], default=256)
-    ap.add_argument("--chunk", type=int, choices=[1, 2, 64, 256], default=2,
-                    help="queue chunk: 1 per-lane fetch, 2 the same with the top BVH4 levels in LDS, 64 or 256")
+    ap.add_argument("--block", type=int, choices=[64, 128, 256], default=128)
+    ap.add_argument("--chunk", type=int, choices=[1, 2, 3, 64, 256], default=3,
+                    help="queue chunk: 1 per-lane fetch, 2 the same with the top BVH4 levels in LDS, 3 that over "
+                         "quantised 64-byte BVH4 nodes, 64 or 256")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")

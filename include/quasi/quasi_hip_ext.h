@@ -93,7 +93,8 @@ typedef struct Q_render_params {
     uint32_t samples_per_pass; /* wavefront: samples per pixel in flight at once (0 = auto: about
                                   2^27 paths per pass; for a batch, the largest power of two that
                                   divides the samples requested and stays within that) */
-    uint32_t block_size;       /* wavefront traversal threads per block: 64, 128, 256 (0 = 256) */
+    uint32_t block_size;       /* wavefront traversal threads per block: 64, 128, 256 (0 = 128 for
+                                  bvh_width 4, 256 for bvh_width 2) */
     uint32_t flags;            /* Q_FLAG_* */
     uint32_t id_sample;        /* sample index whose primary ids are recorded */
     uint32_t queue_chunk;      /* wavefront: 1 = each idle lane takes its next ray from the queue
@@ -101,8 +102,13 @@ typedef struct Q_render_params {
                                   each persistent wave takes that many rays at a time;
                                   Q_QUEUE_LDS_TOP = as 1, with the top BVH4 levels staged in LDS
                                   (single scenes, width 4: four levels at block 256, three at 128;
-                                  batches fetch as 1); 0 = Q_QUEUE_LDS_TOP where a kernel has it,
-                                  else 1;
+                                  batches fetch as 1);
+                                  Q_QUEUE_QUANT = as Q_QUEUE_LDS_TOP over 64-byte BVH4 nodes whose
+                                  child boxes are quantised to 8 bits per plane against the node's
+                                  box (four loads per node step instead of seven; batches too,
+                                  without the LDS staging; the same images);
+                                  0 = Q_QUEUE_QUANT where a kernel has it (width 4, block 128 or
+                                  256), else Q_QUEUE_LDS_TOP, else 1;
                                   Q_QUEUE_FLAT = no persistence: one thread per queue slot, the
                                   plain wavefront tracer (block 128 only) */
     uint32_t bvh_builder;      /* Q_BVH_BUILD_*: where Q_plugin_set_scene builds the BVH */
@@ -115,6 +121,7 @@ typedef struct Q_render_params {
 /* Q_render_params.queue_chunk: the plain (non-persistent) wavefront traversal */
 #define Q_QUEUE_FLAT 0xFFFFFFFFu
 #define Q_QUEUE_LDS_TOP 2u
+#define Q_QUEUE_QUANT 3u
 
 #define Q_BVH_BUILD_AUTO 0u   /* on the GPU from 2^14 triangles, else on the host */
 #define Q_BVH_BUILD_HOST 1u   /* host threads */

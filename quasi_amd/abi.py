@@ -96,6 +96,7 @@ class Q_render_params(C.Structure):
 Q_BVH_BUILD_AUTO, Q_BVH_BUILD_HOST, Q_BVH_BUILD_GPU = 0, 1, 2
 Q_QUEUE_FLAT = 0xFFFFFFFF   # queue_chunk: the plain (non-persistent) wavefront traversal
 Q_QUEUE_LDS_TOP = 2         # queue_chunk: per-lane fetch, top BVH4 levels staged in LDS
+Q_QUEUE_QUANT = 3           # queue_chunk: as Q_QUEUE_LDS_TOP over quantised 64-byte BVH4 nodes
 
 
 class Q_kernel_stat(C.Structure):

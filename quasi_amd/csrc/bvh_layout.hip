@@ -21,14 +21,19 @@
 //     its leaves' triangle order are written as soon as its numbers are known;
 //  4. one pass over the internal nodes writes the 4-wide nodes.
 // The host reads back the level sizes once, between steps 1 and 2.
+//
+// k_quantize4 derives the 64-byte quantised BVH4 nodes (bvh_quant.hpp) from
+// the 128-byte ones, whichever builder made them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <vector>
 
 #include "bvh_gpu.hpp"
+#include "bvh_quant.hpp"
 #include "common.hpp"
 #include "device_buffer.hpp"
+#include "kernels.hpp"
 
 namespace quasi_hip {
 
@@ -338,7 +343,21 @@ uint32_t grid_for(size_t items) {
         if ((x) != hipSuccess) { *why = "HIP call"; return false; } \
     } while (0)
 
+// One thread per BVH4 node: its quantised form (bvh_quant.hpp).
+__global__ void k_quantize4(const float4* __restrict__ n4, uint32_t n, float4* __restrict__ q, uint32_t* failed) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (!quantize_node4(n4 + size_t(i) * 8, q + size_t(i) * 4)) *failed = 1u;
+}
+
 }  // namespace
+
+hipError_t launch_quantize4(const void* nodes4, uint32_t n, void* q, uint32_t* failed, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_quantize4, dim3(blocks), dim3(256), 0, stream, static_cast<const float4*>(nodes4), n,
+                       static_cast<float4*>(q), failed);
+    return hipGetLastError();
+}
 
 void DeviceLayout::release() {
     if (nodes) (void)hipFree(nodes);

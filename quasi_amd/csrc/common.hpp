@@ -26,6 +26,8 @@
 //                    kEmptyChild marks an unused slot, whose box is
 //                    lo = +inf, hi = -inf so that every slab test rejects it)
 //              [7] = unused
+//   qnodes4: the same BVH4 nodes (same indices) in 64 B each, child boxes
+//            quantised to 8 bits against the node's box (bvh_quant.hpp)
 //   lights : one LightEntry per emissive triangle
 #pragma once
 
@@ -115,6 +117,7 @@ struct SceneDesc {
     const void* mats;
     const void* nodes;
     const void* nodes4;
+    const void* qnodes4;   // the BVH4 as quantised 64-B nodes (bvh_quant.hpp)
     const LightEntry* lights;
     uint32_t num_nodes, num_lights;
     CameraBasis cam;
@@ -150,6 +153,7 @@ struct PassParams {
     // area of the persistent kernels (gstack_entries words per thread,
     // lane-interleaved; sized by the host for the deepest possible stack)
     const void* nodes4;
+    const void* qnodes4;   // the same BVH4 as quantised 64-B nodes (bvh_quant.hpp)
     uint32_t num_nodes4;
     uint32_t gstack_entries;
     int* gstack;
@@ -192,9 +196,16 @@ struct TraceConfig {
 };
 constexpr uint32_t kChunkFlat = 0xFFFFFFFFu;
 constexpr uint32_t kChunkLdsTop = 2;  // per-lane fetch, top BVH4 levels staged in LDS (single scenes, width 4)
+constexpr uint32_t kChunkQuant = 3;   // per-lane fetch over the quantised 64-B BVH4 nodes (bvh_quant.hpp),
+                                      // top levels staged in LDS for single scenes (width 4)
 constexpr int kFlatStack = 48;        // LDS stack entries per thread of the non-persistent traversal
 
 constexpr int kShortStack = 16;       // LDS entries per lane in the persistent kernels
+
+// Widening of the far slab distance so that rounding in (lo - o) * (1/d) never
+// culls a box that the ray touches (relative error of each slab distance is a
+// few ulps).
+constexpr float kSlabWiden = 1.000001f;
 
 // Paths in flight per automatically sized wavefront pass: about 25 GB of path
 // state (188 B per path), large enough that the short queues of the last

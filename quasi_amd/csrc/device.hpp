@@ -88,10 +88,6 @@ struct TraceHit {
     uint32_t id;    // input-order index
 };
 
-// Widening of the far slab distance so that rounding in (lo - o) * (1/d) never
-// culls a box that the ray touches (relative error of each slab distance is a
-// few ulps).
-constexpr float kSlabWiden = 1.000001f;
 
 // Traversal stacks.  Entry i of a lane lives at base[i * stride], so the 64
 // lanes of a wave touch 64 consecutive words (distinct LDS banks, coalesced

@@ -107,4 +107,9 @@ constexpr uint32_t kGpuBuildMinTriangles = 1u << 14;   // the GPU build is as fa
 // clears stay valid inside captured graphs).
 hipError_t launch_fill_u32(void* ptr, size_t n, uint32_t value, hipStream_t stream);
 
+// Writes the quantised 64-byte form (bvh_quant.hpp) of n 128-byte BVH4 nodes
+// (nodes4) into q, and sets *failed (a device word the caller cleared) to 1
+// when some node's extent is beyond the quantised range.
+hipError_t launch_quantize4(const void* nodes4, uint32_t n, void* q, uint32_t* failed, hipStream_t stream);
+
 }  // namespace quasi_hip

@@ -18,6 +18,7 @@
 // pass; each ray carries its scene index and every kernel reads the scene's
 // geometry, camera and framebuffer offsets from the scene table.
 #include "kernels.hpp"
+#include "bvh_quant.hpp"
 #include "wave_common.hpp"
 
 namespace quasi_hip {
@@ -253,6 +254,14 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // without staging, 142.3 ms with three levels (block 128), 139.4 ms with four
 // (block 256; 5 resident blocks per CU, limited by LDS), 147.7 ms with four
 // levels and an 8-entry short stack (8 blocks per CU, more stack spills).
+//
+// QUANT (BVH4, single scenes and batches): the nodes are the 64-byte quantised
+// ones of bvh_quant.hpp (P.qnodes4, a scene's SceneDesc::qnodes4), read with
+// four 16-byte loads per node step instead of seven; staged top levels take
+// 4 float4 per node.  The kernel is bound by the texture addresser's per-lane
+// cache-line lookups (busy 86% of the extend launch on the 1M soup with
+// 128-byte nodes), so the loads, not the VALU work, set the node step's cost:
+// extend + shadow per 64-spp frame 106.2 ms against 139.2 ms, same images.
 constexpr int kLdsRef = 0x40000000;   // node references >= this are LDS slots (fewer than 2^30 nodes)
 constexpr int kTopFloat4 = 7;         // float4s of a BVH4 node that the traversal reads ([7] is unused)
 
@@ -261,17 +270,22 @@ constexpr int top_levels() { return BLOCK >= 256 ? 4 : 3; }
 template <int BLOCK>
 constexpr int top_nodes() { return top_levels<BLOCK>() == 4 ? 1 + 4 + 16 + 64 : 1 + 4 + 16; }
 
-template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32, bool TOP = false>
+template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32, bool TOP = false, bool QUANT = false>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
-    constexpr uint32_t kNodeShift = WIDTH == 4 ? 7u : 6u;   // 128-B BVH4 / 64-B BVH2 nodes
+    constexpr bool kQuant = QUANT && WIDTH == 4;
+    // 128-B BVH4 / 64-B quantised BVH4 / 64-B BVH2 nodes
+    constexpr uint32_t kNodeShift = WIDTH == 4 && !kQuant ? 7u : 6u;
     constexpr bool kTop = TOP && WIDTH == 4 && !BATCH;
     constexpr int kTopNodes = top_nodes<BLOCK>();
+    constexpr int kTopF4 = kQuant ? 4 : kTopFloat4;   // float4 staged per node
+    constexpr int kRefF4 = kQuant ? 3 : 6;            // the node's float4 of child references
+    constexpr int kStrideF4 = kQuant ? 4 : 8;         // node size in float4
     __shared__ int s_stack[kShortStack * BLOCK];
-    __shared__ float4 s_top[kTop ? kTopNodes * kTopFloat4 : 1];
+    __shared__ float4 s_top[kTop ? kTopNodes * kTopF4 : 1];
     __shared__ int s_top_map[kTop ? kTopNodes : 1];
     __shared__ int s_top_n;
     if (kTop) {
-        const float4* gn = static_cast<const float4*>(P.nodes4);
+        const float4* gn = static_cast<const float4*>(kQuant ? P.qnodes4 : P.nodes4);
         if (threadIdx.x == 0) {
             // breadth first over the top levels: slot -> global node index
             int n = 0;
@@ -280,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 int lo = 0, hi = 1;
                 for (int level = 1; level < top_levels<BLOCK>(); ++level) {
                     for (int k = lo; k < hi; ++k) {
-                        const float4 chf = gn[size_t(s_top_map[k]) * 8 + 6];
+                        const float4 chf = gn[size_t(s_top_map[k]) * kStrideF4 + kRefF4];
                         const int c[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
                                           __float_as_int(chf.w)};
                         for (int j = 0; j < 4; ++j)
@@ -294,12 +308,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
         }
         __syncthreads();
         const int n = s_top_n;
-        for (int i = int(threadIdx.x); i < n * kTopFloat4; i += BLOCK)
-            s_top[i] = gn[size_t(s_top_map[i / kTopFloat4]) * 8 + (i % kTopFloat4)];
+        for (int i = int(threadIdx.x); i < n * kTopF4; i += BLOCK)
+            s_top[i] = gn[size_t(s_top_map[i / kTopF4]) * kStrideF4 + (i % kTopF4)];
         __syncthreads();
         // child references to staged nodes point at their slots
         for (int i = int(threadIdx.x); i < n * 4; i += BLOCK) {
-            int* ref = reinterpret_cast<int*>(&s_top[(i >> 2) * kTopFloat4 + 6]) + (i & 3);
+            int* ref = reinterpret_cast<int*>(&s_top[(i >> 2) * kTopF4 + kRefF4]) + (i & 3);
             const int c = *ref;
             if (c < 0 || c == kEmptyChild) continue;
             for (int m = 1; m < n; ++m)
@@ -333,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     // the direction component is negative); the exit plane is at offset ^ 16
     uint32_t sel_x = 0, sel_y = 32, sel_z = 64;
     // base of the node and triangle arrays (a batch: the concatenated ones)
-    const char* const nbase = static_cast<const char*>(WIDTH == 4 ? P.nodes4 : P.nodes);
+    const char* const nbase = static_cast<const char*>(kQuant ? P.qnodes4 : (WIDTH == 4 ? P.nodes4 : P.nodes));
     const char* const tbase = static_cast<const char*>(P.tris);
     uint32_t nofs = 0, tofs = 0;                    // OFF32 batch: the lane's scene's byte offsets
     const char* nodes = BATCH ? nullptr : nbase;    // otherwise: the lane's scene's arrays
@@ -358,7 +372,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 if (BATCH) {
-                    const char* sn = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
+                    const char* sn = static_cast<const char*>(kQuant ? sc.qnodes4 : scene_nodes<WIDTH, true>(P, sid));
                     const char* stt = static_cast<const char*>(sc.tris);
                     if (OFF32) {
                         nofs = uint32_t(sn - nbase);
@@ -399,7 +413,49 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 return OFF32 ? *reinterpret_cast<const float4*>(nbase + (off + o))
                              : *reinterpret_cast<const float4*>(rec + o);
             };
-            if (WIDTH == 4) {
+            if (kQuant) {
+                float4 q0, q1, q2, q3;
+                if (kTop && node >= kLdsRef) {
+                    const float4* q = s_top + (node - kLdsRef) * 4;
+                    q0 = q[0]; q1 = q[1]; q2 = q[2]; q3 = q[3];
+                } else {
+                    q0 = ld(0u); q1 = ld(16u); q2 = ld(32u); q3 = ld(48u);
+                }
+                float tn[4], tf[4];
+                slab_quant4(q0, q1, q2, r.o.x, r.o.y, r.o.z, inv.x, inv.y, inv.z, tmin, best_t, tn, tf);
+                const int4 ch = make_int4(__float_as_int(q3.x), __float_as_int(q3.y), __float_as_int(q3.z),
+                                          __float_as_int(q3.w));
+                // unused slots are rejected by their reference (slot 1 is unused
+                // only in the root of a one-triangle scene)
+                constexpr float kMissKey = 3.0e38f;
+                float k0 = tn[0] <= tf[0] ? tn[0] : kMissKey;
+                float k1 = tn[1] <= tf[1] && ch.y != kEmptyChild ? tn[1] : kMissKey;
+                float k2 = tn[2] <= tf[2] && ch.z != kEmptyChild ? tn[2] : kMissKey;
+                float k3 = tn[3] <= tf[3] && ch.w != kEmptyChild ? tn[3] : kMissKey;
+                int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+                cswap(k0, c0, k1, c1);
+                cswap(k2, c2, k3, c3);
+                cswap(k0, c0, k2, c2);
+                cswap(k1, c1, k3, c3);
+                cswap(k1, c1, k2, c2);
+                if (k0 == kMissKey) {
+                    node = st.empty() ? kDone : st.pop();
+                } else {
+                    const int np = int(k1 != kMissKey) + int(k2 != kMissKey) + int(k3 != kMissKey);
+                    if (st.sp + 3 <= kShortStack) {
+                        lds_int_ptr top = st.lds + st.sp * BLOCK;
+                        top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
+                        top[BLOCK] = np == 3 ? c2 : c1;
+                        top[2 * BLOCK] = c1;
+                        st.sp += np;
+                    } else {
+                        if (k3 != kMissKey) st.push(c3);
+                        if (k2 != kMissKey) st.push(c2);
+                        if (k1 != kMissKey) st.push(c1);
+                    }
+                    node = c0;
+                }
+            } else if (WIDTH == 4) {
                 // entry and exit planes chosen by the direction's signs: the same
                 // values slab() takes the min and max of, without the min/max
                 float4 ex, fx, ey, fy, ez, fz, chf;
@@ -622,8 +678,9 @@ int occupancy_of(K kernel, int block) {
 // LDS; CHUNK 0 (kChunkFlat in a TraceConfig): the non-persistent traversal.
 template <int WIDTH, int BLOCK, int CHUNK>
 struct TraceVariant {
-    static constexpr bool kDynamic = CHUNK == 1 || CHUNK == int(kChunkLdsTop);
-    static constexpr bool kTop = CHUNK == int(kChunkLdsTop);
+    static constexpr bool kDynamic = CHUNK == 1 || CHUNK == int(kChunkLdsTop) || CHUNK == int(kChunkQuant);
+    static constexpr bool kTop = CHUNK == int(kChunkLdsTop) || CHUNK == int(kChunkQuant);
+    static constexpr bool kQ = CHUNK == int(kChunkQuant);
     static constexpr int kStatic = kDynamic || CHUNK == 0 ? 64 : CHUNK;
     // one thread per queue slot; the stack must hold the deepest walk
     template <bool SHADOW, bool BATCH>
@@ -635,15 +692,15 @@ struct TraceVariant {
     }
     // node and triangle arrays (a batch's concatenated ones) each under 4 GiB
     static bool off32(const PassParams& p) {
-        const uint64_t node_bytes = uint64_t(WIDTH == 4 ? p.num_nodes4 : p.num_nodes) * (WIDTH == 4 ? 128u : 64u);
+        const uint64_t node_bytes = uint64_t(WIDTH == 4 ? p.num_nodes4 : p.num_nodes) * (WIDTH == 4 && !kQ ? 128u : 64u);
         return node_bytes < (1ull << 32) && uint64_t(p.num_tris) * 48u < (1ull << 32);
     }
     template <bool SHADOW, bool BATCH>
     static void dynamic(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
         if (off32(p))
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, true, kTop>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, true, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         else
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, false, kTop>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, false, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
     }
     template <bool BATCH>
     static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
@@ -672,8 +729,8 @@ struct TraceVariant {
             a = occupancy_of(k_trace_flat<WIDTH, BLOCK, false, false>, BLOCK);
             c = occupancy_of(k_trace_flat<WIDTH, BLOCK, true, false>, BLOCK);
         } else if constexpr (kDynamic) {
-            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, false, true, kTop>, BLOCK);
-            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, false, true, kTop>, BLOCK);
+            a = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, false, false, true, kTop, kQ>, BLOCK);
+            c = occupancy_of(k_trace_dynamic<WIDTH, BLOCK, true, false, true, kTop, kQ>, BLOCK);
         } else {
             a = occupancy_of(k_extend<WIDTH, BLOCK, kStatic, false>, BLOCK);
             c = occupancy_of(k_shadow<WIDTH, BLOCK, kStatic, false>, BLOCK);
@@ -701,6 +758,7 @@ const VariantEntry kVariants[] = {
     QH_VARIANT(4, 64, 1),   QH_VARIANT(4, 128, 1),   QH_VARIANT(4, 256, 1),
     QH_VARIANT(2, 128, 0),  QH_VARIANT(4, 128, 0),
     QH_VARIANT(4, 128, 2),  QH_VARIANT(4, 256, 2),
+    QH_VARIANT(4, 128, 3),  QH_VARIANT(4, 256, 3),
 };
 #undef QH_VARIANT
 
@@ -737,7 +795,12 @@ uint32_t persistent_blocks(int device, const TraceConfig& c) {
 hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const TraceConfig& cfg,
                                  uint32_t trace_blocks, hipStream_t stream, WaveTimers* tm) {
     if (s.paths == 0) return hipSuccess;
-    const VariantEntry* v = find_variant(cfg);
+    // without quantised nodes (a scene beyond their range) the same fetch runs
+    // over the 128-byte nodes
+    TraceConfig c = cfg;
+    if (c.chunk == kChunkQuant && !p.qnodes4)
+        c.chunk = find_variant(TraceConfig{c.width, c.block, kChunkLdsTop}) ? kChunkLdsTop : 1u;
+    const VariantEntry* v = find_variant(c);
     if (!v || p.max_depth > kMaxDepth) return hipErrorInvalidValue;
     const bool batch = p.num_scenes > 0;
     const uint32_t gen_blocks = grid_for(s.paths, 256, 16384);

@@ -72,6 +72,24 @@ bool ensure_buffer(plugin_state* st, DeviceBuffer& b, size_t bytes, const char* 
     return true;
 }
 
+bool quantize_bvh4(plugin_state* st, const void* nodes4, uint32_t n, DeviceBuffer& q, bool& in_range) {
+    in_range = false;
+    if (!ensure_buffer(st, q, std::max<size_t>(64, 64 * size_t(n)), "quantised BVH4 nodes") ||
+        !ensure_buffer(st, st->quant_flag, 4, "quantisation flag"))
+        return false;
+    uint32_t failed = 0;
+    if (!hip_ok(st, launch_fill_u32(st->quant_flag.ptr, 1, 0u, st->stream), "quantisation flag clear") ||
+        !hip_ok(st, launch_quantize4(nodes4, n, q.ptr, static_cast<uint32_t*>(st->quant_flag.ptr), st->stream),
+                "BVH4 quantisation") ||
+        !hip_ok(st, hipMemcpyAsync(&failed, st->quant_flag.ptr, 4, hipMemcpyDeviceToHost, st->stream),
+                "quantisation flag read") ||
+        !hip_ok(st, hipStreamSynchronize(st->stream), "BVH4 quantisation"))
+        return false;
+    in_range = failed == 0;
+    if (!in_range) log_msg(st, "scene: BVH extents beyond the quantised nodes' range; tracing the 128-byte nodes");
+    return true;
+}
+
 hipEvent_t take_event(plugin_state* st) {
     if (!st->event_pool.empty()) {
         hipEvent_t e = st->event_pool.back();
@@ -420,6 +438,7 @@ int32_t upload_scene(plugin_state* st) {
     st->num_lights = uint32_t(sh.lights.size());
     st->num_nodes4 = sh.num_nodes4;
     st->bvh_depth4 = sh.depth4;
+    if (!quantize_bvh4(st, st->nodes4.ptr, sh.num_nodes4, st->qnodes4, st->quant_ok)) return Q_EXT_ERR_DEVICE;
     st->gstack_entries = gentries;
     ++st->scene_generation;
     st->bvh_builder = sh.builder;
@@ -471,6 +490,8 @@ void destroy_state(plugin_state* st) {
     free_buffer(st->mats);
     free_buffer(st->nodes);
     free_buffer(st->nodes4);
+    free_buffer(st->qnodes4);
+    free_buffer(st->quant_flag);
     free_buffer(st->lights);
     free_buffer(st->gstack);
     for (auto& b : st->accum) free_buffer(b);
@@ -652,6 +673,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
     p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
     p.id_sample = rp.id_sample;
     p.nodes4 = st->nodes4.ptr;
+    p.qnodes4 = st->quant_ok ? st->qnodes4.ptr : nullptr;
     p.num_nodes4 = st->num_nodes4;
     p.gstack = static_cast<int*>(st->gstack.ptr);
     p.gstack_entries = st->gstack_entries;
@@ -661,6 +683,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
         sd.mats = p.mats;
         sd.nodes = p.nodes;
         sd.nodes4 = p.nodes4;
+        sd.qnodes4 = p.qnodes4;
         sd.lights = p.lights;
         sd.num_nodes = p.num_nodes;
         sd.num_lights = p.num_lights;
@@ -748,7 +771,7 @@ Q_EXPORT Q_plugin_handle* Q_plugin_create(Q_plugin_context* ctx) {
         hipDeviceProp_t prop{};
         const bool have_prop = hipGetDeviceProperties(&prop, dev) == hipSuccess;
         std::snprintf(st->arch, sizeof st->arch, "%s", have_prop ? prop.gcnArchName : "unknown");
-        st->trace_cfg = TraceConfig{4, 256, kChunkLdsTop};
+        st->trace_cfg = TraceConfig{4, 128, kChunkQuant};
         st->trace_blocks = persistent_blocks(dev, st->trace_cfg);
         st->paths_blocks = paths_pass_blocks(dev, st->trace_cfg);
         if (ctx->gpu->queue) {
@@ -939,10 +962,12 @@ Q_EXPORT int32_t Q_plugin_set_params(Q_plugin_handle* handle, const Q_render_par
             return Q_EXT_ERR_UNSUPPORTED;
         }
         if (p.bvh_width == 0) p.bvh_width = 4;
-        const uint32_t block = p.block_size ? p.block_size : 256u;
-        // the default fetch: per-lane, with the top BVH4 levels in LDS where a
-        // kernel has them (measured fastest on the 1M soup), else per-lane
-        TraceConfig cfg{p.bvh_width, block, p.queue_chunk ? p.queue_chunk : kChunkLdsTop};
+        // the default traversal (measured fastest on the 1M soup): BVH4 in
+        // 128-thread blocks over the quantised nodes with the top levels in
+        // LDS; per-lane fetch where no such kernel exists (BVH2: 256 threads)
+        const uint32_t block = p.block_size ? p.block_size : (p.bvh_width == 4 ? 128u : 256u);
+        TraceConfig cfg{p.bvh_width, block, p.queue_chunk ? p.queue_chunk : kChunkQuant};
+        if (!p.queue_chunk && !trace_config_supported(cfg)) cfg.chunk = kChunkLdsTop;
         if (!p.queue_chunk && !trace_config_supported(cfg)) cfg.chunk = 1u;
         if (!trace_config_supported(cfg)) {
             log_msg(st, "set_params: no traversal kernel for width %u, block %u, chunk %u", cfg.width, cfg.block,

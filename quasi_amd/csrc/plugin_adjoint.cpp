@@ -154,6 +154,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     p.num_nodes = st->num_nodes;
     p.num_tris = st->num_tris;
     p.nodes4 = st->nodes4.ptr;
+    p.qnodes4 = st->quant_ok ? st->qnodes4.ptr : nullptr;
     p.num_nodes4 = st->num_nodes4;
     p.gstack = static_cast<int*>(st->gstack.ptr);
     p.gstack_entries = st->gstack_entries;
@@ -165,6 +166,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     sd.mats = p.mats;
     sd.nodes = p.nodes;
     sd.nodes4 = p.nodes4;
+    sd.qnodes4 = p.qnodes4;
     sd.lights = p.lights;
     sd.num_nodes = p.num_nodes;
     sd.num_lights = p.num_lights;

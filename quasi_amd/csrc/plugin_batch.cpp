@@ -36,8 +36,10 @@ void free_batch(plugin_state* st) {
     b.graph_exec = nullptr;
     b.graph = nullptr;
     b.graph_key = 0;
-    for (auto* x : {&b.tris, &b.mats, &b.nodes, &b.nodes4, &b.lights, &b.table, &b.ids, &b.resolved, &b.sample_begin})
+    for (auto* x : {&b.tris, &b.mats, &b.nodes, &b.nodes4, &b.qnodes4, &b.lights, &b.table, &b.ids, &b.resolved,
+                    &b.sample_begin})
         free_buffer(*x);
+    b.quant_ok = false;
     for (auto& a : b.accum) free_buffer(a);
     b.descs.clear();
     b.active = false;
@@ -102,6 +104,7 @@ PassParams batch_params(plugin_state* st, uint32_t spp_pass) {
     p.mats = b.mats.ptr;
     p.nodes = b.nodes.ptr;
     p.nodes4 = b.nodes4.ptr;
+    p.qnodes4 = b.quant_ok ? b.qnodes4.ptr : nullptr;
     p.lights = static_cast<const LightEntry*>(b.lights.ptr);
     p.num_tris = b.total_tris;
     p.num_nodes = b.total_nodes;
@@ -134,6 +137,7 @@ bool prepare_batch_pass(plugin_state* st, uint32_t spp_pass) {
     k.add(reinterpret_cast<uintptr_t>(st->wave.ray_o[0].ptr)); k.add(st->wave.capacity);
     k.add(reinterpret_cast<uintptr_t>(st->gstack.ptr));
     k.add(reinterpret_cast<uintptr_t>(b.table.ptr));
+    k.add(reinterpret_cast<uintptr_t>(b.qnodes4.ptr)); k.add(b.quant_ok);
     if (k.h == b.graph_key && b.graph_exec) return true;
 
     // scene table for this pass size
@@ -285,6 +289,7 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
         !ensure_buffer(st, b.mats, std::max<size_t>(16, mat_total * 4), "batch materials") ||
         !ensure_buffer(st, b.nodes, std::max<size_t>(64, node_total * 4), "batch BVH nodes") ||
         !ensure_buffer(st, b.nodes4, std::max<size_t>(128, node4_total * 4), "batch BVH4 nodes") ||
+        !ensure_buffer(st, b.qnodes4, std::max<size_t>(64, node4_total * 2), "batch quantised BVH4 nodes") ||
         !ensure_buffer(st, b.lights, std::max<size_t>(16, light_n * sizeof(LightEntry)), "batch lights") ||
         !ensure_buffer(st, b.table, n * sizeof(SceneDesc), "scene table") ||
         !ensure_buffer(st, b.ids, 4 * npix, "batch ids") ||
@@ -302,6 +307,7 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     char* mb = static_cast<char*>(b.mats.ptr);
     char* nb = static_cast<char*>(b.nodes.ptr);
     char* n4b = static_cast<char*>(b.nodes4.ptr);
+    char* q4b = static_cast<char*>(b.qnodes4.ptr);   // 64 B per node: 2 bytes per float of the 128-B nodes
     LightEntry* lb = static_cast<LightEntry*>(b.lights.ptr);
     for (uint32_t i = 0; i < n; ++i) {
         const SceneHost& h = hosts[i];
@@ -327,6 +333,7 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
         d.mats = mb + 4 * mo;
         d.nodes = nb + 4 * no;
         d.nodes4 = n4b + 4 * n4o;
+        d.qnodes4 = q4b + 2 * n4o;
         d.lights = lb + lo;
         d.num_nodes = h.num_nodes;
         d.num_lights = uint32_t(h.lights.size());
@@ -349,6 +356,12 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     b.total_tris = uint32_t(total_tris);
     b.total_nodes = uint32_t(node_total / 16);     // 16 floats per BVH2 pair node
     b.total_nodes4 = uint32_t(node4_total / 32);   // 32 floats per BVH4 node
+    // every scene's nodes keep scene-local child references, so the
+    // concatenated array is quantised node by node like a single scene's
+    if (!quantize_bvh4(st, b.nodes4.ptr, b.total_nodes4, b.qnodes4, b.quant_ok)) {
+        free_batch(st);
+        return Q_EXT_ERR_DEVICE;
+    }
     b.gstack_entries = std::max(gentries, st->gstack_entries);
     st->gstack_entries = b.gstack_entries;
     b.active = true;

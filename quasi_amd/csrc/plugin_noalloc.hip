@@ -205,6 +205,7 @@ Q_EXPORT int32_t Q_plugin_render_noalloc(Q_plugin_handle* handle, const Q_camera
         p.num_nodes = st->num_nodes;
         p.num_tris = st->num_tris;
         p.nodes4 = st->nodes4.ptr;
+        p.qnodes4 = st->quant_ok ? st->qnodes4.ptr : nullptr;
         p.num_nodes4 = st->num_nodes4;
         p.gstack = reinterpret_cast<int*>(base + L.gstack);
         p.gstack_entries = st->gstack_entries;
@@ -213,6 +214,7 @@ Q_EXPORT int32_t Q_plugin_render_noalloc(Q_plugin_handle* handle, const Q_camera
         sd.mats = p.mats;
         sd.nodes = p.nodes;
         sd.nodes4 = p.nodes4;
+        sd.qnodes4 = p.qnodes4;
         sd.lights = p.lights;
         sd.num_nodes = p.num_nodes;
         sd.num_lights = p.num_lights;

@@ -63,6 +63,8 @@ struct BatchState {
     bool active = false;
     std::vector<SceneDesc> descs;          // host copy with device pointers
     DeviceBuffer tris, mats, nodes, nodes4, lights, table;
+    DeviceBuffer qnodes4;                  // quantised BVH4 nodes of the concatenated arrays
+    bool quant_ok = false;                 // qnodes4 holds every scene's nodes
     DeviceBuffer accum[Q_AOV_COUNT], ids, resolved, sample_begin;
     uint32_t num_scenes = 0, total_pixels = 0, total_tris = 0;
     uint32_t total_nodes = 0, total_nodes4 = 0;   // BVH2 / BVH4 nodes of the concatenated arrays
@@ -109,6 +111,9 @@ struct plugin_state {
 
     // device scene
     DeviceBuffer tris, mats, nodes, nodes4, lights, gstack;
+    DeviceBuffer qnodes4;               // the BVH4 as quantised 64-B nodes (bvh_quant.hpp)
+    DeviceBuffer quant_flag;            // one word: set when a node is beyond the quantised range
+    bool quant_ok = false;              // qnodes4 holds the current scene's nodes
     uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
     uint32_t num_nodes4 = 0, bvh_depth4 = 0, gstack_entries = 0;
     double bvh_build_ms = 0;
@@ -125,7 +130,7 @@ struct plugin_state {
 
     // wavefront engine
     WaveBuffers wave;
-    TraceConfig trace_cfg{4, 256, 1};
+    TraceConfig trace_cfg{4, 128, kChunkQuant};
     uint32_t trace_blocks = 0;
     uint32_t paths_blocks = 0;          // grid of the single-launch pass kernel for trace_cfg (0: unsupported)
     hipEvent_t pending_start = nullptr;
@@ -150,6 +155,10 @@ void log_msg(plugin_state* st, const char* fmt, ...);
 bool hip_ok(plugin_state* st, hipError_t e, const char* what);
 void free_buffer(DeviceBuffer& b);
 bool ensure_buffer(plugin_state* st, DeviceBuffer& b, size_t bytes, const char* what);
+// Quantised 64-byte form (bvh_quant.hpp) of n device BVH4 nodes into q;
+// in_range is false when some node is beyond the quantised range (the 128-byte
+// nodes are then traced).  False on a device or allocation error.
+bool quantize_bvh4(plugin_state* st, const void* nodes4, uint32_t n, DeviceBuffer& q, bool& in_range);
 hipEvent_t take_event(plugin_state* st);
 void harvest_events(plugin_state* st);
 bool sync_render(plugin_state* st);

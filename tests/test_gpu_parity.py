@@ -242,13 +242,14 @@ def test_wavefront_pass_split_and_determinism():
 
 TRACE_CONFIGS = ([(w, b, c) for w in (2, 4) for b in (64, 128, 256) for c in (1, 64, 256)] +
                  [(w, 128, abi.Q_QUEUE_FLAT) for w in (2, 4)] +
-                 [(4, b, abi.Q_QUEUE_LDS_TOP) for b in (128, 256)])
+                 [(4, b, c) for c in (abi.Q_QUEUE_LDS_TOP, abi.Q_QUEUE_QUANT) for b in (128, 256)])
 
 
 def test_all_trace_configs_render_identically():
     """Every compiled traversal variant (BVH width 2/4, block 64/128/256, queue
-    chunk: per-lane fetch, 64, 256, the non-persistent form, and per-lane fetch
-    with the top BVH4 levels staged in LDS) yields the same
+    chunk: per-lane fetch, 64, 256, the non-persistent form, per-lane fetch
+    with the top BVH4 levels staged in LDS, and that over the quantised
+    64-byte nodes) yields the same
     image bit for bit: the closest hit is unique under the tie rule and
     occlusion does not depend on which blocker is found."""
     mesh = scenes.soup(150_000, seed=8)
