@@ -144,7 +144,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         // descend through internal nodes (postponed leaves, as k_trace_dynamic)
         for (;;) {
             const bool desc = busy && node >= 0;
-            if (__ballot(desc && leaf == 0) == 0ull) break;
+            if (leave_node_loop(desc && leaf == 0, busy && leaf != 0)) break;
             if (!desc) continue;
             const uint32_t off = nofs + (uint32_t(node) << kNodeShift);
             const char* rec = nodes + (OFF32 ? size_t(0) : (size_t(uint32_t(node)) << kNodeShift));

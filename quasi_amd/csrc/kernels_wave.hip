@@ -401,10 +401,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
         // descend through internal nodes.  A lane that reaches a leaf keeps it
         // in `leaf` and goes on descending from its stack while lanes of the
         // wave without a leaf are still looking for one (postponed leaves:
-        // lanes that would idle do node work instead)
+        // lanes that would idle do node work instead), until only a few lanes
+        // are still looking (leave_node_loop)
         for (;;) {
             const bool desc = busy && node >= 0;
-            if (__ballot(desc && leaf == 0) == 0ull) break;
+            if (leave_node_loop(desc && leaf == 0, busy && leaf != 0)) break;
             if (!desc) continue;
             // the node's record: base + 32-bit offset (OFF32) or a 64-bit address
             const uint32_t off = nofs + (uint32_t(node) << kNodeShift);

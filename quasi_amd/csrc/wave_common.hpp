@@ -64,6 +64,19 @@ __device__ __forceinline__ void add_shadow(float4* rad, const float4* sc_a, uint
 }
 
 constexpr uint32_t kRefillIdle = 32;   // measured best of 8..48 on the 1M soup (28-32 alike)
+
+// The node loop of the while-while traversal hands over to the leaf tests
+// once no more than this many lanes of the wave are still looking for a leaf
+// and some lane holds one (0 would wait for every lane): the lanes holding
+// leaves then test them instead of idling while a few lanes descend.  On the
+// 1M soup at 1080p (quantised nodes), ms of extend + shadow per 64-spp frame:
+// 106.5 at 0, 89.6 at 4, 86.3 at 8, 84.3-85.5 at 16, 86.7 at 24, 91.5 at 32,
+// 103.0 at 40, 130.3 at 64 (leaves as soon as any lane has one).
+constexpr uint32_t kLeafPhaseSearching = 16;
+__device__ __forceinline__ bool leave_node_loop(bool searching, bool holds_leaf) {
+    const uint32_t n = uint32_t(__popcll(__ballot(searching)));
+    return n == 0u || (n <= kLeafPhaseSearching && __ballot(holds_leaf) != 0ull);
+}
 constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
 
 }  // namespace
