@@ -38,9 +38,11 @@ __global__ void k_path_queue(WaveState S) {
     S.fetch[queue_counter(0, seg)] = 0u;
 }
 
-template <int WIDTH, int BLOCK, bool BATCH, bool OFF32>
+template <int WIDTH, int BLOCK, bool BATCH, bool OFF32, bool QUANT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_paths(PassParams P, WaveState S) {
-    constexpr uint32_t kNodeShift = WIDTH == 4 ? 7u : 6u;   // 128-B BVH4 / 64-B BVH2 nodes
+    constexpr bool kQuant = QUANT && WIDTH == 4;
+    // 128-B BVH4 / 64-B quantised BVH4 / 64-B BVH2 nodes
+    constexpr uint32_t kNodeShift = WIDTH == 4 && !kQuant ? 7u : 6u;
     __shared__ int s_stack[kShortStack * BLOCK];
     ShortStack<kShortStack, BLOCK> st{s_stack + threadIdx.x, P.gstack + (blockIdx.x * BLOCK + threadIdx.x),
                                       gridDim.x * BLOCK, 0};
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     uint32_t path = 0, sid = 0, bounce = 0;
     uint32_t n_cam = 0, n_ext = 0, n_shadow = 0;
     uint32_t sel_x = 0, sel_y = 32, sel_z = 64;
-    const char* const nbase = static_cast<const char*>(WIDTH == 4 ? P.nodes4 : P.nodes);
+    const char* const nbase = static_cast<const char*>(kQuant ? P.qnodes4 : (WIDTH == 4 ? P.nodes4 : P.nodes));
     const char* const tbase = static_cast<const char*>(P.tris);
     uint32_t nofs = 0, tofs = 0;
     const char* nodes = BATCH ? nullptr : nbase;
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 sid = BATCH ? scene_of_path(P, path) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 if (BATCH) {
-                    const char* sn = reinterpret_cast<const char*>(scene_nodes<WIDTH, true>(P, sid));
+                    const char* sn = static_cast<const char*>(kQuant ? sc.qnodes4 : scene_nodes<WIDTH, true>(P, sid));
                     const char* stt = static_cast<const char*>(sc.tris);
                     if (OFF32) {
                         nofs = uint32_t(sn - nbase);
@@ -152,7 +154,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 return OFF32 ? *reinterpret_cast<const float4*>(nbase + (off + o))
                              : *reinterpret_cast<const float4*>(rec + o);
             };
-            if (WIDTH == 4) {
+            if (kQuant) {
+                node = quant_node_step<BLOCK>(ld(0u), ld(16u), ld(32u), ld(48u), r, inv, tmin, best_t, st);
+            } else if (WIDTH == 4) {
                 const float4 ex = ld(sel_x), fx = ld(sel_x ^ 16u);
                 const float4 ey = ld(sel_y), fy = ld(sel_y ^ 16u);
                 const float4 ez = ld(sel_z), fz = ld(sel_z ^ 16u);
@@ -162,35 +166,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 slab_sel(ex.y, fx.y, ey.y, fy.y, ez.y, fz.y, r, inv, tmin, best_t, t1n, t1f);
                 slab_sel(ex.z, fx.z, ey.z, fy.z, ez.z, fz.z, r, inv, tmin, best_t, t2n, t2f);
                 slab_sel(ex.w, fx.w, ey.w, fy.w, ez.w, fz.w, r, inv, tmin, best_t, t3n, t3f);
-                constexpr float kMissKey = 3.0e38f;
-                float k0 = t0n <= t0f ? t0n : kMissKey;
-                float k1 = t1n <= t1f ? t1n : kMissKey;
-                float k2 = t2n <= t2f ? t2n : kMissKey;
-                float k3 = t3n <= t3f ? t3n : kMissKey;
-                int c0 = __float_as_int(chf.x), c1 = __float_as_int(chf.y), c2 = __float_as_int(chf.z),
-                    c3 = __float_as_int(chf.w);
-                cswap(k0, c0, k1, c1);
-                cswap(k2, c2, k3, c3);
-                cswap(k0, c0, k2, c2);
-                cswap(k1, c1, k3, c3);
-                cswap(k1, c1, k2, c2);
-                if (k0 == kMissKey) {
-                    node = st.empty() ? kDone : st.pop();
-                } else {
-                    const int np = int(k1 != kMissKey) + int(k2 != kMissKey) + int(k3 != kMissKey);
-                    if (st.sp + 3 <= kShortStack) {
-                        lds_int_ptr top = st.lds + st.sp * BLOCK;
-                        top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
-                        top[BLOCK] = np == 3 ? c2 : c1;
-                        top[2 * BLOCK] = c1;
-                        st.sp += np;
-                    } else {
-                        if (k3 != kMissKey) st.push(c3);
-                        if (k2 != kMissKey) st.push(c2);
-                        if (k1 != kMissKey) st.push(c1);
-                    }
-                    node = c0;
-                }
+                node = visit_children4<BLOCK>(t0n <= t0f ? t0n : kMissKey, t1n <= t1f ? t1n : kMissKey,
+                                              t2n <= t2f ? t2n : kMissKey, t3n <= t3f ? t3n : kMissKey,
+                                              __float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                                              __float_as_int(chf.w), st);
             } else {
                 const float4 a = ld(0u), bb = ld(16u), c = ld(32u), dd = ld(48u);
                 const int cl = __float_as_int(dd.x);
@@ -337,16 +316,22 @@ struct PathsVariant {
         const uint64_t node_bytes = uint64_t(WIDTH == 4 ? p.num_nodes4 : p.num_nodes) * (WIDTH == 4 ? 128u : 64u);
         return node_bytes < (1ull << 32) && uint64_t(p.num_tris) * 48u < (1ull << 32);
     }
+    // the quantised BVH4 nodes whenever the pass has them (bvh_quant.hpp)
+    template <bool BATCH, bool QUANT>
+    static void launch_q(const PassParams& p, const WaveState& s, uint32_t blocks, hipStream_t st) {
+        if (off32(p))
+            hipLaunchKernelGGL((k_paths<WIDTH, BLOCK, BATCH, true, QUANT>), dim3(blocks), dim3(BLOCK), 0, st, p, s);
+        else
+            hipLaunchKernelGGL((k_paths<WIDTH, BLOCK, BATCH, false, QUANT>), dim3(blocks), dim3(BLOCK), 0, st, p, s);
+    }
     template <bool BATCH>
     static void launch(const PassParams& p, const WaveState& s, uint32_t blocks, hipStream_t st) {
-        if (off32(p))
-            hipLaunchKernelGGL((k_paths<WIDTH, BLOCK, BATCH, true>), dim3(blocks), dim3(BLOCK), 0, st, p, s);
-        else
-            hipLaunchKernelGGL((k_paths<WIDTH, BLOCK, BATCH, false>), dim3(blocks), dim3(BLOCK), 0, st, p, s);
+        if (WIDTH == 4 && p.qnodes4) launch_q<BATCH, true>(p, s, blocks, st);
+        else launch_q<BATCH, false>(p, s, blocks, st);
     }
     static int occupancy() {
-        const int a = occupancy_of(k_paths<WIDTH, BLOCK, false, true>, BLOCK);
-        const int b = occupancy_of(k_paths<WIDTH, BLOCK, true, true>, BLOCK);
+        const int a = occupancy_of(k_paths<WIDTH, BLOCK, false, true, WIDTH == 4>, BLOCK);
+        const int b = occupancy_of(k_paths<WIDTH, BLOCK, true, true, WIDTH == 4>, BLOCK);
         return a < b ? a : b;
     }
 };

@@ -422,40 +422,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 } else {
                     q0 = ld(0u); q1 = ld(16u); q2 = ld(32u); q3 = ld(48u);
                 }
-                float tn[4], tf[4];
-                slab_quant4(q0, q1, q2, r.o.x, r.o.y, r.o.z, inv.x, inv.y, inv.z, tmin, best_t, tn, tf);
-                const int4 ch = make_int4(__float_as_int(q3.x), __float_as_int(q3.y), __float_as_int(q3.z),
-                                          __float_as_int(q3.w));
-                // unused slots are rejected by their reference (slot 1 is unused
-                // only in the root of a one-triangle scene)
-                constexpr float kMissKey = 3.0e38f;
-                float k0 = tn[0] <= tf[0] ? tn[0] : kMissKey;
-                float k1 = tn[1] <= tf[1] && ch.y != kEmptyChild ? tn[1] : kMissKey;
-                float k2 = tn[2] <= tf[2] && ch.z != kEmptyChild ? tn[2] : kMissKey;
-                float k3 = tn[3] <= tf[3] && ch.w != kEmptyChild ? tn[3] : kMissKey;
-                int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-                cswap(k0, c0, k1, c1);
-                cswap(k2, c2, k3, c3);
-                cswap(k0, c0, k2, c2);
-                cswap(k1, c1, k3, c3);
-                cswap(k1, c1, k2, c2);
-                if (k0 == kMissKey) {
-                    node = st.empty() ? kDone : st.pop();
-                } else {
-                    const int np = int(k1 != kMissKey) + int(k2 != kMissKey) + int(k3 != kMissKey);
-                    if (st.sp + 3 <= kShortStack) {
-                        lds_int_ptr top = st.lds + st.sp * BLOCK;
-                        top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
-                        top[BLOCK] = np == 3 ? c2 : c1;
-                        top[2 * BLOCK] = c1;
-                        st.sp += np;
-                    } else {
-                        if (k3 != kMissKey) st.push(c3);
-                        if (k2 != kMissKey) st.push(c2);
-                        if (k1 != kMissKey) st.push(c1);
-                    }
-                    node = c0;
-                }
+                node = quant_node_step<BLOCK>(q0, q1, q2, q3, r, inv, tmin, best_t, st);
             } else if (WIDTH == 4) {
                 // entry and exit planes chosen by the direction's signs: the same
                 // values slab() takes the min and max of, without the min/max
@@ -481,37 +448,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 slab_sel(ex.w, fx.w, ey.w, fy.w, ez.w, fz.w, r, inv, tmin, best_t, t3n, t3f);
                 // unused slots hold inverted boxes (lo = +inf, hi = -inf), which
                 // the sign-selected slab test always rejects
-                constexpr float kMissKey = 3.0e38f;
-                float k0 = t0n <= t0f ? t0n : kMissKey;
-                float k1 = t1n <= t1f ? t1n : kMissKey;
-                float k2 = t2n <= t2f ? t2n : kMissKey;
-                float k3 = t3n <= t3f ? t3n : kMissKey;
-                int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-                cswap(k0, c0, k1, c1);
-                cswap(k2, c2, k3, c3);
-                cswap(k0, c0, k2, c2);
-                cswap(k1, c1, k3, c3);
-                cswap(k1, c1, k2, c2);
-                if (k0 == kMissKey) {
-                    node = st.empty() ? kDone : st.pop();
-                } else {
-                    // push the hit children after the nearest, farthest first
-                    const int np = int(k1 != kMissKey) + int(k2 != kMissKey) + int(k3 != kMissKey);
-                    if (st.sp + 3 <= kShortStack) {
-                        // three unconditional LDS stores; slots above sp + np are
-                        // overwritten before they are read
-                        lds_int_ptr top = st.lds + st.sp * BLOCK;
-                        top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
-                        top[BLOCK] = np == 3 ? c2 : c1;
-                        top[2 * BLOCK] = c1;
-                        st.sp += np;
-                    } else {
-                        if (k3 != kMissKey) st.push(c3);
-                        if (k2 != kMissKey) st.push(c2);
-                        if (k1 != kMissKey) st.push(c1);
-                    }
-                    node = c0;
-                }
+                node = visit_children4<BLOCK>(t0n <= t0f ? t0n : kMissKey, t1n <= t1f ? t1n : kMissKey,
+                                              t2n <= t2f ? t2n : kMissKey, t3n <= t3f ? t3n : kMissKey, ch.x, ch.y,
+                                              ch.z, ch.w, st);
             } else {
                 const float4 a = ld(0u), bb = ld(16u), c = ld(32u), dd = ld(48u);
                 const int cl = __float_as_int(dd.x);

@@ -2,6 +2,7 @@
 // (kernels_wave.hip) and the single-launch path kernel (kernels_paths.hip).
 #pragma once
 
+#include "bvh_quant.hpp"
 #include "device.hpp"
 
 namespace quasi_hip {
@@ -78,6 +79,55 @@ __device__ __forceinline__ bool leave_node_loop(bool searching, bool holds_leaf)
     return n == 0u || (n <= kLeafPhaseSearching && __ballot(holds_leaf) != 0ull);
 }
 constexpr int kDone = int(0x80000000);   // no leaf reference has this value (fewer than 2^29 triangles)
+constexpr float kMissKey = 3.0e38f;      // sort key of a child the ray misses
+
+// The children of a BVH4 node in order of entry distance (keys; kMissKey for
+// a miss, references c): returns the nearest as the next node and pushes the
+// other hit ones, farthest first; with no hit, the top of the stack (kDone
+// when empty).
+template <int BLOCK, class Stack>
+__device__ __forceinline__ int visit_children4(float k0, float k1, float k2, float k3, int c0, int c1, int c2, int c3,
+                                               Stack& st) {
+    cswap(k0, c0, k1, c1);
+    cswap(k2, c2, k3, c3);
+    cswap(k0, c0, k2, c2);
+    cswap(k1, c1, k3, c3);
+    cswap(k1, c1, k2, c2);
+    if (k0 == kMissKey) return st.empty() ? kDone : st.pop();
+    // push the hit children after the nearest, farthest first
+    const int np = int(k1 != kMissKey) + int(k2 != kMissKey) + int(k3 != kMissKey);
+    if (st.sp + 3 <= kShortStack) {
+        // three unconditional LDS stores; slots above sp + np are overwritten
+        // before they are read
+        lds_int_ptr top = st.lds + st.sp * BLOCK;
+        top[0] = np == 3 ? c3 : (np == 2 ? c2 : c1);
+        top[BLOCK] = np == 3 ? c2 : c1;
+        top[2 * BLOCK] = c1;
+        st.sp += np;
+    } else {
+        if (k3 != kMissKey) st.push(c3);
+        if (k2 != kMissKey) st.push(c2);
+        if (k1 != kMissKey) st.push(c1);
+    }
+    return c0;
+}
+
+// Node step over a quantised BVH4 node (bvh_quant.hpp; q: its four float4):
+// the next node, with the other hit children pushed.  Unused slots are
+// rejected by their reference (slot 1 is unused only in the root of a
+// one-triangle scene).
+template <int BLOCK, class Stack>
+__device__ __forceinline__ int quant_node_step(const float4& q0, const float4& q1, const float4& q2, const float4& q3,
+                                               const Ray& r, const f3& inv, float tmin, float tmax, Stack& st) {
+    float tn[4], tf[4];
+    slab_quant4(q0, q1, q2, r.o.x, r.o.y, r.o.z, inv.x, inv.y, inv.z, tmin, tmax, tn, tf);
+    const int c0 = __float_as_int(q3.x), c1 = __float_as_int(q3.y), c2 = __float_as_int(q3.z),
+              c3 = __float_as_int(q3.w);
+    return visit_children4<BLOCK>(tn[0] <= tf[0] ? tn[0] : kMissKey,
+                                  tn[1] <= tf[1] && c1 != kEmptyChild ? tn[1] : kMissKey,
+                                  tn[2] <= tf[2] && c2 != kEmptyChild ? tn[2] : kMissKey,
+                                  tn[3] <= tf[3] && c3 != kEmptyChild ? tn[3] : kMissKey, c0, c1, c2, c3, st);
+}
 
 }  // namespace
 }  // namespace quasi_hip
