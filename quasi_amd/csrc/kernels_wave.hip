@@ -270,12 +270,66 @@ constexpr int top_levels() { return BLOCK >= 256 ? 4 : 3; }
 template <int BLOCK>
 constexpr int top_nodes() { return top_levels<BLOCK>() == 4 ? 1 + 4 + 16 + 64 : 1 + 4 + 16; }
 
+// TOP in a batch (quantised nodes): a block's rays belong to many scenes, so
+// each wave stages the top three levels (21 nodes, 1.3 KB) of one scene for
+// itself: slot 0 the root, slot 1 + k its child k, slot 5 + 4 k + j child j of
+// that child, where internal; child references to staged nodes rewritten as
+// kLdsRef + slot.  A wave restages, to the scene of the first ray it has just
+// fetched, only when none of its lanes is still walking the staged copy;
+// lanes of other scenes start at their scene's root in global memory.
+constexpr int kWaveTopNodes = 1 + 4 + 16;
+constexpr uint32_t kNoScene = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool internal_ref(int c) { return c >= 0 && c != kEmptyChild; }
+
+__device__ __forceinline__ int ref_of(const float4& v, int k) {
+    return __float_as_int(k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)));
+}
+
+// The whole wave stages scene sid's top levels into wtop (kWaveTopNodes x 4 float4).
+__device__ __forceinline__ void stage_wave_top(const PassParams& P, uint32_t sid, float4* wtop) {
+    const float4* q = static_cast<const float4*>(P.scenes[sid].qnodes4);
+    const uint32_t lane = lane_id();
+    const float4 root_refs = q[3];
+    int n = -1;   // the node this lane stages (slot = lane)
+    if (lane == 0u) {
+        n = 0;
+    } else if (lane <= 4u) {
+        const int c = ref_of(root_refs, int(lane) - 1);
+        n = internal_ref(c) ? c : -1;
+    } else if (lane < uint32_t(kWaveTopNodes)) {
+        const int p = ref_of(root_refs, int(lane - 5u) >> 2);
+        if (internal_ref(p)) {
+            const int c = ref_of(q[size_t(p) * 4 + 3], int(lane - 5u) & 3);
+            n = internal_ref(c) ? c : -1;
+        }
+    }
+    if (n >= 0) {
+        const float4* s = q + size_t(n) * 4;
+        const float4 q0 = s[0], q1 = s[1], q2 = s[2];
+        float4 q3 = s[3];
+        if (lane <= 4u) {
+            int c[4] = {__float_as_int(q3.x), __float_as_int(q3.y), __float_as_int(q3.z), __float_as_int(q3.w)};
+            for (int j = 0; j < 4; ++j)
+                if (internal_ref(c[j])) c[j] = kLdsRef + (lane == 0u ? 1 + j : 5 + 4 * int(lane - 1u) + j);
+            q3 = make_float4(__int_as_float(c[0]), __int_as_float(c[1]), __int_as_float(c[2]), __int_as_float(c[3]));
+        }
+        float4* d = wtop + lane * 4u;
+        d[0] = q0; d[1] = q1; d[2] = q2; d[3] = q3;
+    }
+    // the other lanes of the wave read these slots next
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32, bool TOP = false, bool QUANT = false>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
     constexpr bool kQuant = QUANT && WIDTH == 4;
     // 128-B BVH4 / 64-B quantised BVH4 / 64-B BVH2 nodes
     constexpr uint32_t kNodeShift = WIDTH == 4 && !kQuant ? 7u : 6u;
     constexpr bool kTop = TOP && WIDTH == 4 && !BATCH;
+    constexpr bool kWaveTop = TOP && kQuant && BATCH;   // per-wave staging in a batch
     constexpr int kTopNodes = top_nodes<BLOCK>();
     constexpr int kTopF4 = kQuant ? 4 : kTopFloat4;   // float4 staged per node
     constexpr int kRefF4 = kQuant ? 3 : 6;            // the node's float4 of child references
@@ -284,6 +338,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     __shared__ float4 s_top[kTop ? kTopNodes * kTopF4 : 1];
     __shared__ int s_top_map[kTop ? kTopNodes : 1];
     __shared__ int s_top_n;
+    __shared__ float4 s_wtop[kWaveTop ? (BLOCK / 64) * kWaveTopNodes * 4 : 1];
+    float4* const wtop = s_wtop + (kWaveTop ? (threadIdx.x >> 6) * kWaveTopNodes * 4 : 0);
+    uint32_t staged = kNoScene;   // wave-uniform: the scene in wtop
     if (kTop) {
         const float4* gn = static_cast<const float4*>(kQuant ? P.qnodes4 : P.nodes4);
         if (threadIdx.x == 0) {
@@ -342,6 +399,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     int node = kDone;
     int leaf = 0;   // postponed leaf reference (leaf references are negative; 0 = none)
     bool busy = false;
+    bool fresh = false;      // per-wave staging: the lane took its ray in this refill
+    bool on_top = false;     // ... and walks the wave's staged copy
+    uint32_t lane_sid = kNoScene;
     uint32_t idx = 0, path = 0;
     // byte offsets in a BVH4 node of each axis's entry plane (lo, or hi when
     // the direction component is negative); the exit plane is at offset ^ 16
@@ -357,6 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
         const unsigned long long idle = __ballot(!busy);
         const unsigned long long act = __ballot(busy);
         unsigned long long want = f.live() && (act == 0ull || uint32_t(__popcll(idle)) >= kRefillIdle) ? idle : 0ull;
+        fresh = false;
         while (want) {
             uint32_t first = 0;
             const uint32_t got = f.take(uint32_t(__popcll(want)), first);
@@ -365,6 +426,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             const bool mine = ((want >> (threadIdx.x & 63u)) & 1ull) && lb < got;
             want &= ~__ballot(mine);
             if (mine) {
+                fresh = true;
+                on_top = false;
                 idx = first + lb;
                 const float4 o = ro[idx];
                 const float4 d = rd[idx];
@@ -392,8 +455,27 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 best_id = 0u;
                 st.sp = 0;
                 node = sc.num_nodes ? (kTop ? kLdsRef : 0) : kDone;
+                lane_sid = sc.num_nodes ? sid : kNoScene;
                 leaf = 0;
                 busy = true;
+            }
+        }
+        if (kWaveTop) {
+            // restage when no lane walks the staged copy any more
+            if (__ballot(busy && on_top) == 0ull) {
+                const unsigned long long cand = __ballot(fresh && lane_sid != kNoScene);
+                if (cand) {
+                    const uint32_t want_sid =
+                        uint32_t(__builtin_amdgcn_readlane(int(lane_sid), int(__builtin_ctzll(cand))));
+                    if (want_sid != staged) {
+                        stage_wave_top(P, want_sid, wtop);
+                        staged = want_sid;
+                    }
+                }
+            }
+            if (fresh && lane_sid == staged && node == 0) {
+                node = kLdsRef;
+                on_top = true;
             }
         }
         // no lane busy after a refill: every segment is exhausted
@@ -416,8 +498,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             };
             if (kQuant) {
                 float4 q0, q1, q2, q3;
-                if (kTop && node >= kLdsRef) {
-                    const float4* q = s_top + (node - kLdsRef) * 4;
+                if ((kTop || kWaveTop) && node >= kLdsRef) {
+                    const float4* q = (kWaveTop ? wtop : s_top) + (node - kLdsRef) * 4;
                     q0 = q[0]; q1 = q[1]; q2 = q[2]; q3 = q[3];
                 } else {
                     q0 = ld(0u); q1 = ld(16u); q2 = ld(32u); q3 = ld(48u);
