@@ -38,12 +38,12 @@ PARAMS = dict(max_depth=6, rr_start=3, rng_mode=1, integrator=1, engine=abi.Q_EN
               bvh_width=4, block_size=128, queue_chunk=64, flags=abi.Q_FLAG_RECORD_IDS, id_sample=1)
 
 
-def plain_render(r, mesh, w, h, seed, spp):
+def plain_render(r, mesh, w, h, seed, spp, params=None):
     r.set_scene(mesh)
     r.resize(w, h)
     c = mesh.camera
     r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
-    r.set_params(spp_per_render=spp, seed=seed, **PARAMS)
+    r.set_params(spp_per_render=spp, seed=seed, **(params or PARAMS))
     r.render(1)
     return r.readback_aov(), r.primary_ids()
 
@@ -111,6 +111,43 @@ def test_batch_with_an_empty_scene():
         for i in (0, 2):
             mesh, w, h, seed = batch[i]
             aov, ids = plain_render(r, mesh, w, h, seed, spp)
+            for k in ("beauty", "albedo", "normal", "depth"):
+                assert np.array_equal(got[i][0][k], aov[k]), (i, k)
+            assert np.array_equal(got[i][1], ids), i
+
+
+def subset(mesh, tris, name):
+    """The triangles `tris` of mesh (all vertices kept)."""
+    t = np.asarray(tris)
+    return scenes.Mesh(mesh.vertices, mesh.indices[t], mesh.albedo[t], mesh.emission[t], dict(mesh.camera), name)
+
+
+@pytest.mark.parametrize("block", [128, 256])
+def test_default_traversal_batch_with_tiny_scenes(block):
+    """The default traversal of a batch (quantised nodes; every wave stages the
+    top three levels of one scene in LDS) over scenes of one, two, five and
+    sixty triangles (with the soup's lights), none, and five thousand: each
+    equals its plain render bit for bit.  A one-triangle scene's root has a
+    single used slot; the small trees have fewer internal nodes than the 21
+    staged slots."""
+    soup = scenes.soup(5000, seed=21)
+    n = soup.num_triangles
+    lights = list(range(n - 8, n))
+    z3 = np.zeros((0, 3), dtype=np.float32)
+    empty = scenes.Mesh(z3, np.zeros((0, 3), dtype=np.uint32), z3, z3, dict(soup.camera), "empty")
+    batch = [(subset(soup, [0], "one"), 16, 16, 1), (empty, 16, 16, 2), (subset(soup, [0, 1], "two"), 24, 16, 3),
+             (soup, 48, 32, 4), (subset(soup, list(range(5)) + lights, "five"), 16, 24, 5),
+             (subset(soup, list(range(60)) + lights, "sixty"), 32, 32, 6)]
+    params = dict(PARAMS, block_size=block, queue_chunk=0)
+    spp = 4
+    say(f"[batch-tiny] block {block}")
+    with HipRenderer(16, 16) as r:
+        r.set_params(spp_per_render=1, seed=0, **params)
+        r.set_batch(batch)
+        r.render_batch(spp)
+        got = [(r.readback_batch(i), r.batch_ids(i)) for i in range(len(batch))]
+        for i, (mesh, w, h, seed) in enumerate(batch):
+            aov, ids = plain_render(r, mesh, w, h, seed, spp, params)
             for k in ("beauty", "albedo", "normal", "depth"):
                 assert np.array_equal(got[i][0][k], aov[k]), (i, k)
             assert np.array_equal(got[i][1], ids), i

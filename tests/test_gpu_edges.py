@@ -182,3 +182,32 @@ def test_rejected_large_scene_through_the_gpu_build():
         r.render(1)
         after = r.readback()
     assert np.array_equal(before, after)
+
+
+def test_scene_beyond_the_quantised_range_traces_the_full_nodes():
+    """One triangle at x = 1e21 beside the room: the root's extent needs a
+    quantisation step above 2^60, so the plugin says so and traces the
+    128-byte nodes under the default parameters; the image equals the
+    128-byte traversal's bit for bit."""
+    room = scenes.cornell36(3)
+    nv = room.vertices.shape[0]
+    far = np.array([[1e21, 0.0, 0.0], [1e21, 1.0, 0.0], [1e21, 0.0, 1.0]], dtype=np.float32)
+    mesh = scenes.Mesh(np.concatenate([room.vertices, far]),
+                       np.concatenate([room.indices, np.array([[nv, nv + 1, nv + 2]], dtype=np.uint32)]),
+                       np.concatenate([room.albedo, np.full((1, 3), 0.5, np.float32)]),
+                       np.concatenate([room.emission, np.zeros((1, 3), np.float32)]), dict(room.camera), "far")
+    c = mesh.camera
+    imgs = []
+    logs = []
+    for chunk in (0, 1):
+        with HipRenderer(40, 32, log=logs.append) as r:
+            r.set_scene(mesh)
+            r.camera = make_camera(c["position"], c["target"], c["up"], c["fov"])
+            r.set_params(spp_per_render=4, max_depth=5, rr_start=3, rng_mode=1, integrator=1, seed=3,
+                         engine=abi.Q_ENGINE_WAVEFRONT, queue_chunk=chunk)
+            r.render(1)
+            imgs.append(r.readback())
+    print("[quant-range]", [m for m in logs if "quantised" in m], flush=True)
+    assert any("beyond the quantised nodes' range" in m for m in logs)
+    assert np.array_equal(imgs[0], imgs[1])
+    assert float(imgs[0][..., :3].sum()) > 0.0
