@@ -118,8 +118,10 @@ __host__ __device__ inline void slab_quant4(const float4& q0, const float4& q1, 
 #pragma clang fp contract(off)
     const float ax = q0.w * ix, ay = q1.x * iy, az = q1.y * iz;
     const float bx = (q0.x - ox) * ix, by = (q0.y - oy) * iy, bz = (q0.z - oz) * iz;
-    const float sx = fabsf(bx) * kQuantSlack, sy = fabsf(by) * kQuantSlack, sz = fabsf(bz) * kQuantSlack;
-    const float bxl = bx - sx, bxh = bx + sx, byl = by - sy, byh = by + sy, bzl = bz - sz, bzh = bz + sz;
+    // b -/+ |b| 2^-22 in one rounding each
+    const float bxl = fmaf(-fabsf(bx), kQuantSlack, bx), bxh = fmaf(fabsf(bx), kQuantSlack, bx);
+    const float byl = fmaf(-fabsf(by), kQuantSlack, by), byh = fmaf(fabsf(by), kQuantSlack, by);
+    const float bzl = fmaf(-fabsf(bz), kQuantSlack, bz), bzh = fmaf(fabsf(bz), kQuantSlack, bz);
     union { float f; uint32_t u; } lx{q1.z}, hx{q1.w}, ly{q2.x}, hy{q2.y}, lz{q2.z}, hz{q2.w};
     const uint32_t wex = ix < 0.f ? hx.u : lx.u, wfx = ix < 0.f ? lx.u : hx.u;
     const uint32_t wey = iy < 0.f ? hy.u : ly.u, wfy = iy < 0.f ? ly.u : hy.u;

@@ -4,9 +4,10 @@ The soup's BVH2 from the plugin's host builder is collapsed to 4- and 8-wide
 trees the way the BVH4 layout is made, and a scalar closest-hit traversal
 counts node steps and child-box tests per ray (camera rays and one secondary
 ray per camera hit).  The traversal kernel's node step tests every slot of a
-node (empty slots hold inverted boxes), so its box work per ray is width x
-steps: going from 4 to 8 children cuts the steps by about a third but adds
-about 40% box tests, and the kernel is bound by VALU issue.
+node, so its box work per ray is width x steps: going from 4 to 8 children
+cuts the steps by about a third but adds about 40% box tests, and the kernel
+(quantised nodes) is bound by VALU issue, with the texture addresser's
+per-lane loads close behind.
 """
 import subprocess
 
