@@ -258,10 +258,11 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // QUANT (BVH4, single scenes and batches): the nodes are the 64-byte quantised
 // ones of bvh_quant.hpp (P.qnodes4, a scene's SceneDesc::qnodes4), read with
 // four 16-byte loads per node step instead of seven; staged top levels take
-// 4 float4 per node.  The kernel is bound by the texture addresser's per-lane
-// cache-line lookups (busy 86% of the extend launch on the 1M soup with
-// 128-byte nodes), so the loads, not the VALU work, set the node step's cost:
-// extend + shadow per 64-spp frame 106.2 ms against 139.2 ms, same images.
+// 4 float4 per node.  With 128-byte nodes the kernel was bound by the texture
+// addresser's per-lane cache-line lookups (busy 86% of the extend launch on the
+// 1M soup), so the loads, not the VALU work, set the node step's cost: extend +
+// shadow per 64-spp frame 106.2 ms (three levels staged, block 128) against
+// 136.1 ms (128-byte nodes, four levels, block 256), same images.
 constexpr int kLdsRef = 0x40000000;   // node references >= this are LDS slots (fewer than 2^30 nodes)
 constexpr int kTopFloat4 = 7;         // float4s of a BVH4 node that the traversal reads ([7] is unused)
 
