@@ -101,6 +101,11 @@ struct WaveState {
     uint32_t* fetch;       // work cursors: group 2b (extend) and 2b+1 (shadow), per segment
     uint32_t paths;        // paths in this pass
     uint32_t seg_cap;      // slots per queue segment: ceil(paths / kQueueSegments)
+    // float4 per slot, double buffered like the rays: throughput.xyz, w = RNG
+    // state (bits) of the path in that slot, written by the shade that queues
+    // the ray and read by the next bounce's shade in the same (coalesced)
+    // order.  Null: the per-path arrays thr and rng carry them instead.
+    void* tq[2];
 };
 
 // One scene of a wavefront pass.  A single-scene pass carries it inline
@@ -207,8 +212,8 @@ constexpr int kShortStack = 16;       // LDS entries per lane in the persistent 
 // few ulps).
 constexpr float kSlabWiden = 1.000001f;
 
-// Paths in flight per automatically sized wavefront pass: about 25 GB of path
-// state (188 B per path), large enough that the short queues of the last
+// Paths in flight per automatically sized wavefront pass: about 30 GB of path
+// state (220 B per path), large enough that the short queues of the last
 // bounces are a small part of a pass.
 constexpr uint32_t kAutoPassPaths = 1u << 27;
 constexpr uint32_t kMaxThreadsPerCU = 2048;

@@ -460,8 +460,8 @@ int32_t upload_scene(plugin_state* st) {
 }
 
 void free_wave(WaveBuffers& w) {
-    for (auto* b : {&w.ray_o[0], &w.ray_o[1], &w.ray_d[0], &w.ray_d[1], &w.hit, &w.thr, &w.radiance, &w.rng,
-                    &w.aov0, &w.aov1, &w.sh_o, &w.sh_d, &w.sh_c, &w.counts, &w.shadow_counts, &w.fetch})
+    for (auto* b : {&w.ray_o[0], &w.ray_o[1], &w.ray_d[0], &w.ray_d[1], &w.tq[0], &w.tq[1], &w.hit, &w.thr,
+                    &w.radiance, &w.rng, &w.aov0, &w.aov1, &w.sh_o, &w.sh_d, &w.sh_c, &w.counts, &w.shadow_counts, &w.fetch})
         free_buffer(*b);
     w.capacity = 0;
 }
@@ -517,6 +517,7 @@ bool ensure_wave(plugin_state* st, uint32_t paths) {
     for (int k = 0; k < 2; ++k) {
         ok = ok && ensure_buffer(st, w.ray_o[k], 16 * q, "wavefront rays");
         ok = ok && ensure_buffer(st, w.ray_d[k], 16 * q, "wavefront rays");
+        ok = ok && ensure_buffer(st, w.tq[k], 16 * q, "queue-ordered path state");
     }
     ok = ok && ensure_buffer(st, w.hit, 8 * q, "wavefront hits") && ensure_buffer(st, w.thr, 16 * n, "throughput") &&
          ensure_buffer(st, w.radiance, 16 * n, "radiance") && ensure_buffer(st, w.rng, 4 * n, "rng states") &&
@@ -533,7 +534,11 @@ bool ensure_wave(plugin_state* st, uint32_t paths) {
 
 WaveState wave_state(const WaveBuffers& w, uint32_t paths) {
     WaveState s{};
-    for (int k = 0; k < 2; ++k) { s.ray_o[k] = w.ray_o[k].ptr; s.ray_d[k] = w.ray_d[k].ptr; }
+    for (int k = 0; k < 2; ++k) {
+        s.ray_o[k] = w.ray_o[k].ptr;
+        s.ray_d[k] = w.ray_d[k].ptr;
+        s.tq[k] = w.tq[k].ptr;
+    }
     s.hit = w.hit.ptr;
     s.thr = w.thr.ptr;
     s.radiance = w.radiance.ptr;

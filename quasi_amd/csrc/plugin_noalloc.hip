@@ -41,7 +41,9 @@ constexpr size_t kCounterGroup = size_t(kQueueSegments) * kCounterStride * 4;
 constexpr size_t kCountersBytes = kCounterGroup * ((kMaxDepth + 1) + kMaxDepth + 2 * kMaxDepth);
 // Path state per path (throughput, radiance, two first-hit records, RNG
 // state) and per queue slot (two ray buffers of origin + direction, the
-// hit, the shadow ray's origin, direction and contribution).
+// hit, the shadow ray's origin, direction and contribution).  Throughput and
+// RNG stay in the per-path arrays (WaveState::tq null), which holds 32 B per
+// path less than the queue-ordered copies the allocating entry points use.
 constexpr size_t kPerPath = 16 * 4 + 4;
 constexpr size_t kPerSlot = 2 * 32 + 8 + 3 * 16;
 

@@ -34,7 +34,7 @@ extern const bool kRenderKernel[K_COUNT];
 
 // Wavefront path-state buffers (see WaveState in common.hpp).
 struct WaveBuffers {
-    DeviceBuffer ray_o[2], ray_d[2], hit, thr, radiance, rng, aov0, aov1, sh_o, sh_d, sh_c;
+    DeviceBuffer ray_o[2], ray_d[2], hit, thr, radiance, rng, aov0, aov1, sh_o, sh_d, sh_c, tq[2];
     DeviceBuffer counts, shadow_counts, fetch;
     uint32_t capacity = 0;
 };
