@@ -156,7 +156,8 @@ struct Builder {
                 }
             }
         }
-        bool leaf = (n <= 1) || (n <= uint32_t(kLeafMax) && (best_axis < 0 || double(n) <= best_cost));
+        bool leaf = (n <= 1) || (n <= uint32_t(kLeafMax) &&
+                                (best_axis < 0 || double(n) <= best_cost - (1.0 - double(kSahNodeCost))));
         if (leaf) {
             nd.first = task.begin;
             nd.count = n;

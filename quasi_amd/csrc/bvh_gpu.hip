@@ -1,8 +1,9 @@
 // bvh_gpu.hip -- binned-SAH BVH build on the GPU.
 //
 // Same algorithm as the host build (bvh_build.cpp): per node, 32 centroid
-// bins on each axis, the cheapest of the 31 x 3 candidate planes by the SAH
-// (traversal cost 1, triangle cost 1), at most kLeafMax triangles per leaf, a
+// bins on each axis, the cheapest of the 31 x 3 candidate planes by the SAH,
+// leaves of at most kLeafMax triangles chosen against the best split with the
+// node cost kSahNodeCost (common.hpp), a
 // median split when every centroid coincides.  The output is a binary tree of
 // BinaryNode plus the triangle order, which bvh_layout.hip turns into the
 // traversal layouts on the device; the triangle and material arrays are then
@@ -683,7 +684,8 @@ __global__ __launch_bounds__(64) void k_subtree(const NodeTask* tasks, uint32_t 
                 }
                 __syncthreads();
             }
-            const bool leaf = cnt <= 1u || (cnt <= uint32_t(kLeafMax) && (best_axis < 0 || float(cnt) <= best_cost));
+            const bool leaf = cnt <= 1u || (cnt <= uint32_t(kLeafMax) &&
+                                             (best_axis < 0 || float(cnt) <= best_cost - (1.f - kSahNodeCost)));
             BinaryNode& nd = nodes[cur.node];
             if (leaf) {
                 if (lane == 0) {

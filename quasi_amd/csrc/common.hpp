@@ -37,6 +37,14 @@ namespace quasi_hip {
 
 constexpr int kLeafMax = 4;
 constexpr int kSahBins = 32;
+// SAH cost of a binary node's traversal relative to one triangle test.  Split
+// planes are ranked by 1 + (A_l n_l + A_r n_r) / A (the constant does not
+// change the ranking); the constant decides only, for a node of at most
+// kLeafMax triangles, between a leaf (cost n) and its best split (cost
+// kSahNodeCost + (A_l n_l + A_r n_r) / A).  Measured on the 1M soup at 1080p,
+// ms of extend + shadow per 64-spp frame: 93.9 at 2, 88.9 at 1.5, 84.8 at 1,
+// 81.4-81.6 at 0.5, 81.9 at 0, 81.8 with every splittable node split.
+constexpr float kSahNodeCost = 0.5f;
 constexpr uint32_t kMissId = 0xFFFFFFFFu;
 constexpr int32_t kEmptyChild = 0x7FFFFFFF;
 constexpr float kEmptyLo = __builtin_huge_valf();    // box of an unused BVH4 slot
