@@ -35,6 +35,7 @@
 
 #include "bvh_build.hpp"
 #include "bvh_gpu.hpp"
+#include "bvh_split.hpp"
 #include "common.hpp"
 #include "device_buffer.hpp"
 
@@ -941,17 +942,20 @@ __global__ void k_gather_corners(const float* __restrict__ verts, uint32_t nv, c
     if (bad) atomicOr(err, 1u);
 }
 
-// Triangle and material arrays in BVH order (layouts of common.hpp) and the
-// inverse order; a triangle that occurs twice or is out of range sets err.
-__global__ void k_scene_arrays(const uint32_t* __restrict__ order, uint32_t n, const float* __restrict__ d0,
-                               const float* __restrict__ d1, const float* __restrict__ d2,
-                               const float* __restrict__ albedo, const float* __restrict__ emission,
-                               float4* __restrict__ tris, float4* __restrict__ mats, uint32_t* __restrict__ inv,
-                               uint32_t* __restrict__ err) {
+// Triangle and material arrays in BVH order (layouts of common.hpp): position
+// i holds triangle ref_tri[r] of reference r = order[i]; inv[r] = i.  A
+// reference that occurs twice or out of range sets err.
+__global__ void k_scene_arrays(const uint32_t* __restrict__ order, uint32_t n, const uint32_t* __restrict__ ref_tri,
+                               uint32_t nt, const float* __restrict__ d0, const float* __restrict__ d1,
+                               const float* __restrict__ d2, const float* __restrict__ albedo,
+                               const float* __restrict__ emission, float4* __restrict__ tris,
+                               float4* __restrict__ mats, uint32_t* __restrict__ inv, uint32_t* __restrict__ err) {
     bool bad = false;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t t = order[i];
-        if (t >= n || atomicExch(&inv[t], i) != 0xFFFFFFFFu) { bad = true; continue; }
+        const uint32_t r = order[i];
+        if (r >= n || atomicExch(&inv[r], i) != 0xFFFFFFFFu) { bad = true; continue; }
+        const uint32_t t = ref_tri[r];
+        if (t >= nt) { bad = true; continue; }
         const size_t o = 3 * size_t(t);
         const float ax = d0[o], ay = d0[o + 1], az = d0[o + 2];
         tris[3 * size_t(i)] = make_float4(ax, ay, az, __uint_as_float(t));
@@ -966,6 +970,102 @@ __global__ void k_scene_arrays(const uint32_t* __restrict__ order, uint32_t n, c
     if (bad) atomicOr(err, 2u);
 }
 
+// fkey bounds of the finite corner coordinates, per axis, into b[0..2] (min)
+// and b[3..5] (max), which hold kIntMax and -kIntMax - 1 beforehand: the scene
+// box of the split threshold.
+__global__ void k_corner_bounds(const float* __restrict__ d0, const float* __restrict__ d1,
+                                const float* __restrict__ d2, uint32_t nt, int* __restrict__ b) {
+    int lo[3] = {int(kIntMax), int(kIntMax), int(kIntMax)};
+    int hi[3] = {-int(kIntMax) - 1, -int(kIntMax) - 1, -int(kIntMax) - 1};
+    const float* corners[3] = {d0, d1, d2};
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) {
+                const float v = corners[c][3 * size_t(t) + k];
+                if (!isfinite(v)) continue;
+                lo[k] = min(lo[k], fkey(v));
+                hi[k] = max(hi[k], fkey(v));
+            }
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo[k] = min(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = max(hi[k], __shfl_xor(hi[k], off));
+        }
+    }
+    if ((threadIdx.x & 63u) == 0u)
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&b[k], lo[k]);
+            atomicMax(&b[3 + k], hi[k]);
+        }
+}
+
+constexpr int kSplitBlock = 256;
+
+__device__ __forceinline__ bool emits(const float* e) {
+    return (__float_as_uint(e[0]) | __float_as_uint(e[1]) | __float_as_uint(e[2])) != 0u;
+}
+
+// References per triangle (bvh_split.hpp), one triangle per thread; the sum
+// of each block's into bsum[block].
+__global__ __launch_bounds__(kSplitBlock) void k_split_count(const float* __restrict__ d0,
+                                                             const float* __restrict__ d1,
+                                                             const float* __restrict__ d2,
+                                                             const float* __restrict__ emission, uint32_t nt, float T,
+                                                             uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t s_sum;
+    if (threadIdx.x == 0) s_sum = 0u;
+    __syncthreads();
+    const uint32_t t = blockIdx.x * kSplitBlock + threadIdx.x;
+    uint32_t c = 0;
+    if (t < nt) {
+        PieceBox pc[kMaxPieces];
+        const size_t o = 3 * size_t(t);
+        c = split_triangle(d0 + o, d1 + o, d2 + o, T, !emits(emission + o), pc);
+    }
+    atomicAdd(&s_sum, c);
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = s_sum;
+}
+
+// The references themselves: triangle t's at [tri_off[t], tri_off[t] +
+// count), block offsets from the host's scan of k_split_count's sums, in
+// triangle order; boxes as lo / hi corners, ref_tri = t.
+__global__ __launch_bounds__(kSplitBlock) void k_split_write(const float* __restrict__ d0,
+                                                             const float* __restrict__ d1,
+                                                             const float* __restrict__ d2,
+                                                             const float* __restrict__ emission, uint32_t nt, float T,
+                                                             const uint32_t* __restrict__ boff,
+                                                             float* __restrict__ rlo, float* __restrict__ rhi,
+                                                             uint32_t* __restrict__ ref_tri,
+                                                             uint32_t* __restrict__ tri_off) {
+    __shared__ uint32_t s_scan[kSplitBlock];
+    const uint32_t t = blockIdx.x * kSplitBlock + threadIdx.x;
+    PieceBox pc[kMaxPieces];
+    uint32_t c = 0;
+    if (t < nt) {
+        const size_t o = 3 * size_t(t);
+        c = split_triangle(d0 + o, d1 + o, d2 + o, T, !emits(emission + o), pc);
+    }
+    s_scan[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < uint32_t(kSplitBlock); off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (t >= nt) return;
+    const uint32_t base = boff[blockIdx.x] + s_scan[threadIdx.x] - c;
+    tri_off[t] = base;
+    for (uint32_t j = 0; j < c; ++j) {
+        for (int k = 0; k < 3; ++k) {
+            rlo[3 * size_t(base + j) + k] = pc[j].lo[k];
+            rhi[3 * size_t(base + j) + k] = pc[j].hi[k];
+        }
+        ref_tri[base + j] = t;
+    }
+}
+
 // Emission of the listed triangles (ids checked on the host) into a zeroed array.
 __global__ void k_scatter_emission(const uint32_t* __restrict__ ids, const float* __restrict__ vals, uint32_t m,
                                    float* __restrict__ emission) {
@@ -977,9 +1077,12 @@ __global__ void k_scatter_emission(const uint32_t* __restrict__ ids, const float
     }
 }
 
-__global__ void k_gather_positions(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ query, uint32_t nq,
-                                   uint32_t* __restrict__ pos) {
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) pos[q] = inv[query[q]];
+// BVH-order position of each queried triangle: that of its first reference
+// (an emitter has only one).
+__global__ void k_gather_positions(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ tri_off,
+                                   const uint32_t* __restrict__ query, uint32_t nq, uint32_t* __restrict__ pos) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x)
+        pos[q] = inv[tri_off[query[q]]];
 }
 
 uint32_t grid256(size_t items) { return uint32_t(std::max<size_t>(1, std::min<size_t>((items + 255) / 256, 16384))); }
@@ -1059,21 +1162,75 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
     didx.release();
     lap(tm.upload_ms);
 
+    // references (bvh_split.hpp): the threshold from the scene box, then the
+    // pieces of each triangle in triangle order
+    DevBuf<int> dbounds;
+    if (!dbounds.alloc(6)) { *why = "allocation"; return false; }
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(dbounds.p), size_t(3),
+                       kIntMax);
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(dbounds.p) + 3, size_t(3),
+                       kIntMax + 1u);
+    hipLaunchKernelGGL(k_corner_bounds, dim3(grid256(nt)), dim3(256), 0, s, dp[0].p, dp[1].p, dp[2].p, nt, dbounds.p);
+    BVH_CHECK(hipGetLastError());
+    int hb[6];
+    BVH_CHECK(hipMemcpyAsync(hb, dbounds.p, sizeof hb, hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipStreamSynchronize(s));
+    float slo[3], shi[3];
+    bool finite_box = true;
+    for (int k = 0; k < 3; ++k) {
+        finite_box = finite_box && hb[k] <= hb[3 + k];
+        const int a = hb[k] >= 0 ? hb[k] : hb[k] ^ int(kIntMax), b = hb[3 + k] >= 0 ? hb[3 + k] : hb[3 + k] ^ int(kIntMax);
+        std::memcpy(&slo[k], &a, 4);
+        std::memcpy(&shi[k], &b, 4);
+    }
+    const float T = finite_box ? split_threshold(slo, shi, nt) : 3.0e38f;
+    const uint32_t nblk = (nt + kSplitBlock - 1) / kSplitBlock;
+    DevBuf<uint32_t> bsum, boff, tri_off, ref_tri;
+    DevBuf<float> rlo, rhi;
+    if (!bsum.alloc(nblk) || !boff.alloc(nblk) || !tri_off.alloc(nt)) { *why = "allocation"; return false; }
+    hipLaunchKernelGGL(k_split_count, dim3(nblk), dim3(kSplitBlock), 0, s, dp[0].p, dp[1].p, dp[2].p, demi.p, nt, T,
+                       bsum.p);
+    BVH_CHECK(hipGetLastError());
+    std::vector<uint32_t> hsum(nblk);
+    BVH_CHECK(hipMemcpyAsync(hsum.data(), bsum.p, 4 * size_t(nblk), hipMemcpyDeviceToHost, s));
+    BVH_CHECK(hipStreamSynchronize(s));
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const uint32_t c = hsum[b];
+        hsum[b] = uint32_t(total);
+        total += c;
+    }
+    if (total < nt || total > (1u << 29)) { *why = "reference count"; return false; }
+    const uint32_t nr = uint32_t(total);
+    if (!rlo.alloc(3 * size_t(nr)) || !rhi.alloc(3 * size_t(nr)) || !ref_tri.alloc(nr)) {
+        *why = "allocation";
+        return false;
+    }
+    BVH_CHECK(hipMemcpyAsync(boff.p, hsum.data(), 4 * size_t(nblk), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_split_write, dim3(nblk), dim3(kSplitBlock), 0, s, dp[0].p, dp[1].p, dp[2].p, demi.p, nt, T,
+                       boff.p, rlo.p, rhi.p, ref_tri.p, tri_off.p);
+    BVH_CHECK(hipGetLastError());
+    bsum.release();
+    lap(tm.split_ms);
+
+    // the tree over the reference boxes (each given as the triangle lo, hi, lo)
     DeviceTree tree;
-    if (!build_tree(dp[0].p, dp[1].p, dp[2].p, nt, s, tree, &tm.device_ms, why)) {
+    if (!build_tree(rlo.p, rhi.p, rlo.p, nr, s, tree, &tm.device_ms, why)) {
         if (!**why) *why = "HIP call";
         return false;
     }
+    rlo.release();
+    rhi.release();
     lap(tm.build_ms);
 
     DeviceLayout lay;
     struct LayGuard { DeviceLayout& l; ~LayGuard() { l.release(); } } lg{lay};
-    if (!layout_tree_gpu(tree.nodes.p, tree.used, tree.unused, tree.prims.p, nt, s, lay, why)) return false;
+    if (!layout_tree_gpu(tree.nodes.p, tree.used, tree.unused, tree.prims.p, nr, s, lay, why)) return false;
     tree.nodes.release();
     tree.prims.release();
     DevBuf<float4> tris, mats;
     DevBuf<uint32_t> inv, dq, dpos;
-    if (!tris.alloc(3 * size_t(nt)) || !mats.alloc(2 * size_t(nt)) || !inv.alloc(nt) ||
+    if (!tris.alloc(3 * size_t(nr)) || !mats.alloc(2 * size_t(nr)) || !inv.alloc(nr) ||
         (nquery && (!dq.alloc(nquery) || !dpos.alloc(nquery)))) {
         *why = "allocation";
         return false;
@@ -1083,12 +1240,13 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return false; }
     struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
     BVH_CHECK(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(grid256(nt)), dim3(256), 0, s, inv.p, size_t(nt), 0xFFFFFFFFu);
-    hipLaunchKernelGGL(k_scene_arrays, dim3(grid256(nt)), dim3(256), 0, s, lay.order, nt, dp[0].p, dp[1].p, dp[2].p,
-                       dalb.p, demi.p, tris.p, mats.p, inv.p, err.p);
+    hipLaunchKernelGGL(k_fill_u32_bvh, dim3(grid256(nr)), dim3(256), 0, s, inv.p, size_t(nr), 0xFFFFFFFFu);
+    hipLaunchKernelGGL(k_scene_arrays, dim3(grid256(nr)), dim3(256), 0, s, lay.order, nr, ref_tri.p, nt, dp[0].p,
+                       dp[1].p, dp[2].p, dalb.p, demi.p, tris.p, mats.p, inv.p, err.p);
     if (nquery) {
         BVH_CHECK(hipMemcpyAsync(dq.p, query, 4 * size_t(nquery), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_gather_positions, dim3(grid256(nquery)), dim3(256), 0, s, inv.p, dq.p, nquery, dpos.p);
+        hipLaunchKernelGGL(k_gather_positions, dim3(grid256(nquery)), dim3(256), 0, s, inv.p, tri_off.p, dq.p, nquery,
+                           dpos.p);
         BVH_CHECK(hipMemcpyAsync(query_pos, dpos.p, 4 * size_t(nquery), hipMemcpyDeviceToHost, s));
     }
     BVH_CHECK(hipGetLastError());
@@ -1104,6 +1262,7 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
     out.nodes = lay.nodes;
     out.nodes4 = lay.nodes4;
     lay.nodes = lay.nodes4 = nullptr;
+    out.num_prims = nr;
     out.num_nodes = lay.num_nodes;
     out.num_leaves = lay.num_leaves;
     out.depth = lay.depth;

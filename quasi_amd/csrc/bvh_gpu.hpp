@@ -17,7 +17,7 @@ namespace quasi_hip {
 struct BvhGpuTimes {
     double device_ms = 0;          // tree build kernels
     double layout_device_ms = 0;   // layout kernels (pair nodes, 4-wide nodes, triangles)
-    double alloc_ms = 0, upload_ms = 0, build_ms = 0, layout_ms = 0;
+    double alloc_ms = 0, upload_ms = 0, split_ms = 0, build_ms = 0, layout_ms = 0;
 };
 
 // Traversal layouts of a binary tree on the device (hipMalloc'd; the caller
@@ -44,21 +44,25 @@ bool layout_tree_gpu(const BinaryNode* tree, uint32_t used, uint32_t unused, con
 // One scene built on the device: the arrays of common.hpp (triangles,
 // materials, pair nodes, 4-wide nodes), hipMalloc'd and owned by the caller.
 struct DeviceScene {
-    void* tris = nullptr;     // 12 floats per triangle, BVH order
-    void* mats = nullptr;     // 8 floats per triangle, BVH order
+    void* tris = nullptr;     // 12 floats per BVH-order position (a triangle reference)
+    void* mats = nullptr;     // 8 floats per BVH-order position
     float* nodes = nullptr;
     float* nodes4 = nullptr;
+    uint32_t num_prims = 0;   // BVH-order positions: references of the split triangles (bvh_split.hpp)
     uint32_t num_nodes = 0, num_leaves = 0, depth = 0, num_nodes4 = 0, depth4 = 0;
     double sah_cost = 0;
     void release();
 };
 
-// Builds a mesh on the current device and stream: vertex gather, binned-SAH
-// tree, layouts and the BVH-order triangle / material arrays.  emitters
+// Builds a mesh on the current device and stream: vertex gather, the split
+// of large triangles into references (bvh_split.hpp), the binned-SAH tree over
+// the references, layouts and the BVH-order triangle / material arrays (one
+// copy of a triangle's record per reference).  emitters
 // (n_emit input triangle ids, ascending) lists every triangle whose emission
 // has a nonzero bit: only those emissions are uploaded when they are few (all
 // others are zero).  query (nquery input triangle ids, each < nt) receives
-// in query_pos the BVH-order position of each.  False (with a reason) on any
+// in query_pos the BVH-order position of each (of its first reference;
+// emitters, the triangles queried for the light table, are never split).  False (with a reason) on any
 // failure, including a vertex index out of range; the caller then builds on
 // the host.
 bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* albedo,

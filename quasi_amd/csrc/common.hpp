@@ -161,7 +161,7 @@ struct PassParams {
     uint32_t num_lights;
     uint32_t num_nodes;
     uint32_t record_ids, id_sample;
-    uint32_t num_tris;     // triangles (a batch: of all scenes, whose arrays are concatenated)
+    uint32_t num_tris;     // BVH-order triangle records (a batch: of all scenes, whose arrays are concatenated)
     // 4-wide BVH over the same triangles, and the traversal-stack overflow
     // area of the persistent kernels (gstack_entries words per thread,
     // lane-interleaved; sized by the host for the deepest possible stack)

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "bvh_build.hpp"
+#include "bvh_split.hpp"
 #include "plugin_state.hpp"
 #include "scene_cornell.hpp"
 
@@ -249,6 +250,7 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
             out.device_ms = out.gpu_times.device_ms;
             out.sah_cost = d.sah_cost;
             out.num_tris = nt;
+            out.num_prims = d.num_prims;
             out.num_nodes = d.num_nodes;
             out.num_nodes4 = d.num_nodes4;
             out.depth = d.depth;
@@ -274,9 +276,53 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
                 p2[3 * t + k] = verts[3 * size_t(idx[3 * t + 2]) + k];
             }
     });
+    // references (bvh_split.hpp): the threshold from the box of the finite
+    // corner coordinates, the pieces of each triangle in triangle order
+    float slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const std::vector<float>* pv : {&p0, &p1, &p2})
+        for (size_t i = 0; i < 3 * size_t(nt); ++i) {
+            const float v = (*pv)[i];
+            if (!std::isfinite(v)) continue;
+            slo[i % 3] = std::min(slo[i % 3], v);
+            shi[i % 3] = std::max(shi[i % 3], v);
+        }
+    const bool finite_box = slo[0] <= shi[0] && slo[1] <= shi[1] && slo[2] <= shi[2];
+    const float T = finite_box ? split_threshold(slo, shi, nt) : 3.0e38f;
+    auto emits = [&](size_t t) {
+        uint32_t b[3];
+        std::memcpy(b, &emission[3 * t], 12);
+        return (b[0] | b[1] | b[2]) != 0u;
+    };
+    std::vector<uint32_t> tri_off(size_t(nt) + 1, 0);
+    parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
+        PieceBox pc[kMaxPieces];
+        for (size_t t = tb; t < te; ++t)
+            tri_off[t + 1] = split_triangle(&p0[3 * t], &p1[3 * t], &p2[3 * t], T, !emits(t), pc);
+    });
+    for (size_t t = 0; t < nt; ++t) tri_off[t + 1] += tri_off[t];
+    if (tri_off[nt] > (1u << 29)) {
+        log_msg(st, "scene: %llu BVH references exceed the builder's limit", (unsigned long long)tri_off[nt]);
+        return Q_EXT_ERR_UNSUPPORTED;
+    }
+    const uint32_t nr = tri_off[nt];
+    std::vector<float> rlo(3 * size_t(nr)), rhi(3 * size_t(nr));
+    std::vector<uint32_t> ref_tri(nr);
+    parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
+        PieceBox pc[kMaxPieces];
+        for (size_t t = tb; t < te; ++t) {
+            const uint32_t c = split_triangle(&p0[3 * t], &p1[3 * t], &p2[3 * t], T, !emits(t), pc);
+            for (uint32_t j = 0; j < c; ++j) {
+                const size_t r = tri_off[t] + j;
+                std::memcpy(&rlo[3 * r], pc[j].lo, 12);
+                std::memcpy(&rhi[3 * r], pc[j].hi, 12);
+                ref_tri[r] = uint32_t(t);
+            }
+        }
+    });
     out.gather_ms = ms_since(t0);
     t0 = clock::now();
-    BvhBuildResult bvh = build_bvh(p0.data(), p1.data(), p2.data(), nt, threads);
+    // the tree over the reference boxes (each given as the triangle lo, hi, lo)
+    BvhBuildResult bvh = build_bvh(rlo.data(), rhi.data(), rlo.data(), nr, threads);
     out.sah_cost = bvh.sah_cost;
     out.bvh_ms = ms_since(t0);
     t0 = clock::now();
@@ -284,13 +330,15 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
         log_msg(st, "scene: BVH depth %u exceeds the traversal stack (%u)", bvh.depth, megakernel_max_depth());
         return Q_EXT_ERR_UNSUPPORTED;
     }
-    std::vector<uint32_t> bvh_pos(nt);
-    out.tri4.assign(12 * size_t(nt), 0.f);
-    out.mat4.assign(8 * size_t(nt), 0.f);
-    parallel_ranges(nt, host_threads, [&](size_t ib, size_t ie) {
+    // position of each reference; a triangle's record at every one of them
+    std::vector<uint32_t> ref_pos(nr);
+    out.tri4.assign(12 * size_t(nr), 0.f);
+    out.mat4.assign(8 * size_t(nr), 0.f);
+    parallel_ranges(nr, host_threads, [&](size_t ib, size_t ie) {
         for (size_t i = ib; i < ie; ++i) {
-            const uint32_t t = bvh.order[i];
-            bvh_pos[t] = uint32_t(i);
+            const uint32_t r = bvh.order[i];
+            const uint32_t t = ref_tri[r];
+            ref_pos[r] = uint32_t(i);
             float* d = &out.tri4[12 * i];
             for (int k = 0; k < 3; ++k) {
                 d[k] = p0[3 * size_t(t) + k];
@@ -305,13 +353,15 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
             m[4] = e[0]; m[5] = e[1]; m[6] = e[2]; m[7] = 0.f;
         }
     });
+    // an emitter has one reference (never split)
     std::vector<uint32_t> lpos(lights.size());
-    for (size_t i = 0; i < lights.size(); ++i) lpos[i] = bvh_pos[lights[i]];
+    for (size_t i = 0; i < lights.size(); ++i) lpos[i] = ref_pos[tri_off[lights[i]]];
     light_table(verts, idx, emission, lights, lpos.data(), out.lights);
     out.nodes = std::move(bvh.nodes);
     out.nodes4 = std::move(bvh.nodes4);
     out.layout_ms = ms_since(t0);
     out.num_tris = nt;
+    out.num_prims = nr;
     out.num_nodes = bvh.num_nodes;
     out.num_nodes4 = bvh.num_nodes4;
     out.depth = bvh.depth;
@@ -410,8 +460,8 @@ int32_t upload_scene(plugin_state* st) {
         // adopt the device arrays of the GPU build (the stream is idle: the build synchronised it)
         DeviceScene& d = sh.device;
         for (DeviceBuffer* b : {&st->tris, &st->mats, &st->nodes, &st->nodes4}) free_buffer(*b);
-        st->tris = {d.tris, 48 * size_t(nt)};
-        st->mats = {d.mats, 32 * size_t(nt)};
+        st->tris = {d.tris, 48 * size_t(d.num_prims)};
+        st->mats = {d.mats, 32 * size_t(d.num_prims)};
         st->nodes = {d.nodes, 64 * size_t(d.num_nodes)};
         st->nodes4 = {d.nodes4, 128 * size_t(d.num_nodes4)};
         d.tris = d.mats = nullptr;
@@ -433,6 +483,7 @@ int32_t upload_scene(plugin_state* st) {
         (!sh.lights.empty() && !hip_ok(st, hipMemcpy(st->lights.ptr, sh.lights.data(), sh.lights.size() * sizeof(LightEntry), hipMemcpyHostToDevice), "light upload")))
         return Q_EXT_ERR_DEVICE;
     st->num_tris = nt;
+    st->num_prims = sh.num_prims;
     st->num_nodes = sh.num_nodes;
     st->bvh_depth = sh.depth;
     st->num_lights = uint32_t(sh.lights.size());
@@ -448,12 +499,14 @@ int32_t upload_scene(plugin_state* st) {
     st->scene_ok = true;
     if (nt >= (1u << 20) && sh.builder == Q_BVH_BUILD_GPU) {
         const BvhGpuTimes& g = sh.gpu_times;
-        log_msg(st, "scene: %u triangles built on the GPU in %.0f ms: allocate %.0f, mesh upload and corners %.0f, "
-                "tree %.0f (kernels %.0f), layouts and triangle arrays %.0f (kernels %.0f)", nt, st->bvh_build_ms,
-                g.alloc_ms, g.upload_ms, g.build_ms, g.device_ms, g.layout_ms, g.layout_device_ms);
+        log_msg(st, "scene: %u triangles (%u references) built on the GPU in %.0f ms: allocate %.0f, mesh upload "
+                "and corners %.0f, split %.0f, tree %.0f (kernels %.0f), layouts and triangle arrays %.0f (kernels "
+                "%.0f)", nt, sh.num_prims, st->bvh_build_ms, g.alloc_ms, g.upload_ms, g.split_ms, g.build_ms,
+                g.device_ms, g.layout_ms, g.layout_device_ms);
     } else if (nt >= (1u << 20)) {
-        log_msg(st, "scene: %u triangles built on the host in %.0f ms: corners %.0f, BVH %.0f, layout %.0f, "
-                "upload %.0f", nt, st->bvh_build_ms, sh.gather_ms, sh.bvh_ms, sh.layout_ms,
+        log_msg(st, "scene: %u triangles (%u references) built on the host in %.0f ms: corners and split %.0f, "
+                "BVH %.0f, layout %.0f, upload %.0f", nt, sh.num_prims, st->bvh_build_ms, sh.gather_ms, sh.bvh_ms,
+                sh.layout_ms,
                 st->bvh_build_ms - sh.gather_ms - sh.bvh_ms - sh.layout_ms);
     }
     return Q_EXT_OK;
@@ -674,7 +727,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
     p.integrator = rp.integrator;
     p.num_lights = st->num_lights;
     p.num_nodes = st->num_nodes;
-    p.num_tris = st->num_tris;
+    p.num_tris = st->num_prims;
     p.record_ids = (rp.flags & Q_FLAG_RECORD_IDS) ? 1u : 0u;
     p.id_sample = rp.id_sample;
     p.nodes4 = st->nodes4.ptr;
@@ -698,7 +751,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
         sd.path_begin = 0;
         sd.pix_begin = 0;
         sd.seed = rp.seed;
-        sd.num_tris = st->num_tris;
+        sd.num_tris = st->num_prims;
         p.num_scenes = 0;
         p.total_pixels = st->width * st->height;
         const uint32_t per_pass = wave_samples_per_pass(st);

@@ -152,7 +152,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     p.integrator = rp.integrator;
     p.num_lights = st->num_lights;
     p.num_nodes = st->num_nodes;
-    p.num_tris = st->num_tris;
+    p.num_tris = st->num_prims;
     p.nodes4 = st->nodes4.ptr;
     p.qnodes4 = st->quant_ok ? st->qnodes4.ptr : nullptr;
     p.num_nodes4 = st->num_nodes4;
@@ -174,7 +174,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
     sd.width = p.width;
     sd.height = p.height;
     sd.seed = p.seed;
-    sd.num_tris = st->num_tris;
+    sd.num_tris = st->num_prims;
     p.total_pixels = uint32_t(npix);
     A.dimg = dimg;
     A.scale = 1.0f / float(d->spp);

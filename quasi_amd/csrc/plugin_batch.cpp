@@ -43,7 +43,7 @@ void free_batch(plugin_state* st) {
     for (auto& a : b.accum) free_buffer(a);
     b.descs.clear();
     b.active = false;
-    b.num_scenes = b.total_pixels = b.total_tris = b.sample_count = 0;
+    b.num_scenes = b.total_pixels = b.total_tris = b.total_prims = b.sample_count = 0;
 }
 
 namespace {
@@ -106,7 +106,7 @@ PassParams batch_params(plugin_state* st, uint32_t spp_pass) {
     p.nodes4 = b.nodes4.ptr;
     p.qnodes4 = b.quant_ok ? b.qnodes4.ptr : nullptr;
     p.lights = static_cast<const LightEntry*>(b.lights.ptr);
-    p.num_tris = b.total_tris;
+    p.num_tris = b.total_prims;
     p.num_nodes = b.total_nodes;
     p.num_nodes4 = b.total_nodes4;
     p.scenes = static_cast<const SceneDesc*>(b.table.ptr);
@@ -267,8 +267,8 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     for (uint32_t i = 0; i < n; ++i)
         if (rcs[i] != Q_EXT_OK) return rcs[i];
     // floats of each array, for host- and GPU-built scenes alike
-    auto tri_f = [](const SceneHost& h) { return size_t(12) * h.num_tris; };
-    auto mat_f = [](const SceneHost& h) { return size_t(8) * h.num_tris; };
+    auto tri_f = [](const SceneHost& h) { return size_t(12) * h.num_prims; };
+    auto mat_f = [](const SceneHost& h) { return size_t(8) * h.num_prims; };
     auto node_f = [](const SceneHost& h) { return size_t(16) * h.num_nodes; };
     auto node4_f = [](const SceneHost& h) { return size_t(32) * h.num_nodes4; };
 
@@ -343,7 +343,7 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
         d.pix_begin = pix;
         d.path_begin = pix;
         d.seed = sc.seed;
-        d.num_tris = h.num_tris;
+        d.num_tris = h.num_prims;
         to += tri_f(h);
         mo += mat_f(h);
         no += node_f(h);
@@ -354,6 +354,7 @@ int32_t set_batch(plugin_state* st, const Q_batch_desc* desc) {
     b.num_scenes = n;
     b.total_pixels = uint32_t(total_pix);
     b.total_tris = uint32_t(total_tris);
+    b.total_prims = uint32_t(tri_total / 12);
     b.total_nodes = uint32_t(node_total / 16);     // 16 floats per BVH2 pair node
     b.total_nodes4 = uint32_t(node4_total / 32);   // 32 floats per BVH4 node
     // every scene's nodes keep scene-local child references, so the

@@ -205,7 +205,7 @@ Q_EXPORT int32_t Q_plugin_render_noalloc(Q_plugin_handle* handle, const Q_camera
         p.integrator = rp.integrator;
         p.num_lights = st->num_lights;
         p.num_nodes = st->num_nodes;
-        p.num_tris = st->num_tris;
+        p.num_tris = st->num_prims;
         p.nodes4 = st->nodes4.ptr;
         p.qnodes4 = st->quant_ok ? st->qnodes4.ptr : nullptr;
         p.num_nodes4 = st->num_nodes4;
@@ -224,7 +224,7 @@ Q_EXPORT int32_t Q_plugin_render_noalloc(Q_plugin_handle* handle, const Q_camera
         sd.width = width;
         sd.height = height;
         sd.seed = rp.seed;
-        sd.num_tris = st->num_tris;
+        sd.num_tris = st->num_prims;
         p.num_scenes = 0;
         p.total_pixels = uint32_t(npix);
         for (uint32_t done = 0; done < spp; done += per_pass) {

@@ -51,7 +51,9 @@ struct SceneHost {
     std::vector<float> tri4, mat4, nodes, nodes4;
     std::vector<LightEntry> lights;
     DeviceScene device;
-    uint32_t num_tris = 0, num_nodes = 0, num_nodes4 = 0, depth = 0, depth4 = 0;
+    uint32_t num_tris = 0;    // input triangles
+    uint32_t num_prims = 0;   // BVH-order positions (references of the split triangles)
+    uint32_t num_nodes = 0, num_nodes4 = 0, depth = 0, depth4 = 0;
     uint32_t builder = Q_BVH_BUILD_HOST;   // where the BVH was built
     double sah_cost = 0, device_ms = 0;
     double gather_ms = 0, bvh_ms = 0, layout_ms = 0;   // host phases of a host build
@@ -67,6 +69,7 @@ struct BatchState {
     bool quant_ok = false;                 // qnodes4 holds every scene's nodes
     DeviceBuffer accum[Q_AOV_COUNT], ids, resolved, sample_begin;
     uint32_t num_scenes = 0, total_pixels = 0, total_tris = 0;
+    uint32_t total_prims = 0;              // BVH-order triangle records of the concatenated arrays
     uint32_t total_nodes = 0, total_nodes4 = 0;   // BVH2 / BVH4 nodes of the concatenated arrays
     uint32_t sample_count = 0;
     uint32_t gstack_entries = 0;
@@ -115,6 +118,7 @@ struct plugin_state {
     DeviceBuffer quant_flag;            // one word: set when a node is beyond the quantised range
     bool quant_ok = false;              // qnodes4 holds the current scene's nodes
     uint32_t num_tris = 0, num_nodes = 0, bvh_depth = 0, num_lights = 0;
+    uint32_t num_prims = 0;   // BVH-order triangle records (references of the split triangles)
     uint32_t num_nodes4 = 0, bvh_depth4 = 0, gstack_entries = 0;
     double bvh_build_ms = 0;
     uint32_t bvh_builder = Q_BVH_BUILD_HOST;
