@@ -525,7 +525,8 @@ def main():
         "config": {"workload": f"plain wavefront-config soup: {mesh.num_triangles} triangles, {W}x{H}, "
                                f"{args.spp} spp, depth {args.depth}, RR from 3, NEE",
                    "engine": args.engine, "spp_per_launch": args.spp_per_launch,
-                   "bvh": f"binned-SAH BVH{args.bvh_width if args.engine == 'wavefront' else 2}",
+                   "bvh": f"binned-SAH BVH{args.bvh_width if args.engine == 'wavefront' else 2} over split "
+                          f"triangle references",
                    "trace_block": args.block, "queue_chunk": args.chunk},
         "rays": rays,
         "rays_per_path": round(total_rays / (W * H * args.spp * args.steps), 4),

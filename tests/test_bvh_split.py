@@ -10,13 +10,14 @@ fractions over the float inputs):
 
 - the threshold is the half area of one of N equal cells of the scene box
   (closed forms for cubes and boxes);
-- a triangle at or below the threshold, or an emitter, keeps its own box;
+- a triangle at or below the threshold, an emitter or an axis-aligned flat
+  triangle keeps its own box;
 - with every triangle cut as far as it goes (threshold 0), the union of the
   references' boxes holds the triangle's corners, points of its edges and
   interior, and the points where its edges cross planes at and near the cuts
   (where the crossing is computed in float: a version without the 2^-20
-  growth of the clipped boxes loses 156 of these points), for soup-like, long thin,
-  axis-aligned flat and plane-touching triangles far from the origin;
+  growth of the clipped boxes loses 175 of these points), for soup-like, long thin,
+  nearly flat and plane-touching triangles, near and far from the origin;
 - each reference's box lies inside the triangle's box, at most four of them,
   and cutting a long diagonal triangle shrinks its total box area.
 """
@@ -122,6 +123,12 @@ def test_small_triangles_and_emitters_keep_their_box(shim):
     assert np.array_equal(out[:, 0, :3], v.min(axis=1)) and np.array_equal(out[:, 0, 3:], v.max(axis=1))
     out2, cnt2 = split(shim, tris, 0.0, may_split=np.zeros(len(tris)))
     assert np.all(cnt2 == 1) and np.array_equal(out2[:, 0], out[:, 0])
+    # axis-aligned flat triangles (walls) are not cut either
+    flat = tris.copy()
+    flat[:, 1] = flat[:, 4] = flat[:, 7] = np.float32(0.5)
+    out3, cnt3 = split(shim, flat, 0.0)
+    fv = flat.reshape(-1, 3, 3)
+    assert np.all(cnt3 == 1) and np.array_equal(out3[:, 0, :3], fv.min(axis=1))
 
 
 def case_triangles(rng):
@@ -137,9 +144,10 @@ def case_triangles(rng):
     a = rng.uniform(-1, 1, size=(40, 3))
     b = a + rng.uniform(-2, 2, size=(40, 3))
     cases.append(np.concatenate([a, b, b + rng.normal(scale=1e-3, size=(40, 3))], axis=1))
-    # axis-aligned flat triangles (zero thickness on one axis)
+    # nearly flat triangles (thickness 2^-20 on one axis)
     f = rng.uniform(-1, 1, size=(40, 9))
-    f[:, 2] = f[:, 5] = f[:, 8] = np.float32(0.25)
+    f[:, 2] = f[:, 5] = np.float32(0.25)
+    f[:, 8] = np.float32(0.25) + np.float32(2.0 ** -20)
     cases.append(f)
     # a vertex exactly on the first cut plane: corners (0,0,0), (2,1,0), (1,-3,0.5) -> cut y at -1
     cases.append(np.array([[0, 0, 0, 2, 1, 0, 1, -3, 0.5], [0, 0, 0, 1, 0, 0, 0.5, 1, 0]], np.float64))
