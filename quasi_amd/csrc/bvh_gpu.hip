@@ -972,9 +972,15 @@ __global__ void k_scene_arrays(const uint32_t* __restrict__ order, uint32_t n, c
 
 // fkey bounds of the finite corner coordinates, per axis, into b[0..2] (min)
 // and b[3..5] (max), which hold kIntMax and -kIntMax - 1 beforehand: the scene
-// box of the split threshold.
-__global__ void k_corner_bounds(const float* __restrict__ d0, const float* __restrict__ d1,
-                                const float* __restrict__ d2, uint32_t nt, int* __restrict__ b) {
+// box of the split threshold.  Blocks of 256 threads over a grid of at most
+// kBoundsBlocks, one atomic per block and bound.
+constexpr uint32_t kBoundsBlocks = 512;
+__global__ __launch_bounds__(256) void k_corner_bounds(const float* __restrict__ d0, const float* __restrict__ d1,
+                                                       const float* __restrict__ d2, uint32_t nt,
+                                                       int* __restrict__ b) {
+    __shared__ int s_b[6];
+    if (threadIdx.x < 6u) s_b[threadIdx.x] = threadIdx.x < 3u ? int(kIntMax) : -int(kIntMax) - 1;
+    __syncthreads();
     int lo[3] = {int(kIntMax), int(kIntMax), int(kIntMax)};
     int hi[3] = {-int(kIntMax) - 1, -int(kIntMax) - 1, -int(kIntMax) - 1};
     const float* corners[3] = {d0, d1, d2};
@@ -994,9 +1000,12 @@ __global__ void k_corner_bounds(const float* __restrict__ d0, const float* __res
     }
     if ((threadIdx.x & 63u) == 0u)
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&b[k], lo[k]);
-            atomicMax(&b[3 + k], hi[k]);
+            atomicMin(&s_b[k], lo[k]);
+            atomicMax(&s_b[3 + k], hi[k]);
         }
+    __syncthreads();
+    if (threadIdx.x < 3u) atomicMin(&b[threadIdx.x], s_b[threadIdx.x]);
+    else if (threadIdx.x < 6u) atomicMax(&b[threadIdx.x], s_b[threadIdx.x]);
 }
 
 constexpr int kSplitBlock = 256;
@@ -1170,7 +1179,8 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
                        kIntMax);
     hipLaunchKernelGGL(k_fill_u32_bvh, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(dbounds.p) + 3, size_t(3),
                        kIntMax + 1u);
-    hipLaunchKernelGGL(k_corner_bounds, dim3(grid256(nt)), dim3(256), 0, s, dp[0].p, dp[1].p, dp[2].p, nt, dbounds.p);
+    hipLaunchKernelGGL(k_corner_bounds, dim3(std::min(grid256(nt), kBoundsBlocks)), dim3(256), 0, s, dp[0].p, dp[1].p,
+                       dp[2].p, nt, dbounds.p);
     BVH_CHECK(hipGetLastError());
     int hb[6];
     BVH_CHECK(hipMemcpyAsync(hb, dbounds.p, sizeof hb, hipMemcpyDeviceToHost, s));
