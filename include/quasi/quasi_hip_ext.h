@@ -156,6 +156,10 @@ typedef struct Q_render_stats {
     uint32_t reserved2;
     double bvh_sah_cost;       /* SAH cost of the current BVH2 (traversal 1, triangle 1, root area 1) */
     double bvh_device_ms;      /* GPU build: hipEvent time of the build kernels */
+    uint32_t bvh_references;   /* triangle records in the BVH leaves: num_triangles plus the extra
+                                  references of triangles split before the build (a large
+                                  triangle is split into up to four with smaller boxes) */
+    uint32_t reserved3;
 } Q_render_stats;
 
 /* Replaces the hard-coded scene of plugin.mm:584 (cornell_box.hpp:95-134) with a

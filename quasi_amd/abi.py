@@ -110,7 +110,8 @@ class Q_render_stats(C.Structure):
                 ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64), ("bvh_build_ms", C.c_double),
                 ("render_ms", C.c_double), ("num_kernels", C.c_uint32), ("reserved", C.c_uint32),
                 ("kernels", Q_kernel_stat * Q_STATS_MAX_KERNELS), ("bvh_builder", C.c_uint32),
-                ("reserved2", C.c_uint32), ("bvh_sah_cost", C.c_double), ("bvh_device_ms", C.c_double)]
+                ("reserved2", C.c_uint32), ("bvh_sah_cost", C.c_double), ("bvh_device_ms", C.c_double),
+                ("bvh_references", C.c_uint32), ("reserved3", C.c_uint32)]
 
 
 class Q_batch_scene(C.Structure):

@@ -1133,6 +1133,7 @@ Q_EXPORT int32_t Q_plugin_get_stats(Q_plugin_handle* handle, Q_render_stats* out
         s.bvh_builder = st->bvh_builder;
         s.bvh_sah_cost = st->bvh_sah_cost;
         s.bvh_device_ms = st->bvh_device_ms;
+        s.bvh_references = st->num_prims;
         s.render_ms = 0;
         for (uint32_t k = 0; k < K_COUNT; ++k)
             if (kRenderKernel[k] && k != K_BATCH_PASS) s.render_ms += st->kernel_ms[k];

@@ -124,6 +124,63 @@ def test_nee_soup_bvh(ntri, w, h, spp, engine):
         assert abs(st.rays_shadow - c.shadow_rays) <= 1e-3 * c.shadow_rays
 
 
+def sliver_mesh(n, seed):
+    """Long thin triangles across the soup's cube (each far larger than the
+    average leaf cell, so the builders split nearly every one into four
+    references) under the soup's four lights."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1.0, 1.0, size=(n, 3))
+    b = a + rng.uniform(-0.6, 0.6, size=(n, 3))
+    c = b + rng.normal(scale=0.01, size=(n, 3))
+    tri = np.stack([a, b, c], axis=1).astype(np.float32)
+    bld = scenes._Builder()
+    bld.triangles(tri, rng.uniform(0.0, 1.0, size=(n, 3)).astype(np.float32), np.zeros((n, 3), np.float32))
+    for cx, cy, cz in scenes.SOUP_LIGHT_CENTERS:
+        bld.quad((cx - 0.25, cy, cz - 0.25), (0.5, 0, 0), (0, 0, 0.5), (0.0, 0.0, 0.0), scenes.LIGHT_E)
+    return bld.mesh(scenes.SOUP_CAMERA, f"slivers{n}")
+
+
+def test_split_references_match_oracle_on_slivers():
+    """A scene whose every sliver is split into references (bvh_split.hpp):
+    both builders give the same references and the same images, bit for bit,
+    and the image matches the oracle's (which traces its own unsplit BVH).
+    The oracle comparison is at depth 2 (camera rays, one bounce and its
+    shadow rays): these crossing slivers make deeper paths diverge between
+    FP32 and FP64 arithmetic (the oracle's own FP32 and FP64 renders differ by
+    7e-6 at depth 2, 8e-5 at depth 3 and 2.4e-4 at depth 8, against 5e-7 at
+    depth 8 on the soup), so depth 8 is compared between the builders only."""
+    mesh = sliver_mesh(20_000, 7)
+    w, h, spp = 128, 72, 4
+    params = dict(width=w, height=h, rng_mode=1, seed=0, camera=mesh.camera)
+    oscene, ref = oracle_render(mesh, w, h, spp, max_depth=2, rr_start=3, rng_mode=1, integrator=1, seed=0)
+    out = {}
+    for builder in (abi.Q_BVH_BUILD_HOST, abi.Q_BVH_BUILD_GPU):
+        say(f"[slivers] builder={builder} {mesh.num_triangles} triangles {w}x{h} spp={spp}")
+        with HipRenderer(w, h) as r:
+            r.set_params(bvh_builder=builder)
+            r.set_scene(mesh)
+            r.camera = cam_of(mesh)
+            common = dict(spp_per_render=spp, rr_start=3, rng_mode=1, integrator=1, seed=0,
+                          flags=abi.Q_FLAG_RECORD_IDS, id_sample=0, engine=abi.Q_ENGINE_WAVEFRONT,
+                          bvh_builder=builder)
+            r.set_params(max_depth=2, **common)
+            r.render(1)
+            st = r.stats()
+            assert st.bvh_builder == builder
+            say(f"[slivers] references {st.bvh_references} for {st.num_triangles} triangles, "
+                f"nodes {st.bvh_nodes}")
+            assert st.num_triangles == mesh.num_triangles
+            # the slivers split (four references each at most), the 8 light triangles stay whole
+            assert 3 * 20_000 < st.bvh_references <= 4 * 20_000 + 8
+            compare(r, ref, mesh, oscene, params, f"slivers-{builder}")
+            r.set_params(max_depth=8, **common)
+            r.render(1)
+            out[builder] = (st.bvh_references, r.readback_aov()["beauty"].copy(), r.primary_ids().copy())
+    (n_h, img_h, ids_h), (n_g, img_g, ids_g) = out[abi.Q_BVH_BUILD_HOST], out[abi.Q_BVH_BUILD_GPU]
+    assert n_h == n_g
+    assert np.array_equal(img_h, img_g) and np.array_equal(ids_h, ids_g)
+
+
 def test_render_is_deterministic_and_passes_compose():
     """Two identical renders are bitwise equal; 8 calls of 1 spp match one call
     of 8 spp (same sample indices) to FP32 summation-order rounding."""
