@@ -262,11 +262,15 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PassParams P, WaveState S, uin
 // cursor of the wave's current queue segment.  SHADOW selects the any-hit
 // shadow queue; otherwise the closest-hit queue.  Same visit order and tie
 // rule as traverse2 / traverse4.
-// OFF32 (node and triangle arrays, a batch's concatenated ones included, each
-// under 4 GiB): nodes and triangles are addressed as the array's base pointer,
-// uniform over the grid, plus a 32-bit byte offset (in a batch: the lane's
-// scene's offset plus the record's), which the loads take as they are;
-// otherwise each lane holds 64-bit pointers to its scene's arrays.
+// OFF32 (single scenes whose node and triangle arrays are each under 4 GiB):
+// nodes and triangles are addressed as the array's base pointer, uniform over
+// the grid, plus a 32-bit byte offset, which the loads take as they are (with
+// the record's field offsets as immediates); otherwise, and in every batch,
+// each lane holds 64-bit pointers to its scene's arrays.  A batch lane's
+// 32-bit scene offset added to the record's kept the compiler from folding the
+// field offsets (four more adds per node step): the 1M soup as a batch of one
+// ran at 4211 Mrays/s that way against 4398 with 64-bit pointers, the 64-scene
+// batch at 4591 against 4696 (73 VGPRs and 6 waves per SIMD against 71 and 7).
 
 // TOP (single-scene BVH4 kernels): the top levels of the tree (the root, its
 // internal children, theirs, ...: four levels, at most 85 nodes, 9.5 KB, in
@@ -352,6 +356,7 @@ __device__ __forceinline__ void stage_wave_top(const PassParams& P, uint32_t sid
 
 template <int WIDTH, int BLOCK, bool SHADOW, bool BATCH, bool OFF32, bool TOP = false, bool QUANT = false>
 __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState S, uint32_t b) {
+    static_assert(!(BATCH && OFF32), "a batch addresses its scenes through 64-bit pointers");
     constexpr bool kQuant = QUANT && WIDTH == 4;
     // 128-B BVH4 / 64-B quantised BVH4 / 64-B BVH2 nodes
     constexpr uint32_t kNodeShift = WIDTH == 4 && !kQuant ? 7u : 6u;
@@ -436,8 +441,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
     // base of the node and triangle arrays (a batch: the concatenated ones)
     const char* const nbase = static_cast<const char*>(kQuant ? P.qnodes4 : (WIDTH == 4 ? P.nodes4 : P.nodes));
     const char* const tbase = static_cast<const char*>(P.tris);
-    uint32_t nofs = 0, tofs = 0;                    // OFF32 batch: the lane's scene's byte offsets
-    const char* nodes = BATCH ? nullptr : nbase;    // otherwise: the lane's scene's arrays
+    const char* nodes = BATCH ? nullptr : nbase;    // the lane's scene's arrays (64-bit addressing)
     const char* tris = BATCH ? nullptr : tbase;
     for (;;) {
         // refill idle lanes (wave-uniform: every lane reaches the ballots)
@@ -462,15 +466,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
                 const uint32_t sid = BATCH ? __float_as_uint(SHADOW ? sc_a[idx].w : d.w) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 if (BATCH) {
-                    const char* sn = static_cast<const char*>(kQuant ? sc.qnodes4 : scene_nodes<WIDTH, true>(P, sid));
-                    const char* stt = static_cast<const char*>(sc.tris);
-                    if (OFF32) {
-                        nofs = uint32_t(sn - nbase);
-                        tofs = uint32_t(stt - tbase);
-                    } else {
-                        nodes = sn;
-                        tris = stt;
-                    }
+                    nodes = static_cast<const char*>(kQuant ? sc.qnodes4 : scene_nodes<WIDTH, true>(P, sid));
+                    tris = static_cast<const char*>(sc.tris);
                 }
                 r = Ray{xyz(o), xyz(d)};
                 inv = mk(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
@@ -517,7 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             if (leave_node_loop(desc && leaf == 0, busy && leaf != 0)) break;
             if (!desc) continue;
             // the node's record: base + 32-bit offset (OFF32) or a 64-bit address
-            const uint32_t off = nofs + (uint32_t(node) << kNodeShift);
+            const uint32_t off = uint32_t(node) << kNodeShift;
             const char* rec = nodes + (OFF32 ? size_t(0) : (size_t(uint32_t(node)) << kNodeShift));
             auto ld = [&](uint32_t o) -> float4 {
                 return OFF32 ? *reinterpret_cast<const float4*>(nbase + (off + o))
@@ -591,7 +588,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dynamic(PassParams P, WaveState
             const int first = int(v >> 2);
             const int count = int(v & 3u) + 1;
             bool found = false;
-            uint32_t toff = tofs + uint32_t(first) * 48u;   // 48-B triangle records
+            uint32_t toff = uint32_t(first) * 48u;   // 48-B triangle records
             const char* trec = tris + (OFF32 ? size_t(0) : size_t(uint32_t(first)) * 48u);
             for (int i = first; i < first + count; ++i, toff += 48u, trec += OFF32 ? 0 : 48) {
                 float t;
@@ -739,17 +736,19 @@ struct TraceVariant {
         hipLaunchKernelGGL((k_trace_flat<WIDTH, BLOCK, SHADOW, BATCH>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         return hipGetLastError();
     }
-    // node and triangle arrays (a batch's concatenated ones) each under 4 GiB
+    // node and triangle arrays each under 4 GiB (single scenes)
     static bool off32(const PassParams& p) {
         const uint64_t node_bytes = uint64_t(WIDTH == 4 ? p.num_nodes4 : p.num_nodes) * (WIDTH == 4 && !kQ ? 128u : 64u);
         return node_bytes < (1ull << 32) && uint64_t(p.num_tris) * 48u < (1ull << 32);
     }
     template <bool SHADOW, bool BATCH>
     static void dynamic(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
-        if (off32(p))
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, true, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        if constexpr (BATCH)
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, true, false, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+        else if (off32(p))
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, false, true, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
         else
-            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, BATCH, false, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
+            hipLaunchKernelGGL((k_trace_dynamic<WIDTH, BLOCK, SHADOW, false, false, kTop, kQ>), dim3(blocks), dim3(BLOCK), 0, st, p, s, b);
     }
     template <bool BATCH>
     static hipError_t extend(const PassParams& p, const WaveState& s, uint32_t b, uint32_t blocks, hipStream_t st) {
