@@ -1193,22 +1193,28 @@ bool build_scene_gpu(const float* verts, uint32_t nv, const uint32_t* idx, uint3
         std::memcpy(&slo[k], &a, 4);
         std::memcpy(&shi[k], &b, 4);
     }
-    const float T = finite_box ? split_threshold(slo, shi, nt) : 3.0e38f;
+    float T = finite_box ? split_threshold(slo, shi, nt) : 3.0e38f;
     const uint32_t nblk = (nt + kSplitBlock - 1) / kSplitBlock;
     DevBuf<uint32_t> bsum, boff, tri_off, ref_tri;
     DevBuf<float> rlo, rhi;
     if (!bsum.alloc(nblk) || !boff.alloc(nblk) || !tri_off.alloc(nt)) { *why = "allocation"; return false; }
-    hipLaunchKernelGGL(k_split_count, dim3(nblk), dim3(kSplitBlock), 0, s, dp[0].p, dp[1].p, dp[2].p, demi.p, nt, T,
-                       bsum.p);
-    BVH_CHECK(hipGetLastError());
     std::vector<uint32_t> hsum(nblk);
-    BVH_CHECK(hipMemcpyAsync(hsum.data(), bsum.p, 4 * size_t(nblk), hipMemcpyDeviceToHost, s));
-    BVH_CHECK(hipStreamSynchronize(s));
     uint64_t total = 0;
-    for (uint32_t b = 0; b < nblk; ++b) {
-        const uint32_t c = hsum[b];
-        hsum[b] = uint32_t(total);
-        total += c;
+    // references beyond the builder's limit: build the triangles unsplit
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        hipLaunchKernelGGL(k_split_count, dim3(nblk), dim3(kSplitBlock), 0, s, dp[0].p, dp[1].p, dp[2].p, demi.p, nt,
+                           T, bsum.p);
+        BVH_CHECK(hipGetLastError());
+        BVH_CHECK(hipMemcpyAsync(hsum.data(), bsum.p, 4 * size_t(nblk), hipMemcpyDeviceToHost, s));
+        BVH_CHECK(hipStreamSynchronize(s));
+        total = 0;
+        for (uint32_t b = 0; b < nblk; ++b) {
+            const uint32_t c = hsum[b];
+            hsum[b] = uint32_t(total);
+            total += c;
+        }
+        if (total <= (1u << 29)) break;
+        T = 3.0e38f;
     }
     if (total < nt || total > (1u << 29)) { *why = "reference count"; return false; }
     const uint32_t nr = uint32_t(total);

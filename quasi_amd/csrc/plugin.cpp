@@ -287,24 +287,25 @@ int32_t prepare_scene(plugin_state* st, const float* verts, uint32_t nv, const u
             shi[i % 3] = std::max(shi[i % 3], v);
         }
     const bool finite_box = slo[0] <= shi[0] && slo[1] <= shi[1] && slo[2] <= shi[2];
-    const float T = finite_box ? split_threshold(slo, shi, nt) : 3.0e38f;
+    float T = finite_box ? split_threshold(slo, shi, nt) : 3.0e38f;
     auto emits = [&](size_t t) {
         uint32_t b[3];
         std::memcpy(b, &emission[3 * t], 12);
         return (b[0] | b[1] | b[2]) != 0u;
     };
-    std::vector<uint32_t> tri_off(size_t(nt) + 1, 0);
-    parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
-        PieceBox pc[kMaxPieces];
-        for (size_t t = tb; t < te; ++t)
-            tri_off[t + 1] = split_triangle(&p0[3 * t], &p1[3 * t], &p2[3 * t], T, !emits(t), pc);
-    });
-    for (size_t t = 0; t < nt; ++t) tri_off[t + 1] += tri_off[t];
-    if (tri_off[nt] > (1u << 29)) {
-        log_msg(st, "scene: %llu BVH references exceed the builder's limit", (unsigned long long)tri_off[nt]);
-        return Q_EXT_ERR_UNSUPPORTED;
+    std::vector<uint64_t> tri_off(size_t(nt) + 1, 0);
+    // references beyond the builder's limit: build the triangles unsplit
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
+            PieceBox pc[kMaxPieces];
+            for (size_t t = tb; t < te; ++t)
+                tri_off[t + 1] = split_triangle(&p0[3 * t], &p1[3 * t], &p2[3 * t], T, !emits(t), pc);
+        });
+        for (size_t t = 0; t < nt; ++t) tri_off[t + 1] += tri_off[t];
+        if (tri_off[nt] <= (1u << 29)) break;
+        T = 3.0e38f;
     }
-    const uint32_t nr = tri_off[nt];
+    const uint32_t nr = uint32_t(tri_off[nt]);
     std::vector<float> rlo(3 * size_t(nr)), rhi(3 * size_t(nr));
     std::vector<uint32_t> ref_tri(nr);
     parallel_ranges(nt, host_threads, [&](size_t tb, size_t te) {
