@@ -90,8 +90,9 @@ constexpr uint32_t queue_counter(uint32_t group, uint32_t seg) {
 }
 
 // Wavefront path state.  Per-path arrays are indexed by the path id
-// p = sample_in_pass * W*H + pixel; per-slot arrays by the position in the
-// segmented queue of the current bounce (double buffered by bounce parity).
+// p = pixel * samples_in_pass + sample_in_pass; per-slot arrays by the position
+// in the segmented queue of the current bounce (double buffered by bounce
+// parity).
 struct WaveState {
     void* ray_o[2];        // float4 per slot: origin.xyz, w = path id (bits)
     void* ray_d[2];        // float4 per slot: direction.xyz
@@ -120,10 +121,11 @@ struct WaveState {
 // (PassParams::scene0); a batched pass reads a device table of them.
 // Scene i owns paths [path_begin, path_begin + width*height*samples) of the
 // pass and pixels [pix_begin, pix_begin + width*height) of the framebuffer
-// arrays; path p_local = s*width*height + y*width + x.
+// arrays; path p_local = (y*width + x)*samples + s (pixel-major: a pixel's
+// samples are neighbours in the camera queue, see path_pixel_sample).
 // A pass may cover a band of rows [row_begin, row_begin + rows) of the image
-// only (rows = 0: every row): its paths are then p_local = s*width*rows +
-// (y - row_begin)*width + x, while the camera and the RNG streams stay those of
+// only (rows = 0: every row): its paths are then p_local = ((y - row_begin)*
+// width + x)*samples + s, while the camera and the RNG streams stay those of
 // the whole width x height image.
 struct SceneDesc {
     const void* tris;

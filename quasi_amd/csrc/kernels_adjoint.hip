@@ -163,8 +163,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(AdjointParams A) {
                 uint32_t rng = path_seed(P.rng_mode, x, y, W, H, P.sample_begin + s, P.seed);
                 Ray ray = camera_ray(P.cam, W, H, x, y, rng);
                 f3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
-                // the path's id in the recorded pass (sample-major, as the wavefront engine numbers paths)
-                const size_t pid = (size_t(s) * rows + (y - row0)) * W + x;
+                // the path's id in the recorded pass (pixel-major, as the wavefront engine numbers paths)
+                const size_t pid = (size_t(y - row0) * W + x) * P.sample_count + s;
                 for (uint32_t b = 0; b < P.max_depth; ++b) {
                     // the recorded closest hit, distance and triangle as traced
                     const uint2 rh = rec_hit[size_t(b) * A.rec_paths + pid];

@@ -122,12 +122,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                     }
                 }
                 scene_empty = sc.num_nodes == 0u;
-                // the camera ray of path p_local = s*width*rows + (y - row_begin)*width + x
-                const uint32_t rows = sc.rows ? sc.rows : sc.height;
-                const uint32_t npix = sc.width * rows;
-                const uint32_t pl = path - sc.path_begin;
-                const uint32_t s = pl / npix;
-                const uint32_t pix = pl - s * npix;
+                // the camera ray of path p_local = ((y - row_begin)*width + x)*samples + s
+                uint32_t pix, s;
+                path_pixel_sample(path - sc.path_begin, P.sample_count, pix, s);
                 const uint32_t yl = pix / sc.width;
                 const uint32_t x = pix - yl * sc.width;
                 const uint32_t y = sc.row_begin + yl;

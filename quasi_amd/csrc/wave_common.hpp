@@ -35,6 +35,18 @@ __device__ __forceinline__ uint32_t scene_of_pixel(const PassParams& P, uint32_t
     return lo;
 }
 
+// Pixel (row-major within the scene's band) and sample of the pass's path
+// p_local = pixel * samples + sample.  A pixel's samples are neighbours in the
+// camera queue, so a wave traces one pixel's camera rays together: on the 1M
+// soup at 1080p, 64 spp, extend took 3.5 ms less per frame than with the
+// sample-major numbering (sample * pixels + pixel), whose queue order puts 64
+// pixels of a row in a wave.
+__device__ __forceinline__ void path_pixel_sample(uint32_t p_local, uint32_t samples, uint32_t& pix,
+                                                  uint32_t& s) {
+    pix = p_local / samples;
+    s = p_local - pix * samples;
+}
+
 __device__ __forceinline__ uint32_t pass_sample_begin(const PassParams& P) {
     return P.dyn_sample_begin ? *P.dyn_sample_begin : P.sample_begin;
 }
