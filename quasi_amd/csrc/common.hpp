@@ -89,6 +89,38 @@ constexpr uint32_t queue_counter(uint32_t group, uint32_t seg) {
     return (group * kQueueSegments + seg) * kCounterStride;
 }
 
+// The camera bounce deals a pass's path ids to the queue segments in chunks of
+// kQueueChunk consecutive ids, round robin (chunk k to segment k %
+// kQueueSegments, after the segment's earlier chunks), so that every segment
+// holds a sample of the whole pass while a wave still takes consecutive paths
+// (one pixel's samples).  Contiguous ranges made each segment an image band
+// of its own cost: on a sphere on a ground plane (512^2, 32 spp) extend and
+// shadow took 1.7x as long, the waves of cheap bands all ending up on the
+// cursors of the dear ones.  A path keeps its segment on later bounces.
+constexpr uint32_t kQueueChunk = 64;
+// Slots per segment: the chunks a segment can receive, whole.
+constexpr uint32_t queue_seg_cap(uint32_t paths) {
+    return ((paths + kQueueChunk - 1) / kQueueChunk + kQueueSegments - 1) / kQueueSegments * kQueueChunk;
+}
+// Slots of all segments (at most paths + kQueueSegments * kQueueChunk - 1).
+constexpr uint64_t queue_slots(uint32_t paths) { return uint64_t(queue_seg_cap(paths)) * kQueueSegments; }
+// Camera-bounce slot of path p, and the path of slot i (i below its segment's length).
+constexpr uint32_t queue_slot_of_path(uint32_t p, uint32_t seg_cap) {
+    return (p / kQueueChunk) % kQueueSegments * seg_cap + (p / kQueueChunk) / kQueueSegments * kQueueChunk +
+           p % kQueueChunk;
+}
+constexpr uint32_t queue_path_of_slot(uint32_t i, uint32_t seg_cap) {
+    return ((i % seg_cap) / kQueueChunk * kQueueSegments + i / seg_cap) * kQueueChunk + i % seg_cap % kQueueChunk;
+}
+// Camera-bounce length of segment seg: its chunks seg, seg + kQueueSegments,
+// ..., the pass's last chunk possibly short.
+constexpr uint32_t queue_segment_paths(uint32_t paths, uint32_t seg) {
+    const uint32_t chunks = (paths + kQueueChunk - 1) / kQueueChunk;
+    if (seg >= chunks) return 0u;
+    const uint32_t n = ((chunks - 1 - seg) / kQueueSegments + 1) * kQueueChunk;
+    return (chunks - 1) % kQueueSegments == seg ? n - (chunks * kQueueChunk - paths) : n;
+}
+
 // Wavefront path state.  Per-path arrays are indexed by the path id
 // p = pixel * samples_in_pass + sample_in_pass; per-slot arrays by the position
 // in the segmented queue of the current bounce (double buffered by bounce
@@ -109,7 +141,7 @@ struct WaveState {
     uint32_t* shadow_counts;  // shadow queue lengths: group b (0..kMaxDepth-1), per segment
     uint32_t* fetch;       // work cursors: group 2b (extend) and 2b+1 (shadow), per segment
     uint32_t paths;        // paths in this pass
-    uint32_t seg_cap;      // slots per queue segment: ceil(paths / kQueueSegments)
+    uint32_t seg_cap;      // slots per queue segment: queue_seg_cap(paths)
     // float4 per slot, double buffered like the rays: throughput.xyz, w = RNG
     // state (bits) of the path in that slot, written by the shade that queues
     // the ray and read by the next bounce's shade in the same (coalesced)

@@ -27,14 +27,13 @@ namespace quasi_hip {
 
 namespace {
 
-// Segment lengths and cursors of the path queue: segment j holds path ids
-// [j * seg_cap, min((j + 1) * seg_cap, paths)).
+// Segment lengths and cursors of the path queue: the path ids dealt to the
+// segments in chunks, as the wavefront engine's camera bounce does
+// (queue_slot_of_path, common.hpp).
 __global__ void k_path_queue(WaveState S) {
     const uint32_t seg = threadIdx.x;
     if (seg >= kQueueSegments) return;
-    const uint64_t lo = uint64_t(seg) * S.seg_cap;
-    const uint32_t n = lo < S.paths ? uint32_t(S.paths - lo < S.seg_cap ? S.paths - lo : S.seg_cap) : 0u;
-    S.counts[queue_counter(0, seg)] = n;
+    S.counts[queue_counter(0, seg)] = queue_segment_paths(S.paths, seg);
     S.fetch[queue_counter(0, seg)] = 0u;
 }
 
@@ -107,7 +106,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             const bool mine = ((want >> (threadIdx.x & 63u)) & 1ull) && lb < got;
             want &= ~__ballot(mine);
             if (mine) {
-                path = first + lb;
+                path = queue_path_of_slot(first + lb, S.seg_cap);
                 sid = BATCH ? scene_of_path(P, path) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 if (BATCH) {

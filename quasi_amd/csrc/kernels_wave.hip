@@ -28,17 +28,13 @@ namespace {
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
-    // queue lengths and cursors of the pass: the camera rays fill segment j
-    // with paths [j * seg_cap, ...) at slots equal to their path ids
+    // queue lengths and cursors of the pass: the camera rays fill the
+    // segments in chunks of consecutive paths (queue_slot_of_path)
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < 2 * kMaxDepth * kQueueSegments;
          e += gridDim.x * blockDim.x) {
         const uint32_t seg = e % kQueueSegments;
-        const uint64_t lo = uint64_t(seg) * S.seg_cap;
-        if (e < (kMaxDepth + 1) * kQueueSegments) {
-            uint32_t v = 0u;
-            if (e < kQueueSegments && lo < S.paths) v = uint32_t(S.paths - lo < S.seg_cap ? S.paths - lo : S.seg_cap);
-            S.counts[e * kCounterStride] = v;
-        }
+        if (e < (kMaxDepth + 1) * kQueueSegments)
+            S.counts[e * kCounterStride] = e < kQueueSegments ? queue_segment_paths(S.paths, seg) : 0u;
         if (e < kMaxDepth * kQueueSegments) S.shadow_counts[e * kCounterStride] = 0u;
         S.fetch[e * kCounterStride] = 0u;
     }
@@ -57,8 +53,9 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
         const uint32_t y = sc.row_begin + yl;
         uint32_t rng = path_seed(P.rng_mode, x, y, sc.width, sc.height, s0 + s, sc.seed);
         const Ray r = camera_ray(sc.cam, sc.width, sc.height, x, y, rng);
-        ro[p] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p));
-        rd[p] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(sid));
+        const uint32_t slot = queue_slot_of_path(p, S.seg_cap);
+        ro[slot] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p));
+        rd[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(sid));
         S.rng[p] = rng;
     }
 }

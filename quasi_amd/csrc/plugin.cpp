@@ -564,8 +564,8 @@ bool ensure_wave(plugin_state* st, uint32_t paths) {
     WaveBuffers& w = st->wave;
     if (w.capacity >= paths) return true;
     const size_t n = paths;
-    // queue slots: kQueueSegments segments of ceil(n / kQueueSegments) slots
-    const size_t q = n + kQueueSegments;
+    // queue slots: kQueueSegments segments of queue_seg_cap(n) slots
+    const size_t q = size_t(queue_slots(paths));
     const size_t group = size_t(kQueueSegments) * kCounterStride * 4;   // bytes per counter group
     bool ok = true;
     for (int k = 0; k < 2; ++k) {
@@ -606,7 +606,7 @@ WaveState wave_state(const WaveBuffers& w, uint32_t paths) {
     s.shadow_counts = static_cast<uint32_t*>(w.shadow_counts.ptr);
     s.fetch = static_cast<uint32_t*>(w.fetch.ptr);
     s.paths = paths;
-    s.seg_cap = (paths + kQueueSegments - 1) / kQueueSegments;
+    s.seg_cap = queue_seg_cap(paths);
     return s;
 }
 
