@@ -90,35 +90,48 @@ constexpr uint32_t queue_counter(uint32_t group, uint32_t seg) {
 }
 
 // The camera bounce deals a pass's path ids to the queue segments in chunks of
-// kQueueChunk consecutive ids, round robin (chunk k to segment k %
-// kQueueSegments, after the segment's earlier chunks), so that every segment
-// holds a sample of the whole pass while a wave still takes consecutive paths
-// (one pixel's samples).  Contiguous ranges made each segment an image band
-// of its own cost: on a sphere on a ground plane (512^2, 32 spp) extend and
-// shadow took 1.7x as long, the waves of cheap bands all ending up on the
-// cursors of the dear ones.  A path keeps its segment on later bounces.
-constexpr uint32_t kQueueChunk = 64;
+// `chunk` consecutive ids (a multiple of 64), round robin: chunk k to segment
+// k % kQueueSegments, after the segment's earlier chunks.  A path keeps its
+// segment on later bounces.  Single-scene passes deal chunks of
+// kQueueChunkScene paths, so that every segment holds a sample of the whole
+// image (contiguous ranges made each segment an image band of its own cost:
+// on a sphere on a ground plane, 512^2 at 32 spp, the forward pass took 5.3
+// ms against 3.2, the waves of cheap bands all ending up on the cursors of
+// the dear ones).  Batches keep contiguous ranges, one chunk per segment
+// (queue_contiguous_chunk), so that a segment's rays come from one scene or
+// a few: dealt in chunks of 64 to 1024 paths the 64-scene batch ran 12-13%
+// slower.
+constexpr uint32_t kQueueChunkScene = 1024;
+constexpr uint32_t queue_contiguous_chunk(uint32_t paths) {
+    return ((paths + kQueueSegments - 1) / kQueueSegments + 63u) / 64u * 64u;
+}
 // Slots per segment: the chunks a segment can receive, whole.
-constexpr uint32_t queue_seg_cap(uint32_t paths) {
-    return ((paths + kQueueChunk - 1) / kQueueChunk + kQueueSegments - 1) / kQueueSegments * kQueueChunk;
+constexpr uint32_t queue_seg_cap(uint32_t paths, uint32_t chunk) {
+    return ((paths + chunk - 1) / chunk + kQueueSegments - 1) / kQueueSegments * chunk;
 }
-// Slots of all segments (at most paths + kQueueSegments * kQueueChunk - 1).
-constexpr uint64_t queue_slots(uint32_t paths) { return uint64_t(queue_seg_cap(paths)) * kQueueSegments; }
+// Slots of all segments: at most paths + kQueueSegments * chunk - 1.
+constexpr uint64_t queue_slots(uint32_t paths, uint32_t chunk) {
+    return uint64_t(queue_seg_cap(paths, chunk)) * kQueueSegments;
+}
+// Slots allocated for passes of up to `paths` paths, either layout.
+constexpr uint64_t queue_slots_max(uint32_t paths) {
+    const uint64_t a = queue_slots(paths, kQueueChunkScene), b = queue_slots(paths, queue_contiguous_chunk(paths));
+    return a > b ? a : b;
+}
 // Camera-bounce slot of path p, and the path of slot i (i below its segment's length).
-constexpr uint32_t queue_slot_of_path(uint32_t p, uint32_t seg_cap) {
-    return (p / kQueueChunk) % kQueueSegments * seg_cap + (p / kQueueChunk) / kQueueSegments * kQueueChunk +
-           p % kQueueChunk;
+constexpr uint32_t queue_slot_of_path(uint32_t p, uint32_t seg_cap, uint32_t chunk) {
+    return (p / chunk) % kQueueSegments * seg_cap + (p / chunk) / kQueueSegments * chunk + p % chunk;
 }
-constexpr uint32_t queue_path_of_slot(uint32_t i, uint32_t seg_cap) {
-    return ((i % seg_cap) / kQueueChunk * kQueueSegments + i / seg_cap) * kQueueChunk + i % seg_cap % kQueueChunk;
+constexpr uint32_t queue_path_of_slot(uint32_t i, uint32_t seg_cap, uint32_t chunk) {
+    return ((i % seg_cap) / chunk * kQueueSegments + i / seg_cap) * chunk + i % seg_cap % chunk;
 }
 // Camera-bounce length of segment seg: its chunks seg, seg + kQueueSegments,
 // ..., the pass's last chunk possibly short.
-constexpr uint32_t queue_segment_paths(uint32_t paths, uint32_t seg) {
-    const uint32_t chunks = (paths + kQueueChunk - 1) / kQueueChunk;
+constexpr uint32_t queue_segment_paths(uint32_t paths, uint32_t seg, uint32_t chunk) {
+    const uint32_t chunks = (paths + chunk - 1) / chunk;
     if (seg >= chunks) return 0u;
-    const uint32_t n = ((chunks - 1 - seg) / kQueueSegments + 1) * kQueueChunk;
-    return (chunks - 1) % kQueueSegments == seg ? n - (chunks * kQueueChunk - paths) : n;
+    const uint32_t n = ((chunks - 1 - seg) / kQueueSegments + 1) * chunk;
+    return (chunks - 1) % kQueueSegments == seg ? n - (chunks * chunk - paths) : n;
 }
 
 // Wavefront path state.  Per-path arrays are indexed by the path id
@@ -141,7 +154,8 @@ struct WaveState {
     uint32_t* shadow_counts;  // shadow queue lengths: group b (0..kMaxDepth-1), per segment
     uint32_t* fetch;       // work cursors: group 2b (extend) and 2b+1 (shadow), per segment
     uint32_t paths;        // paths in this pass
-    uint32_t seg_cap;      // slots per queue segment: queue_seg_cap(paths)
+    uint32_t seg_cap;      // slots per queue segment: queue_seg_cap(paths, qchunk)
+    uint32_t qchunk;       // paths per chunk the camera bounce deals to a segment
     // float4 per slot, double buffered like the rays: throughput.xyz, w = RNG
     // state (bits) of the path in that slot, written by the shade that queues
     // the ray and read by the next bounce's shade in the same (coalesced)

@@ -33,7 +33,7 @@ namespace {
 __global__ void k_path_queue(WaveState S) {
     const uint32_t seg = threadIdx.x;
     if (seg >= kQueueSegments) return;
-    S.counts[queue_counter(0, seg)] = queue_segment_paths(S.paths, seg);
+    S.counts[queue_counter(0, seg)] = queue_segment_paths(S.paths, seg, S.qchunk);
     S.fetch[queue_counter(0, seg)] = 0u;
 }
 
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             const bool mine = ((want >> (threadIdx.x & 63u)) & 1ull) && lb < got;
             want &= ~__ballot(mine);
             if (mine) {
-                path = queue_path_of_slot(first + lb, S.seg_cap);
+                path = queue_path_of_slot(first + lb, S.seg_cap, S.qchunk);
                 sid = BATCH ? scene_of_path(P, path) : 0u;
                 const SceneDesc& sc = scene_at<BATCH>(P, sid);
                 if (BATCH) {

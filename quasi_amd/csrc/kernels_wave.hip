@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
          e += gridDim.x * blockDim.x) {
         const uint32_t seg = e % kQueueSegments;
         if (e < (kMaxDepth + 1) * kQueueSegments)
-            S.counts[e * kCounterStride] = e < kQueueSegments ? queue_segment_paths(S.paths, seg) : 0u;
+            S.counts[e * kCounterStride] = e < kQueueSegments ? queue_segment_paths(S.paths, seg, S.qchunk) : 0u;
         if (e < kMaxDepth * kQueueSegments) S.shadow_counts[e * kCounterStride] = 0u;
         S.fetch[e * kCounterStride] = 0u;
     }
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_raygen(PassParams P, WaveState S) {
         const uint32_t y = sc.row_begin + yl;
         uint32_t rng = path_seed(P.rng_mode, x, y, sc.width, sc.height, s0 + s, sc.seed);
         const Ray r = camera_ray(sc.cam, sc.width, sc.height, x, y, rng);
-        const uint32_t slot = queue_slot_of_path(p, S.seg_cap);
+        const uint32_t slot = queue_slot_of_path(p, S.seg_cap, S.qchunk);
         ro[slot] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p));
         rd[slot] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(sid));
         S.rng[p] = rng;

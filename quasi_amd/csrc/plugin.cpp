@@ -564,8 +564,8 @@ bool ensure_wave(plugin_state* st, uint32_t paths) {
     WaveBuffers& w = st->wave;
     if (w.capacity >= paths) return true;
     const size_t n = paths;
-    // queue slots: kQueueSegments segments of queue_seg_cap(n) slots
-    const size_t q = size_t(queue_slots(paths));
+    // queue slots: enough for either layout (single scene or batch)
+    const size_t q = size_t(queue_slots_max(paths));
     const size_t group = size_t(kQueueSegments) * kCounterStride * 4;   // bytes per counter group
     bool ok = true;
     for (int k = 0; k < 2; ++k) {
@@ -586,7 +586,7 @@ bool ensure_wave(plugin_state* st, uint32_t paths) {
     return true;
 }
 
-WaveState wave_state(const WaveBuffers& w, uint32_t paths) {
+WaveState wave_state(const WaveBuffers& w, uint32_t paths, bool batch) {
     WaveState s{};
     for (int k = 0; k < 2; ++k) {
         s.ray_o[k] = w.ray_o[k].ptr;
@@ -606,7 +606,8 @@ WaveState wave_state(const WaveBuffers& w, uint32_t paths) {
     s.shadow_counts = static_cast<uint32_t*>(w.shadow_counts.ptr);
     s.fetch = static_cast<uint32_t*>(w.fetch.ptr);
     s.paths = paths;
-    s.seg_cap = queue_seg_cap(paths);
+    s.qchunk = batch ? queue_contiguous_chunk(paths) : kQueueChunkScene;
+    s.seg_cap = queue_seg_cap(paths, s.qchunk);
     return s;
 }
 
@@ -763,7 +764,7 @@ bool render_frame(plugin_state* st, const Q_render_frame* frame) {
             const uint32_t n = std::min(per_pass, rp.spp_per_render - done);
             p.sample_begin = st->sample_count + done;
             p.sample_count = n;
-            const WaveState ws = wave_state(st->wave, n * npix);
+            const WaveState ws = wave_state(st->wave, n * npix, false);
             if (rp.engine == Q_ENGINE_PERSISTENT) {
                 LaunchTimer lt(st, s, K_PATHS);
                 if (!hip_ok(st, launch_paths_pass(p, ws, st->trace_cfg, st->paths_blocks, s), "single-launch pass"))

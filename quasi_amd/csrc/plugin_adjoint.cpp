@@ -205,7 +205,7 @@ int32_t adjoint(plugin_state* st, const Q_adjoint_desc* d, Q_adjoint_result* res
             A.row_begin = sd.row_begin;
             A.rows = sd.rows;
             A.rec_paths = uint32_t(size_t(n) * st->width * rows);
-            const WaveState ws = wave_state(st->wave, A.rec_paths);
+            const WaveState ws = wave_state(st->wave, A.rec_paths, false);
             ok = hip_ok(st, launch_wavefront_pass(p, ws, st->trace_cfg, st->trace_blocks, s, nullptr),
                         "adjoint record pass") &&
                  hip_ok(st, launch_adjoint(A, blocks, s), "adjoint");

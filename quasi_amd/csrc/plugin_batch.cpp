@@ -151,7 +151,7 @@ bool prepare_batch_pass(plugin_state* st, uint32_t spp_pass) {
         !hip_ok(st, hipStreamCreateWithFlags(&st->capture_stream, hipStreamNonBlocking), "capture stream"))
         return false;
     const PassParams p = batch_params(st, spp_pass);
-    const WaveState ws = wave_state(st->wave, uint32_t(paths));
+    const WaveState ws = wave_state(st->wave, uint32_t(paths), true);
     if (!hip_ok(st, hipStreamBeginCapture(st->capture_stream, hipStreamCaptureModeThreadLocal), "begin capture"))
         return false;
     // the batch renders with the single-launch pass under Q_ENGINE_PERSISTENT,

@@ -70,9 +70,9 @@ Layout layout_for(const plugin_state* st, uint32_t gstack_entries) {
     L.trace_blocks = blocks;
     L.path_state = align_up(L.gstack + per_block * blocks);
     const size_t left = Q_NOALLOC_SCRATCH_BYTES > L.path_state ? Q_NOALLOC_SCRATCH_BYTES - L.path_state : 0;
-    // 5 per-path arrays and 8 per-slot arrays (queue_slots(paths): at most
-    // paths + kQueueSegments * kQueueChunk slots), each starting on a kAlign boundary
-    const size_t fixed = 13 * kAlign + kPerSlot * size_t(kQueueSegments) * kQueueChunk;
+    // 5 per-path arrays and 8 per-slot arrays (queue_slots(paths, kQueueChunkScene):
+    // at most paths + kQueueSegments * kQueueChunkScene slots), each starting on a kAlign boundary
+    const size_t fixed = 13 * kAlign + kPerSlot * size_t(kQueueSegments) * kQueueChunkScene;
     L.max_paths = left > fixed ? uint32_t(std::min<size_t>((left - fixed) / (kPerPath + kPerSlot), 0x7FFFFFFFu)) : 0u;
     return L;
 }
@@ -81,7 +81,7 @@ Layout layout_for(const plugin_state* st, uint32_t gstack_entries) {
 // its last byte.
 WaveState carve(unsigned char* base, const Layout& L, uint32_t paths, size_t& end) {
     WaveState s{};
-    const size_t n = paths, q = size_t(queue_slots(paths));
+    const size_t n = paths, q = size_t(queue_slots(paths, kQueueChunkScene));
     size_t off = L.path_state;
     auto take = [&](size_t bytes) {
         void* p = base + off;
@@ -107,7 +107,8 @@ WaveState carve(unsigned char* base, const Layout& L, uint32_t paths, size_t& en
     s.shadow_counts = reinterpret_cast<uint32_t*>(c + kCounterGroup * (kMaxDepth + 1));
     s.fetch = reinterpret_cast<uint32_t*>(c + kCounterGroup * (2 * kMaxDepth + 1));
     s.paths = paths;
-    s.seg_cap = queue_seg_cap(paths);
+    s.qchunk = kQueueChunkScene;
+    s.seg_cap = queue_seg_cap(paths, s.qchunk);
     return s;
 }
 

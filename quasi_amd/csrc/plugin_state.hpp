@@ -193,7 +193,8 @@ uint32_t stack_overflow_entries(uint32_t depth, uint32_t depth4);
 bool ensure_gstack(plugin_state* st, uint32_t entries);
 
 bool ensure_wave(plugin_state* st, uint32_t paths);
-WaveState wave_state(const WaveBuffers& w, uint32_t paths);
+// The path state of a pass of `paths` paths (batch: contiguous queue segments).
+WaveState wave_state(const WaveBuffers& w, uint32_t paths, bool batch);
 void wave_timer_before(void* user, uint32_t kernel, hipStream_t s);
 void wave_timer_after(void* user, uint32_t kernel, hipStream_t s);
 
