@@ -657,13 +657,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace_flat(PassParams P, WaveState S,
 // Per pixel: the pass's samples in sample order, added to the accumulation.
 // A pixel's samples are neighbouring paths (p = gp * samples + s: every scene
 // of a batch starts at path pix_begin * samples), so a block stages the
-// records of a tile of pixels in LDS with coalesced loads, one row of (at most
-// kAccRow) samples per pixel padded by one float4 against bank conflicts, and
-// each thread then adds its pixel's samples in order.  Reading them straight
-// from HBM, one pixel per thread, strides the loads by samples x 16 B and took
-// 6.6 ms per 64-spp frame at 1080p against 1.3 before pixel-major paths.
+// records of a tile of up to 64 pixels in LDS with coalesced loads, in rounds
+// of kAccRow samples per pixel (one row per pixel, padded by one float4
+// against bank conflicts).  Each of the block's first three waves then adds
+// one staged array (radiance; albedo and depth; normal and the id sample) for
+// the tile's pixels, one pixel per lane, every component in sample order, so
+// the sums are those of a sequential per-pixel loop.  Reading the records
+// straight from HBM, one pixel per thread, strides the loads by samples x 16 B
+// and took 6.6 ms per 64-spp frame at 1080p.
 constexpr uint32_t kAccEntries = 1040;   // float4 per staged array (16.25 KB)
-constexpr uint32_t kAccRow = 64;         // samples staged per pixel and round
+constexpr uint32_t kAccRow = 16;         // samples staged per pixel and round
+constexpr uint32_t kAccTile = 64;        // pixels per tile at most (one wave's lanes)
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_accumulate(PassParams P, WaveState S) {
     __shared__ float4 s_rad[kAccEntries], s_a0[kAccEntries], s_a1[kAccEntries];
@@ -672,49 +676,51 @@ __global__ __launch_bounds__(256) void k_accumulate(PassParams P, WaveState S) {
     const float4* rad = static_cast<const float4*>(S.radiance);
     const float4* a0 = static_cast<const float4*>(S.aov0);
     const float4* a1 = static_cast<const float4*>(S.aov1);
-    float4* ab = static_cast<float4*>(P.accum_beauty);
-    float4* aa = static_cast<float4*>(P.accum_albedo);
-    float4* an = static_cast<float4*>(P.accum_normal);
-    float4* ad = static_cast<float4*>(P.accum_depth);
     const uint32_t chunk = spp < kAccRow ? spp : kAccRow;      // samples per pixel and round
     const uint32_t stride = chunk + 1u;                          // padded row
     uint32_t tile = kAccEntries / stride;                        // pixels per tile
-    if (tile > blockDim.x) tile = blockDim.x;
+    if (tile > kAccTile) tile = kAccTile;
+    const uint32_t role = threadIdx.x / kAccTile;                // 0 radiance, 1 albedo + depth, 2 normal
+    const uint32_t j = threadIdx.x % kAccTile;                   // the lane's pixel in the tile
+    const float4* src = role == 0 ? s_rad : (role == 1 ? s_a0 : s_a1);
     for (uint32_t t0 = blockIdx.x * tile; t0 < P.total_pixels; t0 += gridDim.x * tile) {
         const uint32_t np = P.total_pixels - t0 < tile ? P.total_pixels - t0 : tile;
-        const bool mine = threadIdx.x < np;
-        const uint32_t gp = t0 + threadIdx.x;
-        float4 sb = {0, 0, 0, 0}, sa = {0, 0, 0, 0}, sn = {0, 0, 0, 0}, sd = {0, 0, 0, 0};
+        const bool mine = role < 3u && j < np;
+        const uint32_t gp = t0 + j;
+        float4 sum = {0, 0, 0, 0}, sd = {0, 0, 0, 0};
         for (uint32_t c0 = 0; c0 < spp; c0 += chunk) {
             const uint32_t cn = spp - c0 < chunk ? spp - c0 : chunk;
             __syncthreads();   // the previous round's rows are read
             for (uint32_t e = threadIdx.x; e < np * cn; e += blockDim.x) {
-                const uint32_t j = e / cn, k = e - j * cn;
-                const size_t p = size_t(t0 + j) * spp + c0 + k;
-                s_rad[j * stride + k] = rad[p];
-                s_a0[j * stride + k] = a0[p];
-                s_a1[j * stride + k] = a1[p];
+                const uint32_t jj = e / cn, k = e - jj * cn;
+                const size_t p = size_t(t0 + jj) * spp + c0 + k;
+                s_rad[jj * stride + k] = rad[p];
+                s_a0[jj * stride + k] = a0[p];
+                s_a1[jj * stride + k] = a1[p];
             }
             __syncthreads();
             if (mine) {
                 for (uint32_t k = 0; k < cn; ++k) {
-                    const uint32_t e = threadIdx.x * stride + k;
-                    const float4 l = s_rad[e], x0 = s_a0[e], x1 = s_a1[e];
-                    const float dn = fminf(fmaxf((x0.w - 1.0f) / 4.0f, 0.0f), 1.0f);   // plugin.mm:301
-                    sb.x += l.x; sb.y += l.y; sb.z += l.z; sb.w += 1.0f;
-                    sa.x += x0.x; sa.y += x0.y; sa.z += x0.z; sa.w += 1.0f;
-                    sn.x += x1.x; sn.y += x1.y; sn.z += x1.z; sn.w += 1.0f;
-                    sd.x += dn; sd.y += dn; sd.z += dn; sd.w += 1.0f;
-                    if (P.record_ids && s0 + c0 + k == P.id_sample) P.ids[gp] = __float_as_uint(x1.w);
+                    const float4 x = src[j * stride + k];
+                    sum.x += x.x; sum.y += x.y; sum.z += x.z; sum.w += 1.0f;
+                    if (role == 1u) {
+                        const float dn = fminf(fmaxf((x.w - 1.0f) / 4.0f, 0.0f), 1.0f);   // plugin.mm:301
+                        sd.x += dn; sd.y += dn; sd.z += dn; sd.w += 1.0f;
+                    } else if (role == 2u && P.record_ids && s0 + c0 + k == P.id_sample) {
+                        P.ids[gp] = __float_as_uint(x.w);
+                    }
                 }
             }
         }
         if (mine) {
-            float4 o;
-            o = ab[gp]; o.x += sb.x; o.y += sb.y; o.z += sb.z; o.w += sb.w; ab[gp] = o;
-            o = aa[gp]; o.x += sa.x; o.y += sa.y; o.z += sa.z; o.w += sa.w; aa[gp] = o;
-            o = an[gp]; o.x += sn.x; o.y += sn.y; o.z += sn.z; o.w += sn.w; an[gp] = o;
-            o = ad[gp]; o.x += sd.x; o.y += sd.y; o.z += sd.z; o.w += sd.w; ad[gp] = o;
+            float4* acc = static_cast<float4*>(role == 0 ? P.accum_beauty : (role == 1 ? P.accum_albedo : P.accum_normal));
+            float4 o = acc[gp];
+            o.x += sum.x; o.y += sum.y; o.z += sum.z; o.w += sum.w;
+            acc[gp] = o;
+            if (role == 1u) {
+                float4* ad = static_cast<float4*>(P.accum_depth);
+                o = ad[gp]; o.x += sd.x; o.y += sd.y; o.z += sd.z; o.w += sd.w; ad[gp] = o;
+            }
         }
     }
 }
@@ -916,9 +922,9 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
 }
 
 hipError_t launch_pass_accumulate(const PassParams& p, const WaveState& s, hipStream_t stream) {
-    // tiles of up to 256 pixels (fewer when a pixel's staged row is longer)
+    // tiles of up to 64 pixels (fewer when a pixel's staged row is longer)
     const uint32_t chunk = p.sample_count < kAccRow ? p.sample_count : kAccRow;
-    const uint32_t tile = std::min<uint32_t>(256u, kAccEntries / (chunk + 1u));
+    const uint32_t tile = std::min<uint32_t>(kAccTile, kAccEntries / (chunk + 1u));
     const uint32_t acc_blocks = grid_for((uint64_t(p.total_pixels) + tile - 1) / tile, 1, 16384);
     if (p.num_scenes > 0) hipLaunchKernelGGL(k_accumulate<true>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
     else hipLaunchKernelGGL(k_accumulate<false>, dim3(acc_blocks), dim3(256), 0, stream, p, s);
