@@ -108,9 +108,10 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PassParams P, WaveState S, uin
 // after bounce 0 neither reads nor writes the path's radiance, which only the
 // shadow rays change from then on (15.2 -> 11.3 ms of shade per 64-spp frame
 // on the 1M soup at 1080p).
-// At least 5 waves per SIMD: 96 VGPRs without spills instead of the 100 the
-// compiler picks on its own (4 waves), more gathers in flight (15.9 -> 15.3 ms
-// of shade per 64-spp frame on the 1M soup at 1080p).
+// At least 5 waves per SIMD: 96 VGPRs instead of the 100 the compiler picks
+// on its own (4 waves), more gathers in flight (15.9 -> 15.3 ms of shade per
+// 64-spp frame on the 1M soup at 1080p).  The single-scene bounce-0
+// instantiation spills 8 bytes there; at 4 waves (no spill) it was slower.
 template <int BLOCK, bool BATCH, bool FIRST>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_shade(PassParams P, WaveState S, uint32_t b) {
     // each wave serves one queue segment; the grid is a multiple of the
