@@ -861,35 +861,23 @@ TraceConfig trace_config_at(uint32_t i) {
     return TraceConfig{v.width, v.block, v.chunk == 0 ? kChunkFlat : v.chunk};
 }
 
-uint32_t persistent_blocks(int device, const TraceConfig& c) {
-    const VariantEntry* v = find_variant(c);
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
-    int per_cu = v ? v->occupancy() : 1;
-    if (per_cu <= 0) per_cu = 1;
-    // never more threads than the stack-overflow area is sized for
-    const int max_per_cu = int(kMaxThreadsPerCU / c.block);
-    if (per_cu > max_per_cu) per_cu = max_per_cu;
-    return uint32_t(cus * per_cu);
-}
-
 namespace {
 
-// Blocks of k_shade that the current device holds resident at once (its CU
-// count times the kernel's blocks per CU), cached per device.  The shade grid
-// is capped there: each wave then walks its segment in consecutive groups of
-// 64 slots beside the segment's other waves, so the rays it appends keep the
-// order of the queue it reads (a pixel's samples stay neighbours bounce after
-// bounce).  A grid of 8192 blocks, whose later waves start on groups far
+// Blocks of k_shade that device `dev` (the current device) holds resident at
+// once: its CU count times the kernel's blocks per CU, cached per device
+// (persistent_blocks, called when a plugin starts, fills the cache outside
+// any stream capture).  The shade grid is capped there: each wave then walks
+// its segment in consecutive groups of 64 slots beside the segment's other
+// waves, so the rays it appends keep the order of the queue it reads (a
+// pixel's samples stay neighbours bounce after bounce).  A grid of 8192 blocks, whose later waves start on groups far
 // from the first ones', scattered that order: on the 1M soup at 1080p, 64
 // spp, extend + shadow + shade per frame took 79.4-80.0 ms against 74.8-76.7
 // with the resident grid (same images); half or three quarters of the
 // resident grid left shade short of waves (14.5 and 12.7 ms against 10.9).
 constexpr int kShadeCacheDevices = 64;
-uint32_t shade_resident_blocks() {
+uint32_t shade_resident_blocks(int dev) {
     static std::atomic<uint32_t> cache[kShadeCacheDevices];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+    if (dev < 0) dev = 0;
     const bool cached = dev < kShadeCacheDevices;
     if (cached) {
         const uint32_t c = cache[dev].load(std::memory_order_relaxed);
@@ -904,6 +892,19 @@ uint32_t shade_resident_blocks() {
 }
 
 }  // namespace
+
+uint32_t persistent_blocks(int device, const TraceConfig& c) {
+    (void)shade_resident_blocks(device);
+    const VariantEntry* v = find_variant(c);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    int per_cu = v ? v->occupancy() : 1;
+    if (per_cu <= 0) per_cu = 1;
+    // never more threads than the stack-overflow area is sized for
+    const int max_per_cu = int(kMaxThreadsPerCU / c.block);
+    if (per_cu > max_per_cu) per_cu = max_per_cu;
+    return uint32_t(cus * per_cu);
+}
 
 hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const TraceConfig& cfg,
                                  uint32_t trace_blocks, hipStream_t stream, WaveTimers* tm) {
@@ -920,7 +921,9 @@ hipError_t launch_wavefront_pass(const PassParams& p, const WaveState& s, const 
     // at most the resident grid, a multiple of the queue segments in waves
     // (one segment per wave)
     constexpr uint32_t kShadeGrain = kQueueSegments / (kShadeBlock / 64);
-    const uint32_t shade_cap = std::max(kShadeGrain, shade_resident_blocks() / kShadeGrain * kShadeGrain);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const uint32_t shade_cap = std::max(kShadeGrain, shade_resident_blocks(dev) / kShadeGrain * kShadeGrain);
     const uint32_t shade_blocks = (grid_for(s.paths, kShadeBlock, shade_cap) + kShadeGrain - 1) / kShadeGrain * kShadeGrain;
     auto before = [&](uint32_t k) { if (tm && tm->before) tm->before(tm->user, k, stream); };
     auto after = [&](uint32_t k) { if (tm && tm->after) tm->after(tm->user, k, stream); };
